@@ -1,0 +1,396 @@
+// Device-side numerics shared by the EDGE kernels (gfx950, wave64).
+//
+// Interpolation follows util/taichi_utils.py and sim/hermite.py of the
+// reference, re-expressed for the GPU:
+//  * The quadratic B-spline (util/taichi_utils.py:230-319) is evaluated with
+//    its 3 non-zero taps per axis (27 per component) instead of the reference's
+//    64-tap loop; the per-tap weights are the reference's N_2 / dN_2 pieces,
+//    selected by the fractional position.
+//  * Every Hermite variant of sim/hermite.py is the tricubic Hermite of
+//    sim/hermite.py:515-607 with per-corner data (psi_c, T_c, dT_c, d2T_c); the
+//    garm-FD and FD-unrolled variants differ only in how dT/d2T are derived
+//    (see edge_tables.hip), so one evaluator reads a packed per-cell table.
+//  * dx is folded into the "g" basis functions, so the table holds the
+//    reference's un-scaled derivative data.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define EDGE_DEV __device__ __forceinline__
+
+namespace edge {
+
+// ---------------------------------------------------------------------------
+// Grid parameters passed by value to every kernel.
+// ---------------------------------------------------------------------------
+struct Grid {
+  int nx, ny, nz;   // cells
+  float dx;
+  // Clamp upper bounds dim-2-1e-6 rounded to f32 (util/taichi_utils.py:262-264):
+  // hi_c[a] for a cell-sized axis, hi_f[a] for the staggered (+1) axis.
+  float hi_c[3], hi_f[3];
+  // interp_1 clamp dim-1-1e-6 for cell fields (util/taichi_utils.py:179-181).
+  float hi1_c[3];
+};
+
+__host__ __device__ __forceinline__ int64_t cell_id(const Grid& g, int i, int j, int k) {
+  return ((int64_t)i * g.ny + j) * g.nz + k;
+}
+__host__ __device__ __forceinline__ int64_t ncell(const Grid& g) { return (int64_t)g.nx * g.ny * g.nz; }
+
+// Face grid `a` has dims n + e_a.
+EDGE_DEV void face_dims(const Grid& g, int a, int d[3]) {
+  d[0] = g.nx + (a == 0);
+  d[1] = g.ny + (a == 1);
+  d[2] = g.nz + (a == 2);
+}
+
+// ---------------------------------------------------------------------------
+// Quadratic B-spline, 3 taps per axis.
+// ---------------------------------------------------------------------------
+struct Axis3 {
+  int i0;        // first tap (>= 0); taps i0, i0+1, i0+2 clamped to dmax
+  float w[3];    // N_2 at the taps
+  float dw[3];   // dN_2 at the taps
+};
+
+// u = p/dx - BL (already shifted); hi = dim-2-eps; dmax = dim-1.
+// The reference's 64-tap loop (util/taichi_utils.py:304-317) visits taps
+// iu-1..iu+2 where iu = floor(clamped u); the weight of iu-1 vanishes when the
+// fraction is >= 0.5 and that of iu+2 when it is <= 0.5.
+EDGE_DEV Axis3 bspline_axis(float u, float hi) {
+  float s = fmaxf(1.0f, fminf(u, hi));
+  float iu = floorf(s);
+  float fr = s - iu;
+  Axis3 A;
+  int start = (int)iu - (fr < 0.5f ? 1 : 0);
+  A.i0 = start;
+  float x0 = s - (float)start;         // in [0.5, 1.5)
+  float x1 = x0 - 1.0f;                // in [-0.5, 0.5)
+  float a2 = 2.0f - x0;                // |x2|, in (0.5, 1.5]
+  A.w[0] = 0.5f * ((1.5f - x0) * (1.5f - x0));
+  A.w[1] = 0.75f - x1 * x1;
+  A.w[2] = 0.5f * ((1.5f - a2) * (1.5f - a2));
+  A.dw[0] = 0.5f * (2.0f * x0 - 3.0f);
+  A.dw[1] = -2.0f * x1;
+  A.dw[2] = -(0.5f * (2.0f * a2 - 3.0f));
+  return A;
+}
+
+// Value and gradient (in index units) of the quadratic B-spline of one field.
+// f: the field; dims (d0,d1,d2); A/B/C: axis weights.
+template <bool GRAD>
+EDGE_DEV float bspline_eval(const float* __restrict__ f, int d1, int d2, int m0, int m1, int m2,
+                            const Axis3& A, const Axis3& B, const Axis3& C, float g[3]) {
+  int kk[3];
+#pragma unroll
+  for (int t = 0; t < 3; ++t) kk[t] = min(C.i0 + t, m2);
+  float val = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int ii = min(A.i0 + a, m0);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      int jj = min(B.i0 + b, m1);
+      const float* row = f + ((int64_t)ii * d1 + jj) * d2;
+      float v0 = row[kk[0]], v1 = row[kk[1]], v2 = row[kk[2]];
+      float s = v0 * C.w[0] + v1 * C.w[1] + v2 * C.w[2];
+      float wab = A.w[a] * B.w[b];
+      val += s * wab;
+      if (GRAD) {
+        float sd = v0 * C.dw[0] + v1 * C.dw[1] + v2 * C.dw[2];
+        gx += s * (A.dw[a] * B.w[b]);
+        gy += s * (A.w[a] * B.dw[b]);
+        gz += sd * wab;
+      }
+    }
+  }
+  if (GRAD) {
+    g[0] = gx;
+    g[1] = gy;
+    g[2] = gz;
+  }
+  return val;
+}
+
+// MAC velocity (run_edge.py:193-195): u_x (nx+1,ny,nz), u_y (nx,ny+1,nz), u_z (nx,ny,nz+1).
+struct Mac {
+  const float* __restrict__ ux;
+  const float* __restrict__ uy;
+  const float* __restrict__ uz;
+};
+
+// util/taichi_utils.py:322-341: interp_u_MAC(_grad).  Component a samples its
+// face grid with BL = 0 on axis a and 0.5 elsewhere.  J rows = grad u_a (1/dx units).
+template <bool GRAD>
+EDGE_DEV void interp_u_mac(const Grid& g, const Mac& U, const float p[3], float out[3], float J[3][3]) {
+  float q0 = p[0] / g.dx, q1 = p[1] / g.dx, q2 = p[2] / g.dx;
+  // own (BL 0, staggered dims) and other (BL 0.5, cell dims) weights per axis
+  Axis3 ow0 = bspline_axis(q0, g.hi_f[0]), ot0 = bspline_axis(q0 - 0.5f, g.hi_c[0]);
+  Axis3 ow1 = bspline_axis(q1, g.hi_f[1]), ot1 = bspline_axis(q1 - 0.5f, g.hi_c[1]);
+  Axis3 ow2 = bspline_axis(q2, g.hi_f[2]), ot2 = bspline_axis(q2 - 0.5f, g.hi_c[2]);
+  float gr[3];
+  out[0] = bspline_eval<GRAD>(U.ux, g.ny, g.nz, g.nx, g.ny - 1, g.nz - 1, ow0, ot1, ot2, gr);
+  if (GRAD) { float inv = 1.0f / g.dx; J[0][0] = gr[0] * inv; J[0][1] = gr[1] * inv; J[0][2] = gr[2] * inv; }
+  out[1] = bspline_eval<GRAD>(U.uy, g.ny + 1, g.nz, g.nx - 1, g.ny, g.nz - 1, ot0, ow1, ot2, gr);
+  if (GRAD) { float inv = 1.0f / g.dx; J[1][0] = gr[0] * inv; J[1][1] = gr[1] * inv; J[1][2] = gr[2] * inv; }
+  out[2] = bspline_eval<GRAD>(U.uz, g.ny, g.nz + 1, g.nx - 1, g.ny - 1, g.nz, ot0, ot1, ow2, gr);
+  if (GRAD) { float inv = 1.0f / g.dx; J[2][0] = gr[0] * inv; J[2][1] = gr[1] * inv; J[2][2] = gr[2] * inv; }
+}
+
+// ---------------------------------------------------------------------------
+// Tricubic Hermite over a packed table.
+//
+// Table layout: for component c, cell q: 8 floats at tab[(c*N + q)*8]:
+//   {psi_c, T_c0, T_c1, T_c2, dT_c0 (yz), dT_c1 (xz), dT_c2 (xy), d2T_c (xyz)}
+// (sim/hermite.py:610-620 "Hermite" row layout; dT as in sim/hermite.py:500-510).
+// ---------------------------------------------------------------------------
+struct HAxis {
+  int i0;          // corner base (cells i0, i0+1)
+  float F[2], G[2], dF[2], dG[2];  // basis at the two corners; G, dG pre-scaled by dx
+};
+
+// sim/hermite.py:4-41 for x = frac (corner 0) and x = frac - 1 (corner 1);
+// the "g" functions carry the dx of value_1..3 (sim/hermite.py:500-510).
+EDGE_DEV HAxis hermite_axis(float u, float hi, float dx) {
+  float s = fmaxf(1.0f, fminf(u, hi));
+  float iu = floorf(s);
+  HAxis H;
+  H.i0 = (int)iu;
+  float a0 = s - iu;       // x >= 0 at corner 0
+  float a1 = 1.0f - a0;    // |x| at corner 1 (x < 0 there)
+  float a02 = a0 * a0, a12 = a1 * a1;
+  H.F[0] = 2.0f * (a02 * a0) - 3.0f * a02 + 1.0f;
+  H.F[1] = 2.0f * (a12 * a1) - 3.0f * a12 + 1.0f;
+  H.dF[0] = 6.0f * a02 - 6.0f * a0;
+  H.dF[1] = -(6.0f * a12 - 6.0f * a1);
+  H.G[0] = ((a02 * a0) - 2.0f * a02 + a0) * dx;
+  H.G[1] = -((a12 * a1) - 2.0f * a12 + a1) * dx;
+  H.dG[0] = (3.0f * a02 - 4.0f * a0 + 1.0f) * dx;
+  H.dG[1] = (3.0f * a12 - 4.0f * a1 + 1.0f) * dx;
+  return H;
+}
+
+struct HPoint {
+  HAxis X, Y, Z;
+};
+
+EDGE_DEV HPoint hermite_point(const Grid& g, const float p[3]) {
+  HPoint P;
+  P.X = hermite_axis(p[0] / g.dx - 0.5f, g.hi_c[0], g.dx);
+  P.Y = hermite_axis(p[1] / g.dx - 0.5f, g.hi_c[1], g.dx);
+  P.Z = hermite_axis(p[2] / g.dx - 0.5f, g.hi_c[2], g.dx);
+  return P;
+}
+
+// Value (and gradient in index units, i.e. before the final 1/dx) of one
+// component from its table.  Corner indices are always inside the grid: the
+// clamp keeps the base in [1, dim-2].
+template <bool GRAD>
+EDGE_DEV float hermite_eval(const Grid& g, const float4* __restrict__ tab, const HPoint& P, float gr[3]) {
+  float val = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        int64_t q = cell_id(g, P.X.i0 + i, P.Y.i0 + j, P.Z.i0 + k);
+        float4 lo = tab[2 * q], hi = tab[2 * q + 1];
+        // coefficient tensor c[a][b][cz] (a,b,cz in {f=0, g=1})
+        float cfff = lo.x, cgff = lo.y, cfgf = lo.z, cffg = lo.w;
+        float cfgg = hi.x, cgfg = hi.y, cggf = hi.z, cggg = hi.w;
+        float Zf = P.Z.F[k], Zg = P.Z.G[k];
+        // contract z
+        float sff = cfff * Zf + cffg * Zg;
+        float sfg = cfgf * Zf + cfgg * Zg;
+        float sgf = cgff * Zf + cgfg * Zg;
+        float sgg = cggf * Zf + cggg * Zg;
+        float Yf = P.Y.F[j], Yg = P.Y.G[j];
+        float tf = sff * Yf + sfg * Yg;
+        float tg = sgf * Yf + sgg * Yg;
+        float Xf = P.X.F[i], Xg = P.X.G[i];
+        val += tf * Xf + tg * Xg;
+        if (GRAD) {
+          float dZf = P.Z.dF[k], dZg = P.Z.dG[k];
+          float dff = cfff * dZf + cffg * dZg;
+          float dfg = cfgf * dZf + cfgg * dZg;
+          float dgf = cgff * dZf + cgfg * dZg;
+          float dgg = cggf * dZf + cggg * dZg;
+          float dYf = P.Y.dF[j], dYg = P.Y.dG[j];
+          float ty_f = sff * dYf + sfg * dYg;
+          float ty_g = sgf * dYf + sgg * dYg;
+          float tz_f = dff * Yf + dfg * Yg;
+          float tz_g = dgf * Yf + dgg * Yg;
+          gx += tf * P.X.dF[i] + tg * P.X.dG[i];
+          gy += ty_f * Xf + ty_g * Xg;
+          gz += tz_f * Xf + tz_g * Xg;
+        }
+      }
+    }
+  }
+  if (GRAD) {
+    gr[0] = gx;
+    gr[1] = gy;
+    gr[2] = gz;
+  }
+  return val;
+}
+
+// All three components of the map at p: value v and rows G (grad psi_c, 1/dx units).
+template <bool GRAD>
+EDGE_DEV void map_eval(const Grid& g, const float4* __restrict__ tab, const float p[3], float v[3], float G[3][3]) {
+  HPoint P = hermite_point(g, p);
+  int64_t N = ncell(g);
+  float inv = 1.0f / g.dx;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float gr[3];
+    v[c] = hermite_eval<GRAD>(g, tab + 2 * N * c, P, gr);
+    if (GRAD) {
+      G[c][0] = gr[0] * inv;
+      G[c][1] = gr[1] * inv;
+      G[c][2] = gr[2] * inv;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RK4 marches
+// ---------------------------------------------------------------------------
+// sim/hermite.py:806-860: RK4 of a point psi and its Jacobian.  On entry and
+// exit T holds rows (grad psi_c); the columns C[b] = d psi / d x_b evolve as
+// dC/dt = J(psi) C.
+EDGE_DEV void rk4_psi_T(const Grid& g, const Mac& U, float psi[3], float T[3][3], float dt) {
+  float C[3][3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) C[b][a] = T[a][b];
+  float ps[3] = {psi[0], psi[1], psi[2]};
+  float Cs[3][3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) Cs[b][a] = C[b][a];
+  float su[3] = {0.f, 0.f, 0.f}, sC[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+  const float half = 0.5f * dt;
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    float u[3], J[3][3];
+    interp_u_mac<true>(g, U, ps, u, J);
+    float dC[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) dC[b][a] = J[a][0] * Cs[b][0] + J[a][1] * Cs[b][1] + J[a][2] * Cs[b][2];
+    float wgt = (st == 0 || st == 3) ? 1.0f : 2.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) su[a] += wgt * u[a];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) sC[b][a] += wgt * dC[b][a];
+    if (st < 3) {
+      float cst = (st < 2) ? half : dt;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) ps[a] = psi[a] + cst * u[a];
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) Cs[b][a] = C[b][a] + cst * dC[b][a];
+    }
+  }
+  float h = dt / 6.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) psi[a] = psi[a] + h * su[a];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) T[a][b] = C[b][a] + h * sC[b][a];
+}
+
+// sim/hermite.py:786-803: RK4 of a point only.
+EDGE_DEV void rk4_psi(const Grid& g, const Mac& U, float psi[3], float dt) {
+  float ps[3] = {psi[0], psi[1], psi[2]};
+  float su[3] = {0.f, 0.f, 0.f};
+  const float half = 0.5f * dt;
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    float u[3];
+    interp_u_mac<false>(g, U, ps, u, nullptr);
+    float wgt = (st == 0 || st == 3) ? 1.0f : 2.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) su[a] += wgt * u[a];
+    if (st < 3) {
+      float cst = (st < 2) ? half : dt;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) ps[a] = psi[a] + cst * u[a];
+    }
+  }
+  float h = dt / 6.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) psi[a] = psi[a] + h * su[a];
+}
+
+// run_edge.py:336-357: RK4 of a face point and one column F.
+EDGE_DEV void rk4_phi_F(const Grid& g, const Mac& U, float phi[3], float F[3], float dt) {
+  float ps[3] = {phi[0], phi[1], phi[2]}, Fs[3] = {F[0], F[1], F[2]};
+  float su[3] = {0.f, 0.f, 0.f}, sF[3] = {0.f, 0.f, 0.f};
+  const float half = 0.5f * dt;
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    float u[3], J[3][3];
+    interp_u_mac<true>(g, U, ps, u, J);
+    float dF[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dF[a] = J[a][0] * Fs[0] + J[a][1] * Fs[1] + J[a][2] * Fs[2];
+    float wgt = (st == 0 || st == 3) ? 1.0f : 2.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      su[a] += wgt * u[a];
+      sF[a] += wgt * dF[a];
+    }
+    if (st < 3) {
+      float cst = (st < 2) ? half : dt;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        ps[a] = phi[a] + cst * u[a];
+        Fs[a] = F[a] + cst * dF[a];
+      }
+    }
+  }
+  float h = dt / 6.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    phi[a] = phi[a] + h * su[a];
+    F[a] = F[a] + h * sF[a];
+  }
+}
+
+// Compose a marched point (psi, rows T) with the old map in `tab`
+// (sim/hermite.py:959-980): psi <- psi_old(psi), T <- grad psi_old(psi) . T.
+EDGE_DEV void compose(const Grid& g, const float4* __restrict__ tab, float psi[3], float T[3][3]) {
+  float v[3], G[3][3];
+  map_eval<true>(g, tab, psi, v, G);
+  float Tn[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) Tn[a][b] = G[a][0] * T[0][b] + G[a][1] * T[1][b] + G[a][2] * T[2][b];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    psi[a] = v[a];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) T[a][b] = Tn[a][b];
+  }
+}
+
+// Face coordinate (util/taichi_utils.py:29-44): axis a integral, others +0.5.
+EDGE_DEV void face_coord(int a, int i, int j, int k, float dx, float X[3]) {
+  X[0] = (a == 0 ? (float)i : (float)i + 0.5f) * dx;
+  X[1] = (a == 1 ? (float)j : (float)j + 0.5f) * dx;
+  X[2] = (a == 2 ? (float)k : (float)k + 0.5f) * dx;
+}
+
+}  // namespace edge

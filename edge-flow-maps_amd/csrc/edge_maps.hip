@@ -1,0 +1,550 @@
+// Step-loop driver kernels of run_edge.py / run_ed8.py, the semi-Lagrangian
+// velocity advection of sim/hermite.py, the field arithmetic of
+// util/taichi_utils.py and the vortex-ring initial condition of
+// sim/init_conditions.py.  Face kernels cover the three face grids in one
+// launch (run_edge.py's three `for I in ti.grouped(u_*1)` loops).
+#include "edge_common.h"
+
+using namespace edge;
+
+namespace {
+
+struct Faces {
+  float* f[3];
+};
+struct CFaces {
+  const float* f[3];
+};
+
+// Decode a flat index over the concatenated face grids x, y, z.
+__device__ __forceinline__ bool face_of(const Grid& g, int64_t t, int& a, int64_t& local, int64_t& nf, int& i, int& j,
+                                        int& k) {
+  const int64_t n0 = (int64_t)(g.nx + 1) * g.ny * g.nz;
+  const int64_t n1 = (int64_t)g.nx * (g.ny + 1) * g.nz;
+  const int64_t n2 = (int64_t)g.nx * g.ny * (g.nz + 1);
+  if (t < n0) { a = 0; nf = n0; }
+  else if (t < n0 + n1) { a = 1; t -= n0; nf = n1; }
+  else if (t < n0 + n1 + n2) { a = 2; t -= n0 + n1; nf = n2; }
+  else return false;
+  local = t;
+  int d[3];
+  face_dims(g, a, d);
+  k = (int)(t % d[2]);
+  j = (int)((t / d[2]) % d[1]);
+  i = (int)(t / ((int64_t)d[1] * d[2]));
+  return true;
+}
+
+__device__ __forceinline__ int64_t nfaces(const Grid& g) {
+  return (int64_t)(g.nx + 1) * g.ny * g.nz + (int64_t)g.nx * (g.ny + 1) * g.nz + (int64_t)g.nx * g.ny * (g.nz + 1);
+}
+
+// run_edge.py:360-373 map_u: u1_a = F_a . interp_u_MAC(u0, phi_a).
+__global__ void __launch_bounds__(256) k_map_u(Grid g, Mac U0, Faces U1, CFaces F, CFaces PHI) {
+  int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  int a, i, j, k;
+  int64_t l, nf;
+  if (!face_of(g, t, a, l, nf, i, j, k)) return;
+  const float* ph = PHI.f[a];
+  const float* Fa = F.f[a];
+  float p[3] = {ph[l], ph[nf + l], ph[2 * nf + l]};
+  float v[3];
+  interp_u_mac<false>(g, U0, p, v, nullptr);
+  U1.f[a][l] = Fa[l] * v[0] + Fa[nf + l] * v[1] + Fa[2 * nf + l] * v[2];
+}
+
+// run_edge.py:376-417 map_u_psi: u1_a(X_a) = sum_c d psi_c/d x_a (X_a) u0_c(psi(X_a)).
+__global__ void __launch_bounds__(256) k_map_u_psi(Grid g, Mac U0, Faces U1, const float4* __restrict__ tab) {
+  int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  int a, i, j, k;
+  int64_t l, nf;
+  if (!face_of(g, t, a, l, nf, i, j, k)) return;
+  float X[3], psi[3], G[3][3], v[3];
+  face_coord(a, i, j, k, g.dx, X);
+  map_eval<true>(g, tab, X, psi, G);
+  interp_u_mac<false>(g, U0, psi, v, nullptr);
+  U1.f[a][l] = G[0][a] * v[0] + G[1][a] * v[1] + G[2][a] * v[2];
+}
+
+// run_ed8.py:190-223 split_psi_hermite: phi_a = psi(X_a), F_a = column a of grad psi.
+__global__ void __launch_bounds__(256) k_split_psi(Grid g, const float4* __restrict__ tab, Faces PHI, Faces F) {
+  int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  int a, i, j, k;
+  int64_t l, nf;
+  if (!face_of(g, t, a, l, nf, i, j, k)) return;
+  float X[3], psi[3], G[3][3];
+  face_coord(a, i, j, k, g.dx, X);
+  map_eval<true>(g, tab, X, psi, G);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    PHI.f[a][c * nf + l] = psi[c];
+    F.f[a][c * nf + l] = G[c][a];
+  }
+}
+
+// util/taichi_utils.py:171-227: trilinear value or 8-tap min/max of a
+// cell-centred field with nc <= 4 channels, BL 0.5.
+template <int NC>
+__device__ __forceinline__ void linear_taps(const Grid& g, const float* __restrict__ s0, const float p[3], int mode,
+                                            float out[NC], float mn[NC], float mx[NC]) {
+  int64_t N = ncell(g);
+  float ss[3];
+  int i0[3], i1[3];
+  float fr[3];
+  const int dims[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float u = p[d] / g.dx - 0.5f;
+    ss[d] = fmaxf(1.0f, fminf(u, g.hi1_c[d]));
+    float fl = floorf(ss[d]);
+    fr[d] = ss[d] - fl;
+    i0[d] = min((int)fl, dims[d] - 1);
+    i1[d] = min((int)fl + 1, dims[d] - 1);
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[c] = 0.f;
+  bool first = true;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int cz = 0; cz < 2; ++cz) {
+        int ii = a ? i1[0] : i0[0], jj = b ? i1[1] : i0[1], kk = cz ? i1[2] : i0[2];
+        // N_1(x) = 1 - |x| with x = s - (iu + a)
+        float wx = a ? fr[0] : 1.0f - fr[0];
+        float wy = b ? fr[1] : 1.0f - fr[1];
+        float wz = cz ? fr[2] : 1.0f - fr[2];
+        int64_t q = cell_id(g, ii, jj, kk);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          float v = s0[c * N + q];
+          if (mode == 0) {
+            out[c] += v * wx * wy * wz;
+          } else {
+            mn[c] = first ? v : fminf(mn[c], v);
+            mx[c] = first ? v : fmaxf(mx[c], v);
+          }
+        }
+        first = false;
+      }
+}
+
+// run_edge.py:420-427 map_smoke (mode 0) and 469-477 clamp_smoke (mode 1):
+// psi_c = mean of the six face points of phi.
+template <int NC>
+__global__ void __launch_bounds__(256) k_map_smoke(Grid g, int mode, const float* __restrict__ s0, float* __restrict__ s1,
+                                                   CFaces PHI) {
+  const int64_t N = ncell(g);
+  const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  int k = (int)(q % g.nz), j = (int)((q / g.nz) % g.ny), i = (int)(q / ((int64_t)g.ny * g.nz));
+  const int64_t n0 = (int64_t)(g.nx + 1) * g.ny * g.nz, n1 = (int64_t)g.nx * (g.ny + 1) * g.nz,
+                n2 = (int64_t)g.nx * g.ny * (g.nz + 1);
+  int64_t fx0 = ((int64_t)i * g.ny + j) * g.nz + k, fx1 = fx0 + (int64_t)g.ny * g.nz;
+  int64_t fy0 = ((int64_t)i * (g.ny + 1) + j) * g.nz + k, fy1 = fy0 + g.nz;
+  int64_t fz0 = ((int64_t)i * g.ny + j) * (g.nz + 1) + k, fz1 = fz0 + 1;
+  float p[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float s = PHI.f[0][c * n0 + fx0] + PHI.f[0][c * n0 + fx1] + PHI.f[1][c * n1 + fy0] + PHI.f[1][c * n1 + fy1] +
+              PHI.f[2][c * n2 + fz0] + PHI.f[2][c * n2 + fz1];
+    p[c] = (1.0f / 6.0f) * s;
+  }
+  float out[NC], mn[NC], mx[NC];
+  linear_taps<NC>(g, s0, p, mode, out, mn, mx);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (mode == 0) s1[c * N + q] = out[c];
+    else s1[c * N + q] = fminf(fmaxf(s1[c * N + q], mn[c]), mx[c]);
+  }
+}
+
+// run_edge.py:443-466 map_smoke_psi (mode 0) and 480-505 clamp_smoke_psi
+// (mode 1): psi_c = mean of the map's values at the six face centres.
+template <int NC>
+__global__ void __launch_bounds__(256) k_map_smoke_psi(Grid g, int mode, const float* __restrict__ s0,
+                                                       float* __restrict__ s1, const float4* __restrict__ tab) {
+  const int64_t N = ncell(g);
+  const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  int k = (int)(q % g.nz), j = (int)((q / g.nz) % g.ny), i = (int)(q / ((int64_t)g.ny * g.nz));
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int f = 0; f < 6; ++f) {
+    int a = f >> 1, up = f & 1;
+    float X[3], v[3];
+    face_coord(a, i + (a == 0 ? up : 0), j + (a == 1 ? up : 0), k + (a == 2 ? up : 0), g.dx, X);
+    map_eval<false>(g, tab, X, v, nullptr);
+    acc[0] += v[0];
+    acc[1] += v[1];
+    acc[2] += v[2];
+  }
+  float p[3] = {(1.0f / 6.0f) * acc[0], (1.0f / 6.0f) * acc[1], (1.0f / 6.0f) * acc[2]};
+  float out[NC], mn[NC], mx[NC];
+  linear_taps<NC>(g, s0, p, mode, out, mn, mx);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (mode == 0) s1[c * N + q] = out[c];
+    else s1[c * N + q] = fminf(fmaxf(s1[c * N + q], mn[c]), mx[c]);
+  }
+}
+
+// run_edge.py:508-519 clamp_u over all three face grids: u2 = clamp(u, min, max
+// of the six neighbours in u1).
+__global__ void __launch_bounds__(256) k_clamp_u(Grid g, CFaces U, CFaces U1, Faces U2) {
+  int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  int a, i, j, k;
+  int64_t l, nf;
+  if (!face_of(g, t, a, l, nf, i, j, k)) return;
+  int d[3];
+  face_dims(g, a, d);
+  const float* u1 = U1.f[a];
+  auto at = [&](int ii, int jj, int kk) {
+    ii = max(0, min(ii, d[0] - 1));
+    jj = max(0, min(jj, d[1] - 1));
+    kk = max(0, min(kk, d[2] - 1));
+    return u1[((int64_t)ii * d[1] + jj) * d[2] + kk];
+  };
+  float l_ = at(i - 1, j, k), r_ = at(i + 1, j, k), b_ = at(i, j - 1, k), t_ = at(i, j + 1, k),
+        a_ = at(i, j, k - 1), c_ = at(i, j, k + 1);
+  float mx = fmaxf(fmaxf(fmaxf(l_, r_), fmaxf(b_, t_)), fmaxf(a_, c_));
+  float mn = fminf(fminf(fminf(l_, r_), fminf(b_, t_)), fminf(a_, c_));
+  U2.f[a][l] = fminf(fmaxf(U.f[a][l], mn), mx);
+}
+
+// sim/hermite.py:1578-1621 advect_u_semi_lagrangian: RK4 backtrace from the face
+// centre, then component a of interp_u_MAC at the foot.
+__global__ void __launch_bounds__(256) k_advect_sl(Grid g, Mac U0, Faces U1, float dt) {
+  int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  int a, i, j, k;
+  int64_t l, nf;
+  if (!face_of(g, t, a, l, nf, i, j, k)) return;
+  float p1[3], q[3], v1[3], v2[3], v3[3], v4[3], v5[3];
+  face_coord(a, i, j, k, g.dx, p1);
+  interp_u_mac<false>(g, U0, p1, v1, nullptr);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) q[d] = p1[d] - v1[d] * dt * 0.5f;
+  interp_u_mac<false>(g, U0, q, v2, nullptr);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) q[d] = p1[d] - v2[d] * dt * 0.5f;
+  interp_u_mac<false>(g, U0, q, v3, nullptr);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) q[d] = p1[d] - v3[d] * dt;
+  interp_u_mac<false>(g, U0, q, v4, nullptr);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) q[d] = p1[d] - (v1[d] + 2.0f * v2[d] + 2.0f * v3[d] + v4[d]) / 6.0f * dt;
+  interp_u_mac<false>(g, U0, q, v5, nullptr);
+  U1.f[a][l] = v5[a];
+}
+
+// run_edge.py:295-303 calc_max_speed: max over cells of the centred speed, floor 1e-3.
+// out holds the float bits; positive floats order like their int bits.
+__global__ void __launch_bounds__(256) k_max_speed(Grid g, Mac U, unsigned int* __restrict__ out) {
+  const int64_t N = ncell(g);
+  float m = 1.e-3f;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    int k = (int)(q % g.nz), j = (int)((q / g.nz) % g.ny), i = (int)(q / ((int64_t)g.ny * g.nz));
+    float u = 0.5f * (U.ux[q] + U.ux[q + (int64_t)g.ny * g.nz]);
+    int64_t qy = ((int64_t)i * (g.ny + 1) + j) * g.nz + k;
+    float v = 0.5f * (U.uy[qy] + U.uy[qy + g.nz]);
+    int64_t qz = ((int64_t)i * g.ny + j) * (g.nz + 1) + k;
+    float w = 0.5f * (U.uz[qz] + U.uz[qz + 1]);
+    m = fmaxf(m, sqrtf(u * u + v * v + w * w));
+  }
+  // wave64 reduction, then one atomic per wave
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+__global__ void k_set_u32(unsigned int* p, unsigned int v) { *p = v; }
+
+// util/taichi_utils.py:12-15 add_fields (dest = f1 + m f2) and 6-9 scale_field.
+__global__ void __launch_bounds__(256) k_axpy(const float* __restrict__ f1, const float* __restrict__ f2,
+                                              float* __restrict__ dest, float m, int64_t n) {
+  int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (t + 3 < n) {
+    float4 a = *(const float4*)(f1 + t), b = *(const float4*)(f2 + t);
+    float4 r = {a.x + m * b.x, a.y + m * b.y, a.z + m * b.z, a.w + m * b.w};
+    *(float4*)(dest + t) = r;
+  } else {
+    for (; t < n; ++t) dest[t] = f1[t] + m * f2[t];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_scale(const float* __restrict__ a, float alpha, float* __restrict__ r,
+                                               int64_t n) {
+  int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (t + 3 < n) {
+    float4 v = *(const float4*)(a + t);
+    float4 o = {alpha * v.x, alpha * v.y, alpha * v.z, alpha * v.w};
+    *(float4*)(r + t) = o;
+  } else {
+    for (; t < n; ++t) r[t] = alpha * a[t];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill(float* __restrict__ p, float v, int64_t n) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+// sim/init_conditions.py:13-36 add_vortex_ring_and_smoke (pf = cell centres).
+__global__ void __launch_bounds__(256) k_vortex_ring(Grid g, float w, float rad, float delta, float3 c, float3 ex,
+                                                     float3 ey, const float* __restrict__ pf, float* __restrict__ vf,
+                                                     float* __restrict__ smokef, float3 color, int num_samples) {
+  const int64_t N = ncell(g);
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  const float pi = 3.14159265358979323846f;
+  float curve_length = (2.0f * pi * rad) / (float)num_samples;
+  float P[3] = {pf[q], pf[N + q], pf[2 * N + q]};
+  float acc[3] = {0.f, 0.f, 0.f}, sm = 0.f;
+  for (int l = 0; l < num_samples; ++l) {
+    float theta = (float)l / (float)num_samples * 2.0f * pi;
+    float ct = cosf(theta), st = sinf(theta);
+    float ps[3] = {rad * (ct * ex.x + st * ey.x) + c.x, rad * (ct * ex.y + st * ey.y) + c.y,
+                   rad * (ct * ex.z + st * ey.z) + c.z};
+    float pd[3] = {P[0] - ps[0], P[1] - ps[1], P[2] - ps[2]};
+    float wv[3] = {w * (-st * ex.x + ct * ey.x), w * (-st * ex.y + ct * ey.y), w * (-st * ex.z + ct * ey.z)};
+    float r = sqrtf(pd[0] * pd[0] + pd[1] * pd[1] + pd[2] * pd[2]);
+    float rd = r / delta;
+    float e = expf(-(rd * rd * rd));
+    float fac = curve_length * (-1.0f / (4.0f * pi * (r * r * r)) * (1.0f - e));
+    acc[0] += fac * (pd[1] * wv[2] - pd[2] * wv[1]);
+    acc[1] += fac * (pd[2] * wv[0] - pd[0] * wv[2]);
+    acc[2] += fac * (pd[0] * wv[1] - pd[1] * wv[0]);
+    sm += curve_length * e;
+  }
+  vf[q] += acc[0];
+  vf[N + q] += acc[1];
+  vf[2 * N + q] += acc[2];
+  float s3 = smokef[3 * N + q] + sm;
+  if (s3 > 0.002f) {
+    smokef[q] = color.x;
+    smokef[N + q] = color.y;
+    smokef[2 * N + q] = color.z;
+    smokef[3 * N + q] = 1.0f;
+  } else {
+    smokef[q] = 0.f;
+    smokef[N + q] = 0.f;
+    smokef[2 * N + q] = 0.f;
+    smokef[3 * N + q] = 0.f;
+  }
+}
+
+// util/taichi_utils.py:103-116 split_central_vector: cell vector -> MAC faces.
+__global__ void __launch_bounds__(256) k_split_central(Grid g, const float* __restrict__ vc, Faces U) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int a, i, j, k;
+  int64_t l, nf;
+  if (!face_of(g, t, a, l, nf, i, j, k)) return;
+  const int64_t N = ncell(g);
+  auto at = [&](int ii, int jj, int kk) {
+    ii = max(0, min(ii, g.nx - 1));
+    jj = max(0, min(jj, g.ny - 1));
+    kk = max(0, min(kk, g.nz - 1));
+    return vc[(int64_t)a * N + cell_id(g, ii, jj, kk)];
+  };
+  float r = at(i, j, k);
+  float lft = at(i - (a == 0), j - (a == 1), k - (a == 2));
+  U.f[a][l] = 0.5f * (r + lft);
+}
+
+}  // namespace
+
+extern "C" {
+
+int edge_map_u(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+               const float* Fx, const float* Fy, const float* Fz, const float* phix, const float* phiy,
+               const float* phiz, int nx, int ny, int nz, float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_map_u: bad dims");
+  EDGE_REQUIRE(u0x && u0y && u0z && u1x && u1y && u1z && Fx && Fy && Fz && phix && phiy && phiz,
+               "edge_map_u: null pointer");
+  Grid g = make_grid(nx, ny, nz, dx);
+  hipLaunchKernelGGL(k_map_u, dim3(blocks_for(
+                                  (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1),
+                                  256)),
+                     dim3(256), 0, (hipStream_t)stream, g, Mac{u0x, u0y, u0z}, Faces{{u1x, u1y, u1z}},
+                     CFaces{{Fx, Fy, Fz}}, CFaces{{phix, phiy, phiz}});
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_map_u_psi(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+                   const float* tab, int nx, int ny, int nz, float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_map_u_psi: bad dims");
+  EDGE_REQUIRE(u0x && u0y && u0z && u1x && u1y && u1z && tab, "edge_map_u_psi: null pointer");
+  Grid g = make_grid(nx, ny, nz, dx);
+  int64_t nf = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
+  hipLaunchKernelGGL(k_map_u_psi, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     Mac{u0x, u0y, u0z}, Faces{{u1x, u1y, u1z}}, (const float4*)tab);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_split_psi_hermite(const float* tab, float* phix, float* phiy, float* phiz, float* Fx, float* Fy, float* Fz,
+                           int nx, int ny, int nz, float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_split_psi_hermite: bad dims");
+  EDGE_REQUIRE(tab && phix && phiy && phiz && Fx && Fy && Fz, "edge_split_psi_hermite: null pointer");
+  Grid g = make_grid(nx, ny, nz, dx);
+  int64_t nf = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
+  hipLaunchKernelGGL(k_split_psi, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     (const float4*)tab, Faces{{phix, phiy, phiz}}, Faces{{Fx, Fy, Fz}});
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+// mode 0: map_smoke, 1: clamp_smoke (smoke1 updated in place).  nc in 1..4.
+int edge_map_smoke(int mode, const float* s0, float* s1, int nc, const float* phix, const float* phiy,
+                   const float* phiz, int nx, int ny, int nz, float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_map_smoke: bad dims");
+  EDGE_REQUIRE(s0 && s1 && phix && phiy && phiz && (mode == 0 || mode == 1), "edge_map_smoke: bad arguments");
+  EDGE_REQUIRE(nc == 4 || nc == 1, "edge_map_smoke: nc must be 1 or 4 (got %d)", nc);
+  Grid g = make_grid(nx, ny, nz, dx);
+  dim3 grid(blocks_for(ncell(g), 256));
+  if (nc == 4)
+    hipLaunchKernelGGL(k_map_smoke<4>, grid, dim3(256), 0, (hipStream_t)stream, g, mode, s0, s1,
+                       CFaces{{phix, phiy, phiz}});
+  else
+    hipLaunchKernelGGL(k_map_smoke<1>, grid, dim3(256), 0, (hipStream_t)stream, g, mode, s0, s1,
+                       CFaces{{phix, phiy, phiz}});
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+// mode 0: map_smoke_psi, 1: clamp_smoke_psi.
+int edge_map_smoke_psi(int mode, const float* s0, float* s1, int nc, const float* tab, int nx, int ny, int nz,
+                       float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_map_smoke_psi: bad dims");
+  EDGE_REQUIRE(s0 && s1 && tab && (mode == 0 || mode == 1), "edge_map_smoke_psi: bad arguments");
+  EDGE_REQUIRE(nc == 4 || nc == 1, "edge_map_smoke_psi: nc must be 1 or 4 (got %d)", nc);
+  Grid g = make_grid(nx, ny, nz, dx);
+  dim3 grid(blocks_for(ncell(g), 256));
+  if (nc == 4)
+    hipLaunchKernelGGL(k_map_smoke_psi<4>, grid, dim3(256), 0, (hipStream_t)stream, g, mode, s0, s1,
+                       (const float4*)tab);
+  else
+    hipLaunchKernelGGL(k_map_smoke_psi<1>, grid, dim3(256), 0, (hipStream_t)stream, g, mode, s0, s1,
+                       (const float4*)tab);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_clamp_u(const float* ux, const float* uy, const float* uz, const float* u1x, const float* u1y,
+                 const float* u1z, float* u2x, float* u2y, float* u2z, int nx, int ny, int nz, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_clamp_u: bad dims");
+  EDGE_REQUIRE(ux && uy && uz && u1x && u1y && u1z && u2x && u2y && u2z, "edge_clamp_u: null pointer");
+  Grid g = make_grid(nx, ny, nz, 1.0f);
+  int64_t nf = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
+  hipLaunchKernelGGL(k_clamp_u, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     CFaces{{ux, uy, uz}}, CFaces{{u1x, u1y, u1z}}, Faces{{u2x, u2y, u2z}});
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_advect_u_sl(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+                     int nx, int ny, int nz, float dx, float dt, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_advect_u_sl: bad dims");
+  EDGE_REQUIRE(u0x && u0y && u0z && u1x && u1y && u1z, "edge_advect_u_sl: null pointer");
+  EDGE_REQUIRE(u0x != u1x && u0y != u1y && u0z != u1z, "edge_advect_u_sl: input and output must differ");
+  Grid g = make_grid(nx, ny, nz, dx);
+  int64_t nf = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
+  hipLaunchKernelGGL(k_advect_sl, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     Mac{u0x, u0y, u0z}, Faces{{u1x, u1y, u1z}}, dt);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+// max speed into a device float (out_dev); the caller reads it back.
+int edge_max_speed(const float* ux, const float* uy, const float* uz, float* out_dev, int nx, int ny, int nz,
+                   void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_max_speed: bad dims");
+  EDGE_REQUIRE(ux && uy && uz && out_dev, "edge_max_speed: null pointer");
+  Grid g = make_grid(nx, ny, nz, 1.0f);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, (hipStream_t)stream, (unsigned int*)out_dev, 0u);
+  EDGE_CHECK_LAUNCH();
+  int nb = std::min(blocks_for(ncell(g), 256), 4096);
+  hipLaunchKernelGGL(k_max_speed, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, Mac{ux, uy, uz},
+                     (unsigned int*)out_dev);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_axpy(const float* f1, const float* f2, float* dest, float m, long n, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(f1 && f2 && dest && n >= 0, "edge_axpy: bad arguments");
+  EDGE_REQUIRE((((uintptr_t)f1 | (uintptr_t)f2 | (uintptr_t)dest) & 15) == 0, "edge_axpy: pointers not 16-byte aligned");
+  if (n == 0) return EDGE_OK;
+  hipLaunchKernelGGL(k_axpy, dim3(blocks_for((n + 3) / 4, 256)), dim3(256), 0, (hipStream_t)stream, f1, f2, dest, m,
+                     (int64_t)n);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_scale(const float* a, float alpha, float* r, long n, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(a && r && n >= 0, "edge_scale: bad arguments");
+  EDGE_REQUIRE((((uintptr_t)a | (uintptr_t)r) & 15) == 0, "edge_scale: pointers not 16-byte aligned");
+  if (n == 0) return EDGE_OK;
+  hipLaunchKernelGGL(k_scale, dim3(blocks_for((n + 3) / 4, 256)), dim3(256), 0, (hipStream_t)stream, a, alpha, r,
+                     (int64_t)n);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_copy(const float* src, float* dst, long n, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(src && dst && n >= 0, "edge_copy: bad arguments");
+  if (n == 0 || src == dst) return EDGE_OK;
+  EDGE_CHECK_HIP(hipMemcpyAsync(dst, src, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return EDGE_OK;
+}
+
+int edge_fill(float* p, float v, long n, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(p && n >= 0, "edge_fill: bad arguments");
+  if (n == 0) return EDGE_OK;
+  hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, p, v, (int64_t)n);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_vortex_ring(float w, float rad, float delta, const float* c, const float* ex, const float* ey,
+                     const float* pf, float* vf, float* smokef, const float* color, int num_samples, int nx, int ny,
+                     int nz, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_vortex_ring: bad dims");
+  EDGE_REQUIRE(c && ex && ey && pf && vf && smokef && color && num_samples > 0, "edge_vortex_ring: bad arguments");
+  Grid g = make_grid(nx, ny, nz, 1.0f);
+  hipLaunchKernelGGL(k_vortex_ring, dim3(blocks_for(ncell(g), 256)), dim3(256), 0, (hipStream_t)stream, g, w, rad,
+                     delta, make_float3(c[0], c[1], c[2]), make_float3(ex[0], ex[1], ex[2]),
+                     make_float3(ey[0], ey[1], ey[2]), pf, vf, smokef, make_float3(color[0], color[1], color[2]),
+                     num_samples);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_split_central_vector(const float* vc, float* ux, float* uy, float* uz, int nx, int ny, int nz,
+                              void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_split_central_vector: bad dims");
+  EDGE_REQUIRE(vc && ux && uy && uz, "edge_split_central_vector: null pointer");
+  Grid g = make_grid(nx, ny, nz, 1.0f);
+  int64_t nf = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
+  hipLaunchKernelGGL(k_split_central, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g, vc,
+                     Faces{{ux, uy, uz}});
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+}  // extern "C"

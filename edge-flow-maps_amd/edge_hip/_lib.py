@@ -1,0 +1,91 @@
+"""ctypes binding of libedge_hip.so (the C-ABI declared in include/edge_hip.h).
+
+The library is loaded from this package directory only; there is no fallback:
+if it is missing or fails to load, every caller gets a RuntimeError.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libedge_hip.so")
+
+_P, _I, _L, _F, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float, ctypes.c_double
+_CODES = {"p": _P, "i": _I, "l": _L, "f": _F, "d": _D}
+
+# name -> argument codes (every function returns int status unless listed in _RESTYPE)
+SIGNATURES = {
+    "edge_last_error": "",
+    "edge_abi_version": "",
+    "edge_stream_sync": "p",
+    # tables / FD
+    "edge_pack_hermite_table": "ipppppppppppiiifp",
+    "edge_calc_fd3": "ipppppppppiiifp",
+    # marches
+    "edge_march_edge": "ppppppppppppiiifffp",
+    "edge_march_ed8": "pppppppppppppppiiifffp",
+    "edge_march_ge": "pppppppppiiiffp",
+    "edge_march_ed4": "pppppppppiiifffp",
+    "edge_advance_map_faces": "pppppppppiiiffp",
+    "edge_advance_points": "ipppppppliiiffp",
+    "edge_compose_points": "pppppliiifp",
+    "edge_calc_ed4_garm": "ppppppppppppppplfp",
+    "edge_calc_ed": "ipppppppppppplfp",
+    "edge_offset_points": "pplfffp",
+    "edge_eval_points": "ipiiipppppiiiplfppp",
+    "edge_reset_identity": "pppppppppppiiip",
+    "edge_reset_faces": "ppppppiiifp",
+    "edge_coords": "ipiiifp",
+    # driver kernels
+    "edge_map_u": "ppppppppppppiiifp",
+    "edge_map_u_psi": "pppppppiiifp",
+    "edge_split_psi_hermite": "pppppppiiifp",
+    "edge_map_smoke": "ippipppiiifp",
+    "edge_map_smoke_psi": "ippipiiifp",
+    "edge_clamp_u": "pppppppppiiip",
+    "edge_advect_u_sl": "ppppppiiiffp",
+    "edge_max_speed": "ppppiiip",
+    "edge_axpy": "pppflp",
+    "edge_scale": "pfplp",
+    "edge_copy": "pplp",
+    "edge_fill": "pflp",
+    "edge_vortex_ring": "fffpppppppiiiip",
+    "edge_split_central_vector": "ppppiiip",
+}
+
+_RESTYPE = {"edge_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+def lib():
+    """Load libedge_hip.so once; raise loudly when it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libedge_hip.so not found at {LIB_PATH}; build it with "
+                "`python edge-flow-maps_amd/csrc/build.py` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, codes in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = [_CODES[c] for c in codes]
+            f.restype = _RESTYPE.get(name, ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def last_error():
+    msg = lib().edge_last_error()
+    return msg.decode() if msg else ""
+
+
+def call(name, *args):
+    """Call an exported function and raise RuntimeError on a non-zero status."""
+    st = getattr(lib(), name)(*args)
+    if st != 0:
+        raise RuntimeError(f"{name} failed (status {st}): {last_error()}")
+    return st
+
+
+def exported_symbols():
+    return list(SIGNATURES)

@@ -1,0 +1,123 @@
+"""The driver-local kernels of the reference's run_edge.py / run_ed8.py / run_ge.py.
+
+In the reference these @ti.kernel functions read module globals (res_*, dx,
+X_x..., psi_garm...); here the same computations take those fields as
+arguments.  Names match the reference; argument lists append what the
+reference took from globals.
+"""
+import torch
+
+from edge_hip import _lib
+from edge_hip.field import as_ptr, stream_handle, touch
+from edge_hip.tables import hermite_table
+
+
+def _cells_of_mac(u_x):
+    return u_x.shape[0] - 1, u_x.shape[1], u_x.shape[2]
+
+
+_speed_buf = {}
+
+
+def calc_max_speed(u_x, u_y, u_z):
+    """run_edge.py:295-303; returns max(1e-3, max cell-centred speed) (host sync, as max_speed[None])."""
+    nx, ny, nz = _cells_of_mac(u_x)
+    dev = u_x.data.device
+    buf = _speed_buf.get(dev)
+    if buf is None:
+        buf = _speed_buf[dev] = torch.zeros(1, dtype=torch.float32, device=dev)
+    _lib.call("edge_max_speed", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), buf.data_ptr(), nx, ny, nz, stream_handle())
+    return float(buf.item())
+
+
+def reset_to_identity(psi_x, psi_y, psi_z, T_x, T_y, T_z, dx):
+    """run_edge.py:306-320: face maps phi_a = X_a, F_a = e_a."""
+    nx, ny, nz = _cells_of_mac(psi_x)
+    _lib.call("edge_reset_faces", as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z), as_ptr(T_x), as_ptr(T_y), as_ptr(T_z),
+              nx, ny, nz, float(dx), stream_handle())
+    touch(psi_x, psi_y, psi_z, T_x, T_y, T_z)
+
+
+def march_phi_grid(phi_x, phi_y, phi_z, F_x, F_y, F_z, u_x, u_y, u_z, dx, dt):
+    """run_edge.py:330-357: advance_map on the three face grids (+dt), one launch."""
+    nx, ny, nz = _cells_of_mac(u_x)
+    _lib.call("edge_advance_map_faces", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), as_ptr(phi_x), as_ptr(phi_y),
+              as_ptr(phi_z), as_ptr(F_x), as_ptr(F_y), as_ptr(F_z), nx, ny, nz, float(dx), float(dt),
+              stream_handle())
+    touch(phi_x, phi_y, phi_z, F_x, F_y, F_z)
+
+
+def map_u(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, T_x, T_y, T_z, psi_x, psi_y, psi_z, dx):
+    """run_edge.py:360-373."""
+    nx, ny, nz = _cells_of_mac(u_x0)
+    _lib.call("edge_map_u", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
+              as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z), nx, ny, nz,
+              float(dx), stream_handle())
+    touch(u_x1, u_y1, u_z1)
+
+
+def _garm_table(m, dx):
+    psi, T_x, T_y, T_z, dT_x, dT_y, dT_z = m
+    return hermite_table("garm", psi, (T_x, T_y, T_z), (dT_x, dT_y, dT_z), None, dx)
+
+
+def map_u_psi(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, dx, garm_map):
+    """run_edge.py:376-417; garm_map = (psi_garm, T_x/y/z_garm, dT_x/y/z_garm)."""
+    nx, ny, nz = _cells_of_mac(u_x0)
+    tab = _garm_table(garm_map, dx)
+    _lib.call("edge_map_u_psi", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
+              tab.data_ptr(), nx, ny, nz, float(dx), stream_handle())
+    touch(u_x1, u_y1, u_z1)
+
+
+def _smoke(mode, smoke0, smoke1, psi_x, psi_y, psi_z, dx):
+    nx, ny, nz = smoke0.shape
+    _lib.call("edge_map_smoke", mode, as_ptr(smoke0), as_ptr(smoke1), smoke0.n or 1, as_ptr(psi_x), as_ptr(psi_y),
+              as_ptr(psi_z), nx, ny, nz, float(dx), stream_handle())
+    touch(smoke1)
+
+
+def map_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx):
+    """run_edge.py:420-427."""
+    _smoke(0, smoke0, smoke1, psi_x, psi_y, psi_z, dx)
+
+
+def clamp_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx):
+    """run_edge.py:469-477 (smoke1 clamped in place)."""
+    _smoke(1, smoke0, smoke1, psi_x, psi_y, psi_z, dx)
+
+
+def _smoke_psi(mode, smoke0, smoke1, dx, garm_map):
+    nx, ny, nz = smoke0.shape
+    tab = _garm_table(garm_map, dx)
+    _lib.call("edge_map_smoke_psi", mode, as_ptr(smoke0), as_ptr(smoke1), smoke0.n or 1, tab.data_ptr(), nx, ny, nz,
+              float(dx), stream_handle())
+    touch(smoke1)
+
+
+def map_smoke_psi(smoke0, smoke1, dx, garm_map):
+    """run_edge.py:443-466."""
+    _smoke_psi(0, smoke0, smoke1, dx, garm_map)
+
+
+def clamp_smoke_psi(smoke0, smoke1, dx, garm_map):
+    """run_edge.py:480-505 (smoke1 clamped in place)."""
+    _smoke_psi(1, smoke0, smoke1, dx, garm_map)
+
+
+def clamp_u3(u_x, u_y, u_z, u1_x, u1_y, u1_z, u2_x, u2_y, u2_z):
+    """run_edge.py:508-519 clamp_u on the three face grids in one launch."""
+    nx, ny, nz = _cells_of_mac(u_x)
+    _lib.call("edge_clamp_u", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), as_ptr(u1_x), as_ptr(u1_y), as_ptr(u1_z),
+              as_ptr(u2_x), as_ptr(u2_y), as_ptr(u2_z), nx, ny, nz, stream_handle())
+    touch(u2_x, u2_y, u2_z)
+
+
+def split_psi_hermite(psi_x, psi_y, psi_z, T_x, T_y, T_z, full_map, dx):
+    """run_ed8.py:190-223 / run_ge.py:184-217; full_map = (psi, T_x/y/z, dT_x/y/z, d2T_x/y/z)."""
+    psi, Tx, Ty, Tz, dTx, dTy, dTz, d2x, d2y, d2z = full_map
+    nx, ny, nz = psi.shape
+    tab = hermite_table("full", psi, (Tx, Ty, Tz), (dTx, dTy, dTz), (d2x, d2y, d2z), dx)
+    _lib.call("edge_split_psi_hermite", tab.data_ptr(), as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z), as_ptr(T_x),
+              as_ptr(T_y), as_ptr(T_z), nx, ny, nz, float(dx), stream_handle())
+    touch(psi_x, psi_y, psi_z, T_x, T_y, T_z)

@@ -1,0 +1,62 @@
+"""Drop-in for the kernels of the reference's util/taichi_utils.py.
+
+The @ti.func interpolators of that file (sample, interp_2, interp_grad_2,
+interp_u_MAC(_grad), interp_1, sample_min_max_1) are device code in
+libedge_hip.so (csrc/edge_device.h); the @ti.kernel entry points the step
+loop calls are here with the reference's names and arguments.
+"""
+from edge_hip import _lib
+from edge_hip.field import Field, as_ptr, stream_handle, touch, ti  # noqa: F401  (ti re-exported)
+
+eps = 1.e-6  # util/taichi_utils.py:3
+
+
+def scale_field(a, alpha, result):
+    """util/taichi_utils.py:6-9: result = alpha * a."""
+    assert a.data.shape == result.data.shape
+    _lib.call("edge_scale", as_ptr(a), float(alpha), as_ptr(result), result.numel, stream_handle())
+    touch(result)
+
+
+def add_fields(f1, f2, dest, multiplier):
+    """util/taichi_utils.py:12-15: dest = f1 + multiplier * f2."""
+    assert f1.data.shape == f2.data.shape == dest.data.shape
+    _lib.call("edge_axpy", as_ptr(f1), as_ptr(f2), as_ptr(dest), float(multiplier), dest.numel, stream_handle())
+    touch(dest)
+
+
+def _coords(mode, pf, dx, cell_shape):
+    nx, ny, nz = cell_shape
+    _lib.call("edge_coords", mode, as_ptr(pf), nx, ny, nz, float(dx), stream_handle())
+    touch(pf)
+
+
+def center_coords_func(pf, dx):
+    """util/taichi_utils.py:23-26."""
+    _coords(0, pf, dx, pf.shape)
+
+
+def x_coords_func(pf, dx):
+    """util/taichi_utils.py:29-32 (pf on the x-face grid)."""
+    n0, n1, n2 = pf.shape
+    _coords(1, pf, dx, (n0 - 1, n1, n2))
+
+
+def y_coords_func(pf, dx):
+    """util/taichi_utils.py:35-38."""
+    n0, n1, n2 = pf.shape
+    _coords(2, pf, dx, (n0, n1 - 1, n2))
+
+
+def z_coords_func(pf, dx):
+    """util/taichi_utils.py:41-44."""
+    n0, n1, n2 = pf.shape
+    _coords(3, pf, dx, (n0, n1, n2 - 1))
+
+
+def split_central_vector(vc, vx, vy, vz):
+    """util/taichi_utils.py:103-116: cell-centred vector -> MAC faces."""
+    nx, ny, nz = vc.shape
+    _lib.call("edge_split_central_vector", as_ptr(vc), as_ptr(vx), as_ptr(vy), as_ptr(vz), nx, ny, nz,
+              stream_handle())
+    touch(vx, vy, vz)
