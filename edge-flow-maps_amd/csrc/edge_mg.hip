@@ -1,0 +1,1255 @@
+// Multigrid-preconditioned CG pressure projection (sim/mgpcg.py,
+// sim/mgpcg_solid.py) on gfx950.
+//
+// Two storage modes share one algorithm skeleton:
+//  * EDGE_MG_PLAIN (0): the reference V-cycle (2 pre/post red-black
+//    Gauss-Seidel half-sweep pairs, 20 bottom pairs, restriction r[l+1] +=
+//    res/2, injection prolongation; sim/mgpcg.py:147-195) in FP32.
+//  * EDGE_MG_MIXED (1): the same smoother and transfers with the multigrid
+//    levels stored in BF16 (FP32 arithmetic) and the 20 bottom pairs replaced
+//    by a dense FP64 solve of the coarsest level.
+// Both keep x, p, Ap and the CG residual in FP32, do every inner product in
+// FP64 with a deterministic two-level reduction (per-block partials, the last
+// block to finish sums them in a fixed order and updates the CG scalars on the
+// device), and never read a scalar back except for the convergence flag.
+//
+// Fusions (none changes the arithmetic of a step, only where it happens):
+//  * fill(0) + first red half-sweep of a level = one pass (z = r/diag on red
+//    cells, 0 elsewhere), also fused into the restriction that produces r[l+1];
+//  * prolongation + the first black post-smoothing half-sweep = one pass (the
+//    black update reads its red neighbours as z + z_coarse on the fly; red
+//    cells are overwritten by the next half-sweep before anything reads them);
+//  * recenter of r (sim/mgpcg.py:169-175) is folded into the level-0 first pass;
+//  * update_p (p = z + beta p) + compute_Ap + reduce(p, Ap) = one pass;
+//  * update_x + update_r + reduce(r, r) + the sum for the next recenter = one pass;
+//  * the last red half-sweep of level 0 + reduce(z, r) = one pass.
+#include <hip/hip_bf16.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "edge_common.h"
+
+using namespace edge;
+
+namespace {
+
+constexpr int MAXL = 12;
+constexpr int NB = 1024;       // blocks of every reduction kernel
+constexpr int BS = 256;
+
+// device scalar slots (double)
+enum Slot {
+  S_ZTR = 0,     // z.r of the current direction
+  S_PAP,         // p.Ap
+  S_RTR,         // r.r after the update
+  S_B2,          // ||b||^2 (recentred)
+  S_MEAN,        // mean to subtract from r in the next level-0 pass
+  S_ALPHA,
+  S_BETA,
+  S_TOL2,        // stop when rTr < tol2 (reference) or <= tol2 (relative)
+  S_DONE,        // 0 running, 1 converged, 2 hit max_iters
+  S_ITERS,
+  S_MAXIT,
+  S_DSUM,        // solid: sum of fluid divergence
+  S_NFLUID,
+  S_CORR,        // solid: correction subtracted from fluid divergence
+  S_R02,         // warm start: ||b - A x0||^2
+  S_USEX0,
+  S_STOPMODE,
+  S_ABSTOL,
+  S_RELTOL,
+  S_SOLID,
+  S_NCELL,
+  S_ZTR0,
+  S_NEU,         // all walls: recentre r (sim/mgpcg.py:211-217)
+  S_COUNT
+};
+
+// finalize actions of the last block
+enum Fin { F_NONE = 0, F_MEAN, F_B2, F_ZR_INIT, F_ZR, F_ALPHA, F_XR, F_DIVSUM, F_X0, F_SUM2 };
+
+template <class S>
+__device__ __forceinline__ float ld(const S* p, int64_t i);
+template <>
+__device__ __forceinline__ float ld<float>(const float* p, int64_t i) { return p[i]; }
+template <>
+__device__ __forceinline__ float ld<__hip_bfloat16>(const __hip_bfloat16* p, int64_t i) {
+  return __bfloat162float(p[i]);
+}
+template <class S>
+__device__ __forceinline__ void st(S* p, int64_t i, float v);
+template <>
+__device__ __forceinline__ void st<float>(float* p, int64_t i, float v) { p[i] = v; }
+template <>
+__device__ __forceinline__ void st<__hip_bfloat16>(__hip_bfloat16* p, int64_t i, float v) {
+  p[i] = __float2bfloat16(v);
+}
+
+struct Lv {
+  int n0, n1, n2;
+  const int8_t* bm;   // 1 = solid (null when no solids)
+};
+
+__device__ __forceinline__ int64_t lid(const Lv& L, int i, int j, int k) {
+  return ((int64_t)i * L.n1 + j) * L.n2 + k;
+}
+__device__ __forceinline__ bool solid_at(const Lv& L, int i, int j, int k) {
+  return L.bm && L.bm[lid(L, i, j, k)] > 0;
+}
+
+// sim/mgpcg.py:101-110 / sim/mgpcg_solid.py:113-130 (bt: 6 bits, bit 2a+s set
+// when boundary_types[a][s] == 2).
+__device__ __forceinline__ float num_fluid(const Lv& L, int bt, int i, int j, int k) {
+  float num = 6.0f;
+  int I[3] = {i, j, k};
+  int n[3] = {L.n0, L.n1, L.n2};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+    lo[a] -= 1;
+    hi[a] += 1;
+    if (I[a] <= 0 && ((bt >> (2 * a)) & 1))
+      num -= 1.0f;
+    else if (L.bm && I[a] > 0 && solid_at(L, lo[0], lo[1], lo[2]))
+      num -= 1.0f;
+    if (I[a] >= n[a] - 1 && ((bt >> (2 * a + 1)) & 1))
+      num -= 1.0f;
+    else if (L.bm && I[a] < n[a] - 1 && solid_at(L, hi[0], hi[1], hi[2]))
+      num -= 1.0f;
+  }
+  return num;
+}
+
+// sim/mgpcg.py:87-99 / sim/mgpcg_solid.py:99-111; the value of a neighbour is
+// fetched by `get(i,j,k)` so callers can add the prolonged coarse value.
+template <class G>
+__device__ __forceinline__ float neighbor_sum(const Lv& L, int i, int j, int k, G get) {
+  float ret = 0.f;
+  if (i < L.n0 - 1 && !solid_at(L, i + 1, j, k)) ret += get(i + 1, j, k);
+  if (i > 0 && !solid_at(L, i - 1, j, k)) ret += get(i - 1, j, k);
+  if (j < L.n1 - 1 && !solid_at(L, i, j + 1, k)) ret += get(i, j + 1, k);
+  if (j > 0 && !solid_at(L, i, j - 1, k)) ret += get(i, j - 1, k);
+  if (k < L.n2 - 1 && !solid_at(L, i, j, k + 1)) ret += get(i, j, k + 1);
+  if (k > 0 && !solid_at(L, i, j, k - 1)) ret += get(i, j, k - 1);
+  return ret;
+}
+
+__device__ __forceinline__ void decode(const Lv& L, int64_t q, int& i, int& j, int& k) {
+  k = (int)(q % L.n2);
+  j = (int)((q / L.n2) % L.n1);
+  i = (int)(q / ((int64_t)L.n1 * L.n2));
+}
+
+// cell of colour `phase` number t (n2 even)
+__device__ __forceinline__ void colour_cell(const Lv& L, int64_t t, int phase, int& i, int& j, int& k) {
+  int h = L.n2 >> 1;
+  int kk = (int)(t % h);
+  int64_t row = t / h;
+  j = (int)(row % L.n1);
+  i = (int)(row / L.n1);
+  k = 2 * kk + ((i + j + phase) & 1);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions: per-block FP64 partials; the last block to finish
+// sums them in a fixed order and runs the finalize action.
+// ---------------------------------------------------------------------------
+struct Red {
+  double* partials;      // NB * 2
+  unsigned int* counter; // 1
+  double* sc;            // scalars
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ void finalize(double* sc, int fin, double s0, double s1) {
+  switch (fin) {
+    case F_MEAN:
+      sc[S_MEAN] = s0 / sc[S_NCELL];
+      break;
+    case F_SUM2:
+      sc[S_B2] = s0;
+      sc[S_MEAN] = 0.0;
+      break;
+    case F_B2:
+      sc[S_B2] = s0;
+      sc[S_MEAN] = 0.0;  // r is mean-free now
+      sc[S_TOL2] = sc[S_RELTOL] * sc[S_RELTOL] * s0;  // relative rule; replaced after z.r in reference mode
+      break;
+    case F_ZR_INIT:
+      sc[S_ZTR] = s0;
+      sc[S_ZTR0] = s0;
+      sc[S_BETA] = 0.0;
+      if (sc[S_STOPMODE] == 0.0) {
+        // sim/mgpcg.py:228 tol = max(abs_tol, z0'r0 * rel_tol); solid: abs_tol (sim/mgpcg_solid.py:281)
+        sc[S_TOL2] = sc[S_SOLID] != 0.0 ? sc[S_ABSTOL] : fmax(sc[S_ABSTOL], s0 * sc[S_RELTOL]);
+      }
+      break;
+    case F_ZR: {
+      // sim/mgpcg.py:270-276
+      double nz = s0;
+      sc[S_BETA] = nz / (sc[S_ZTR] + 1e-12);
+      sc[S_ZTR] = nz;
+      break;
+    }
+    case F_ALPHA:
+      // sim/mgpcg.py:236-239
+      sc[S_PAP] = s0;
+      sc[S_ALPHA] = sc[S_ZTR] / (s0 + 1e-12);
+      break;
+    case F_XR: {
+      // sim/mgpcg.py:248-259: rTr and the stop test; s1 = sum of r for recenter
+      sc[S_RTR] = s0;
+      sc[S_MEAN] = sc[S_NEU] != 0.0 ? s1 / sc[S_NCELL] : 0.0;
+      double it = sc[S_ITERS] + 1.0;
+      sc[S_ITERS] = it;
+      bool conv = sc[S_STOPMODE] == 0.0 ? (s0 < sc[S_TOL2]) : (s0 <= sc[S_TOL2]);
+      if (conv)
+        sc[S_DONE] = 1.0;
+      else if (it >= sc[S_MAXIT])
+        sc[S_DONE] = 2.0;
+      break;
+    }
+    case F_DIVSUM:
+      // sim/mgpcg_solid.py:372-383, 421-424: one-sided correction
+      sc[S_DSUM] = s0;
+      sc[S_NFLUID] = s1;
+      sc[S_CORR] = (s0 > 0.0 && s1 > 0.0) ? s0 / s1 : 0.0;
+      break;
+    case F_X0:
+      sc[S_R02] = s0;
+      sc[S_USEX0] = (s0 <= sc[S_B2]) ? 1.0 : 0.0;
+      break;
+    default:
+      break;
+  }
+}
+
+__device__ void block_reduce_finish(const Red& R, int fin, double s0, double s1) {
+  __shared__ double sh[2][BS / 64];
+  __shared__ bool last;
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  if (ln == 0) {
+    sh[0][w] = s0;
+    sh[1][w] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      a += sh[0][q];
+      b += sh[1][q];
+    }
+    R.partials[2 * blockIdx.x] = a;
+    R.partials[2 * blockIdx.x + 1] = b;
+    __threadfence();
+    unsigned t = atomicAdd(R.counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile double* P = R.partials;
+  double a = 0, b = 0;
+  for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x) {
+    a += P[2 * q];
+    b += P[2 * q + 1];
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (ln == 0) {
+    sh[0][w] = a;
+    sh[1][w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0, tb = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+      ta += sh[0][q];
+      tb += sh[1][q];
+    }
+    finalize(R.sc, fin, ta, tb);
+    *R.counter = 0u;
+    __threadfence();
+  }
+}
+
+#define EXIT_IF_DONE(sc) \
+  if ((sc)[S_DONE] != 0.0) return
+
+// ---------------------------------------------------------------------------
+// Boundary conditions, divergence, gradient (sim/mgpcg.py:373-432,
+// sim/mgpcg_solid.py:326-411)
+// ---------------------------------------------------------------------------
+struct Faces3 {
+  float* f[3];
+};
+
+__device__ __forceinline__ bool face_decode(int nx, int ny, int nz, int64_t t, int& a, int64_t& l, int& i, int& j,
+                                            int& k) {
+  const int64_t n0 = (int64_t)(nx + 1) * ny * nz, n1 = (int64_t)nx * (ny + 1) * nz,
+                n2 = (int64_t)nx * ny * (nz + 1);
+  if (t < n0) a = 0;
+  else if (t < n0 + n1) { a = 1; t -= n0; }
+  else if (t < n0 + n1 + n2) { a = 2; t -= n0 + n1; }
+  else return false;
+  l = t;
+  int d1 = ny + (a == 1), d2 = nz + (a == 2);
+  k = (int)(t % d2);
+  j = (int)((t / d2) % d1);
+  i = (int)(t / ((int64_t)d1 * d2));
+  return true;
+}
+
+// apply_bc: box walls zeroed, then the faces of solid cells set to boundary_vel;
+// where two solid cells share a face the higher-index cell's value stands (the
+// result of a sequential loop over cells).
+__global__ void __launch_bounds__(BS) k_apply_bc(int nx, int ny, int nz, int bt, Faces3 U, const int* __restrict__ bm,
+                                                const float* __restrict__ bvel) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int a, i, j, k;
+  int64_t l;
+  if (!face_decode(nx, ny, nz, t, a, l, i, j, k)) return;
+  int I[3] = {i, j, k};
+  int n[3] = {nx, ny, nz};
+  float v = U.f[a][l];
+  bool changed = false;
+  if (I[a] == 0 && ((bt >> (2 * a)) & 1)) { v = 0.f; changed = true; }
+  if (I[a] == n[a] && ((bt >> (2 * a + 1)) & 1)) { v = 0.f; changed = true; }
+  if (bm) {
+    const int64_t N = (int64_t)nx * ny * nz;
+    // cell on the + side (index I[a]) and on the - side (I[a]-1)
+    int hi[3] = {i, j, k}, lo[3] = {i, j, k};
+    lo[a] -= 1;
+    bool done = false;
+    if (I[a] < n[a]) {
+      int64_t q = ((int64_t)hi[0] * ny + hi[1]) * nz + hi[2];
+      if (bm[q] > 0) { v = bvel[a * N + q]; done = true; changed = true; }
+    }
+    if (!done && I[a] > 0) {
+      int64_t q = ((int64_t)lo[0] * ny + lo[1]) * nz + lo[2];
+      if (bm[q] > 0) { v = bvel[a * N + q]; changed = true; }
+    }
+  }
+  if (changed) U.f[a][l] = v;
+}
+
+__device__ __forceinline__ float divergence_at(int nx, int ny, int nz, const Faces3& U, int i, int j, int k) {
+  // sim/mgpcg.py:394-404 (the sample() clamps never bind for i+1 <= nx)
+  int64_t qx = ((int64_t)i * ny + j) * nz + k;
+  int64_t qy = ((int64_t)i * (ny + 1) + j) * nz + k;
+  int64_t qz = ((int64_t)i * ny + j) * (nz + 1) + k;
+  float vl = U.f[0][qx], vr = U.f[0][qx + (int64_t)ny * nz];
+  float vb = U.f[1][qy], vt = U.f[1][qy + nz];
+  float va = U.f[2][qz], vc = U.f[2][qz + 1];
+  return vr - vl + vt - vb + vc - va;
+}
+
+// solid: sum and count of the fluid divergence
+__global__ void __launch_bounds__(BS) k_div_sum(int nx, int ny, int nz, Faces3 U, const int8_t* __restrict__ bm0, Red R) {
+  const int64_t N = (int64_t)nx * ny * nz;
+  double s = 0, c = 0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    if (bm0[q] > 0) continue;
+    int k = (int)(q % nz), j = (int)((q / nz) % ny), i = (int)(q / ((int64_t)ny * nz));
+    s += (double)divergence_at(nx, ny, nz, U, i, j, k);
+    c += 1.0;
+  }
+  block_reduce_finish(R, F_DIVSUM, s, c);
+}
+
+// init(u_div, -1) (sim/mgpcg.py:67-75): r = -(div - corr) on fluid cells (0 on
+// solids), x = x0 or 0, p = 0; partial sum of r for the recenter.
+// When b is given instead (solve API), r = b.
+__global__ void __launch_bounds__(BS) k_init(int nx, int ny, int nz, Faces3 U, const float* __restrict__ b,
+                                            const int8_t* __restrict__ bm0, float* __restrict__ r, float* __restrict__ x,
+                                            float* __restrict__ p, const float* __restrict__ x0, Red R) {
+  const int64_t N = (int64_t)nx * ny * nz;
+  const double* sc = R.sc;
+  float corr = (float)sc[S_CORR];
+  double s = 0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    float v;
+    if (b) {
+      v = b[q];
+    } else if (bm0 && bm0[q] > 0) {
+      v = 0.f;
+    } else {
+      int k = (int)(q % nz), j = (int)((q / nz) % ny), i = (int)(q / ((int64_t)ny * nz));
+      float d = divergence_at(nx, ny, nz, U, i, j, k);
+      if (bm0) d -= corr;
+      v = d * -1.0f;
+    }
+    r[q] = v;
+    x[q] = x0 ? x0[q] : 0.f;
+    p[q] = 0.f;
+    s += (double)v;
+  }
+  block_reduce_finish(R, F_MEAN, s, 0.0);
+}
+
+// r -= mean over all cells (sim/mgpcg.py:169-175) and ||r||^2 over fluid cells.
+__global__ void __launch_bounds__(BS) k_recenter(Lv L, float* __restrict__ r, Red R, int fin) {
+  const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
+  float mean = (float)R.sc[S_MEAN];
+  double s = 0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    float v = r[q] - mean;
+    r[q] = v;
+    if (!(L.bm && L.bm[q] > 0)) s += (double)v * (double)v;
+  }
+  block_reduce_finish(R, fin, s, 0.0);
+}
+
+// warm start: w = r - A x0 and ||w||^2; decide whether to keep x0.
+__global__ void __launch_bounds__(BS) k_residual_x0(Lv L, int bt, const float* __restrict__ r, const float* __restrict__ x,
+                                                   float* __restrict__ w, Red R) {
+  const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
+  double s = 0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(L, q, i, j, k);
+    if (L.bm && L.bm[q] > 0) {
+      w[q] = r[q];
+      continue;
+    }
+    float ax = num_fluid(L, bt, i, j, k) * x[q] -
+               neighbor_sum(L, i, j, k, [&](int a, int b, int c) { return x[lid(L, a, b, c)]; });
+    float v = r[q] - ax;
+    w[q] = v;
+    s += (double)v * (double)v;
+  }
+  block_reduce_finish(R, F_X0, s, 0.0);
+}
+
+__global__ void __launch_bounds__(BS) k_apply_x0(int64_t N, float* __restrict__ r, const float* __restrict__ w,
+                                                float* __restrict__ x, const double* __restrict__ sc) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  if (sc[S_USEX0] != 0.0) r[q] = w[q];
+  else x[q] = 0.f;
+}
+
+// subtract_grad_p (sim/mgpcg.py:406-432) with p = x, + apply_bc, and the
+// optional force buffer -(grad p) (sim/mgpcg.py:447-479).
+__global__ void __launch_bounds__(BS) k_subtract_grad(int nx, int ny, int nz, Faces3 U, Faces3 F,
+                                                     const float* __restrict__ P) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int a, i, j, k;
+  int64_t l;
+  if (!face_decode(nx, ny, nz, t, a, l, i, j, k)) return;
+  int I[3] = {i, j, k};
+  int n[3] = {nx, ny, nz};
+  auto pat = [&](int ii, int jj, int kk) {
+    ii = max(0, min(ii, nx - 1));
+    jj = max(0, min(jj, ny - 1));
+    kk = max(0, min(kk, nz - 1));
+    return P[((int64_t)ii * ny + jj) * nz + kk];
+  };
+  float pr = pat(i, j, k);
+  float pl = pat(i - (a == 0), j - (a == 1), k - (a == 2));
+  if (I[a] - 1 < 0) pl = 0.f;
+  if (I[a] >= n[a]) pr = 0.f;
+  U.f[a][l] -= (pr - pl);
+  if (F.f[a]) F.f[a][l] = -(pr - pl);
+}
+
+// ---------------------------------------------------------------------------
+// CG vector kernels
+// ---------------------------------------------------------------------------
+// p = z + beta p (fluid cells; sim/mgpcg.py:142-145), Ap = A p with neighbours'
+// new p computed on the fly, and p.Ap (sim/mgpcg.py:112-130).
+template <class S>
+__global__ void __launch_bounds__(BS) k_p_Ap(Lv L, int bt, const S* __restrict__ z, float* __restrict__ p,
+                                            float* __restrict__ Ap, Red R) {
+  EXIT_IF_DONE(R.sc);
+  const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
+  const float beta = (float)R.sc[S_BETA];
+  double s = 0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    int i, j, k;
+    decode(L, q, i, j, k);
+    if (L.bm && L.bm[q] > 0) continue;  // solid p stays (0), Ap untouched
+    auto pn = [&](int a, int b, int c) {
+      int64_t m = lid(L, a, b, c);
+      return ld(z, m) + beta * p[m];
+    };
+    float pq = pn(i, j, k);
+    float ap = num_fluid(L, bt, i, j, k) * pq - neighbor_sum(L, i, j, k, pn);
+    Ap[q] = ap;
+    s += (double)pq * (double)ap;
+  }
+  // p itself is written by k_update_xr: neighbours above read the old p
+  block_reduce_finish(R, F_ALPHA, s, 0.0);
+}
+
+// p = z + beta p, x += alpha p, r -= alpha Ap (fluid cells); r.r over fluid
+// cells and the sum of r over all cells (sim/mgpcg.py:132-145, 248-249, 169-175).
+template <class S>
+__global__ void __launch_bounds__(BS) k_update_xr(Lv L, const S* __restrict__ z, float* __restrict__ x,
+                                                 float* __restrict__ r, float* __restrict__ p,
+                                                 const float* __restrict__ Ap, Red R) {
+  EXIT_IF_DONE(R.sc);
+  const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
+  const float alpha = (float)R.sc[S_ALPHA];
+  const float beta = (float)R.sc[S_BETA];
+  double s2 = 0, s1 = 0;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+    float rv = r[q];
+    if (!(L.bm && L.bm[q] > 0)) {
+      float pn = ld(z, q) + beta * p[q];
+      p[q] = pn;
+      x[q] += alpha * pn;
+      rv -= alpha * Ap[q];
+      r[q] = rv;
+      s2 += (double)rv * (double)rv;
+    }
+    s1 += (double)rv;
+  }
+  block_reduce_finish(R, F_XR, s2, s1);
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle kernels (sim/mgpcg.py:147-195)
+// ---------------------------------------------------------------------------
+// Level-0 entry: r <- r - mean (recenter) on every cell; the level's residual
+// copy rl (== r in FP32 mode) is written; z = r/diag on red fluid cells, 0
+// elsewhere (fill(0) + the first red half-sweep).
+template <class S>
+__global__ void __launch_bounds__(BS) k_first0(Lv L, int bt, float* __restrict__ r, S* __restrict__ rl,
+                                              S* __restrict__ z, const double* __restrict__ sc, int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  float mean = (float)sc[S_MEAN];
+  float v = r[q] - mean;
+  r[q] = v;
+  if ((void*)rl != (void*)r) st(rl, q, v);
+  int i, j, k;
+  decode(L, q, i, j, k);
+  float zv = 0.f;
+  if (((i + j + k) & 1) == 0 && !(L.bm && L.bm[q] > 0)) zv = v / num_fluid(L, bt, i, j, k);
+  st(z, q, zv);
+}
+
+// One red-black Gauss-Seidel half-sweep (sim/mgpcg.py:160-167).
+template <class S>
+__global__ void __launch_bounds__(BS) k_smooth(Lv L, int bt, const S* __restrict__ r, S* __restrict__ z, int phase,
+                                              const double* __restrict__ sc, int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  const int64_t half = ((int64_t)L.n0 * L.n1 * L.n2) >> 1;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= half) return;
+  int i, j, k;
+  colour_cell(L, t, phase, i, j, k);
+  int64_t q = lid(L, i, j, k);
+  if (L.bm && L.bm[q] > 0) return;
+  float ns = neighbor_sum(L, i, j, k, [&](int a, int b, int c) { return ld(z, lid(L, a, b, c)); });
+  st(z, q, (ld(r, q) + ns) / num_fluid(L, bt, i, j, k));
+}
+
+// Last red half-sweep of level 0 fused with reduce(z, r) over fluid cells:
+// thread per (k even, k+1) pair, updates the red cell, reads the black one.
+template <class S>
+__global__ void __launch_bounds__(BS) k_smooth_last_dot(Lv L, int bt, const S* __restrict__ rl,
+                                                       const float* __restrict__ r, S* __restrict__ z, Red R, int fin,
+                                                       int ignore_done) {
+  if (!ignore_done && R.sc[S_DONE] != 0.0) return;
+  const int64_t pairs = ((int64_t)L.n0 * L.n1 * L.n2) >> 1;
+  double s = 0;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < pairs; t += (int64_t)gridDim.x * blockDim.x) {
+    int h = L.n2 >> 1;
+    int kk = (int)(t % h);
+    int64_t row = t / h;
+    int j = (int)(row % L.n1), i = (int)(row / L.n1);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      int k = 2 * kk + e;
+      int64_t q = lid(L, i, j, k);
+      if (L.bm && L.bm[q] > 0) continue;
+      float zv;
+      if (((i + j + k) & 1) == 0) {
+        float ns = neighbor_sum(L, i, j, k, [&](int a, int b, int c) { return ld(z, lid(L, a, b, c)); });
+        zv = (ld(rl, q) + ns) / num_fluid(L, bt, i, j, k);
+        st(z, q, zv);
+        zv = ld(z, q);  // the stored value is what CG uses
+      } else {
+        zv = ld(z, q);
+      }
+      s += (double)zv * (double)r[q];
+    }
+  }
+  block_reduce_finish(R, fin, s, 0.0);
+}
+
+// Restriction (sim/mgpcg.py:147-153) in gather form: each coarse cell sums
+// res/2 of its fluid children, then gets the fused fill(0) + first red
+// half-sweep of its level.
+template <class S>
+__global__ void __launch_bounds__(BS) k_restrict(Lv F, Lv C, int bt, const S* __restrict__ rf, const S* __restrict__ zf,
+                                                S* __restrict__ rc, S* __restrict__ zc, const double* __restrict__ sc,
+                                                int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  const int64_t NC = (int64_t)C.n0 * C.n1 * C.n2;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= NC) return;
+  int I, J, K;
+  decode(C, q, I, J, K);
+  float acc = 0.f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    int i = 2 * I + (c >> 2), j = 2 * J + ((c >> 1) & 1), k = 2 * K + (c & 1);
+    int64_t f = lid(F, i, j, k);
+    if (F.bm && F.bm[f] > 0) continue;
+    float ns = neighbor_sum(F, i, j, k, [&](int a, int b, int cc) { return ld(zf, lid(F, a, b, cc)); });
+    float res = ld(rf, f) - (num_fluid(F, bt, i, j, k) * ld(zf, f) - ns);
+    acc += res * 1.0f / 2.0f;
+  }
+  st(rc, q, acc);
+  float zv = 0.f;
+  if (((I + J + K) & 1) == 0 && !(C.bm && C.bm[q] > 0)) zv = ld(rc, q) / num_fluid(C, bt, I, J, K);
+  st(zc, q, zv);
+}
+
+// Prolongation (sim/mgpcg.py:155-158) fused with the first black post-smoothing
+// half-sweep: neighbours read as z + z_coarse.
+template <class S>
+__global__ void __launch_bounds__(BS) k_prolong_black(Lv F, Lv C, int bt, const S* __restrict__ rf, S* __restrict__ zf,
+                                                     const S* __restrict__ zc, const double* __restrict__ sc,
+                                                     int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  const int64_t half = ((int64_t)F.n0 * F.n1 * F.n2) >> 1;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= half) return;
+  int i, j, k;
+  colour_cell(F, t, 1, i, j, k);
+  int64_t q = lid(F, i, j, k);
+  // solid cells: the reference's unmasked prolongation (sim/mgpcg_solid.py:182-185)
+  // changes z there, but no kernel ever reads z of a solid cell
+  if (F.bm && F.bm[q] > 0) return;
+  float ns = neighbor_sum(F, i, j, k, [&](int a, int b, int c) {
+    return ld(zf, lid(F, a, b, c)) + ld(zc, lid(C, a >> 1, b >> 1, c >> 1));
+  });
+  st(zf, q, (ld(rf, q) + ns) / num_fluid(F, bt, i, j, k));
+}
+
+// Bottom of the plain V-cycle: the 20 red/black pairs of sim/mgpcg.py:187-189
+// (the first red half-sweep was fused into the restriction) in one workgroup,
+// the level held in LDS.
+template <class S>
+__global__ void __launch_bounds__(1024) k_bottom_rbgs(Lv L, int bt, const S* __restrict__ r, S* __restrict__ z,
+                                                     const double* __restrict__ sc, int ignore_done, int pairs) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  extern __shared__ float lds[];
+  const int N = L.n0 * L.n1 * L.n2;
+  float* zs = lds;
+  float* rs = lds + N;
+  float* dg = lds + 2 * N;
+  for (int q = threadIdx.x; q < N; q += blockDim.x) {
+    zs[q] = ld(z, q);
+    rs[q] = ld(r, q);
+    int i, j, k;
+    decode(L, q, i, j, k);
+    dg[q] = (L.bm && L.bm[q] > 0) ? 0.f : num_fluid(L, bt, i, j, k);
+  }
+  __syncthreads();
+  for (int s = 0; s < 2 * pairs - 1; ++s) {
+    int phase = (s & 1) == 0 ? 1 : 0;  // black, red, black, ...
+    for (int q = threadIdx.x; q < N; q += blockDim.x) {
+      int i, j, k;
+      decode(L, q, i, j, k);
+      if (((i + j + k) & 1) != phase || dg[q] == 0.f) continue;
+      float ns = neighbor_sum(L, i, j, k, [&](int a, int b, int c) { return zs[lid(L, a, b, c)]; });
+      zs[q] = (rs[q] + ns) / dg[q];
+    }
+    __syncthreads();
+  }
+  for (int q = threadIdx.x; q < N; q += blockDim.x) st(z, q, zs[q]);
+}
+
+// Bottom of the mixed V-cycle: z = B^-1 r over the fluid cells of the coarsest
+// level, B = A + c 1 1^T (SPD; equal to the pseudo-inverse on mean-free r), FP64.
+template <class S>
+__global__ void __launch_bounds__(256) k_bottom_dense(int nf, const int* __restrict__ fidx, const double* __restrict__ Binv,
+                                                     const S* __restrict__ r, S* __restrict__ z, int64_t ncoarse,
+                                                     const double* __restrict__ sc, int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  int row = blockIdx.x;
+  if (row >= nf) return;
+  double s = 0;
+  const double* Brow = Binv + (int64_t)row * nf;
+  for (int c = threadIdx.x; c < nf; c += blockDim.x) s += Brow[c] * (double)ld(r, fidx[c]);
+  s = wave_sum(s);
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+    st(z, fidx[row], (float)t);
+  }
+}
+
+// solid masks: level-0 mask from the user's int32 boundary_mask, then the
+// coarser levels (sim/mgpcg_solid.py:87-97, every solve).
+__global__ void k_mask0(int64_t N, const int* __restrict__ bm, int8_t* __restrict__ out) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < N) out[q] = bm[q] > 0 ? 1 : 0;
+}
+__global__ void k_downsample(Lv F, Lv C, int8_t* __restrict__ out) {
+  const int64_t NC = (int64_t)C.n0 * C.n1 * C.n2;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= NC) return;
+  int I, J, K;
+  decode(C, q, I, J, K);
+  int all = 1;
+  for (int c = 0; c < 8; ++c)
+    if (F.bm[lid(F, 2 * I + (c >> 2), 2 * J + ((c >> 1) & 1), 2 * K + (c & 1))] <= 0) all = 0;
+  out[q] = (int8_t)all;
+}
+
+__global__ void k_set_scalars(double* sc, int n, const double* vals) {
+  int t = threadIdx.x;
+  if (t < n) sc[t] = vals[t];
+}
+__global__ void k_set_slot(double* sc, int slot, double v) { sc[slot] = v; }
+
+// A p for a given p (test hook; sim/mgpcg.py:119-124).
+__global__ void __launch_bounds__(BS) k_apply_A(Lv L, int bt, const float* __restrict__ p, float* __restrict__ Ap) {
+  const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  if (L.bm && L.bm[q] > 0) {
+    Ap[q] = 0.f;
+    return;
+  }
+  int i, j, k;
+  decode(L, q, i, j, k);
+  Ap[q] = num_fluid(L, bt, i, j, k) * p[q] - neighbor_sum(L, i, j, k, [&](int a, int b, int c) { return p[lid(L, a, b, c)]; });
+}
+
+template <class S>
+__global__ void k_to_float(int64_t n, const S* __restrict__ a, float* __restrict__ o) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) o[q] = ld(a, q);
+}
+
+// ---------------------------------------------------------------------------
+// Solver object
+// ---------------------------------------------------------------------------
+struct Solver {
+  int nx, ny, nz, L, mode, bt;
+  int64_t N;
+  Lv lv[MAXL];
+  int64_t nl[MAXL];
+  void* rl[MAXL];   // level residual (level 0 aliases r in plain mode)
+  void* zl[MAXL];
+  int8_t* bm[MAXL];  // null when no solids
+  float *r, *x, *p, *Ap;
+  double* partials;
+  unsigned int* counter;
+  double* sc;
+  double* sc_host;   // pinned
+  bool solid = false;
+  const int* user_bm = nullptr;
+  const float* user_bvel = nullptr;
+  // dense coarse solve
+  double* Binv = nullptr;
+  int* fidx = nullptr;
+  int nf = 0;
+  bool dense_ok = false;
+  std::vector<int8_t> dense_mask;  // coarse mask the inverse was built for
+  size_t elem;
+};
+
+template <class S>
+struct Ops {
+  static void vcycle(Solver& s, hipStream_t st, int ignore_done);
+};
+
+int bottom_lds_bytes(const Solver& s) { return (int)(s.nl[s.L - 1] * 3 * sizeof(float)); }
+
+template <class S>
+void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
+  const int L = s.L;
+  const double* sc = s.sc;
+  S* r0 = (S*)s.rl[0];
+  S* z0 = (S*)s.zl[0];
+  hipLaunchKernelGGL(k_first0<S>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig);
+  for (int l = 0; l < L - 1; ++l) {
+    S* rl = (S*)s.rl[l];
+    S* zl = (S*)s.zl[l];
+    int64_t half = s.nl[l] / 2;
+    // remaining pre-smoothing: black, red, black (sim/mgpcg.py:180-182)
+    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 1, sc, ig);
+    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 0, sc, ig);
+    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 1, sc, ig);
+    hipLaunchKernelGGL(k_restrict<S>, dim3(blocks_for(s.nl[l + 1], BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt,
+                       rl, zl, (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
+  }
+  // bottom
+  {
+    int l = L - 1;
+    if (s.mode == 1 && s.dense_ok) {
+      hipLaunchKernelGGL(k_bottom_dense<S>, dim3(s.nf), dim3(256), 0, st, s.nf, s.fidx, s.Binv, (const S*)s.rl[l],
+                         (S*)s.zl[l], s.nl[l], sc, ig);
+    } else if (s.nl[l] <= 4096) {
+      hipLaunchKernelGGL(k_bottom_rbgs<S>, dim3(1), dim3(1024), bottom_lds_bytes(s), st, s.lv[l], s.bt,
+                         (const S*)s.rl[l], (S*)s.zl[l], sc, ig, 20);
+    } else {
+      int64_t half = s.nl[l] / 2;
+      for (int it = 0; it < 39; ++it)
+        hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt,
+                           (const S*)s.rl[l], (S*)s.zl[l], (it & 1) == 0 ? 1 : 0, sc, ig);
+    }
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    S* rl = (S*)s.rl[l];
+    S* zl = (S*)s.zl[l];
+    int64_t half = s.nl[l] / 2;
+    hipLaunchKernelGGL(k_prolong_black<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt, rl,
+                       zl, (const S*)s.zl[l + 1], sc, ig);
+    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 0, sc, ig);
+    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 1, sc, ig);
+    if (l > 0)
+      hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 0, sc, ig);
+    // level 0's last red half-sweep is fused with z.r by the caller
+  }
+}
+
+Red red(Solver& s) { return Red{s.partials, s.counter, s.sc}; }
+
+// Build B^-1 for the coarsest level on the host (FP64 Cholesky).  Returns
+// false when B is not positive definite (e.g. a disconnected fluid region).
+bool build_dense(Solver& s, const std::vector<int8_t>& cm) {
+  const Lv& C = s.lv[s.L - 1];
+  int n0 = C.n0, n1 = C.n1, n2 = C.n2;
+  int nc = n0 * n1 * n2;
+  std::vector<int> idx(nc, -1), fl;
+  for (int q = 0; q < nc; ++q)
+    if (!(s.solid && cm[q] > 0)) {
+      idx[q] = (int)fl.size();
+      fl.push_back(q);
+    }
+  int nf = (int)fl.size();
+  if (nf == 0 || nf > 4096) return false;
+  std::vector<double> A((size_t)nf * nf, 0.0);
+  bool all_neu = (s.bt == 63);
+  auto solid = [&](int i, int j, int k) { return s.solid && cm[(i * n1 + j) * n2 + k] > 0; };
+  for (int r = 0; r < nf; ++r) {
+    int q = fl[r];
+    int k = q % n2, j = (q / n2) % n1, i = q / (n1 * n2);
+    int I[3] = {i, j, k}, n[3] = {n0, n1, n2};
+    double num = 6;
+    for (int a = 0; a < 3; ++a) {
+      int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+      lo[a]--;
+      hi[a]++;
+      if (I[a] <= 0 && ((s.bt >> (2 * a)) & 1)) num -= 1;
+      else if (I[a] > 0 && solid(lo[0], lo[1], lo[2])) num -= 1;
+      if (I[a] >= n[a] - 1 && ((s.bt >> (2 * a + 1)) & 1)) num -= 1;
+      else if (I[a] < n[a] - 1 && solid(hi[0], hi[1], hi[2])) num -= 1;
+      if (I[a] > 0 && !solid(lo[0], lo[1], lo[2])) A[(size_t)r * nf + idx[(lo[0] * n1 + lo[1]) * n2 + lo[2]]] -= 1;
+      if (I[a] < n[a] - 1 && !solid(hi[0], hi[1], hi[2]))
+        A[(size_t)r * nf + idx[(hi[0] * n1 + hi[1]) * n2 + hi[2]]] -= 1;
+    }
+    A[(size_t)r * nf + r] += num;
+  }
+  if (all_neu) {
+    double c = 1.0 / nf;
+    for (auto& v : A) v += c;
+  }
+  // Cholesky A = L L^T (in place, lower)
+  for (int j = 0; j < nf; ++j) {
+    double d = A[(size_t)j * nf + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * nf + k] * A[(size_t)j * nf + k];
+    if (!(d > 1e-10)) return false;
+    d = std::sqrt(d);
+    A[(size_t)j * nf + j] = d;
+    for (int i = j + 1; i < nf; ++i) {
+      double v = A[(size_t)i * nf + j];
+      for (int k = 0; k < j; ++k) v -= A[(size_t)i * nf + k] * A[(size_t)j * nf + k];
+      A[(size_t)i * nf + j] = v / d;
+    }
+  }
+  // Linv (lower), then B^-1 = Linv^T Linv
+  std::vector<double> Li((size_t)nf * nf, 0.0);
+  for (int c = 0; c < nf; ++c) {
+    Li[(size_t)c * nf + c] = 1.0 / A[(size_t)c * nf + c];
+    for (int i = c + 1; i < nf; ++i) {
+      double v = 0;
+      for (int k = c; k < i; ++k) v -= A[(size_t)i * nf + k] * Li[(size_t)k * nf + c];
+      Li[(size_t)i * nf + c] = v / A[(size_t)i * nf + i];
+    }
+  }
+  std::vector<double> Bi((size_t)nf * nf, 0.0);
+  for (int i = 0; i < nf; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double v = 0;
+      for (int k = i; k < nf; ++k) v += Li[(size_t)k * nf + i] * Li[(size_t)k * nf + j];
+      Bi[(size_t)i * nf + j] = v;
+      Bi[(size_t)j * nf + i] = v;
+    }
+  if (s.Binv) (void)hipFree(s.Binv);
+  if (s.fidx) (void)hipFree(s.fidx);
+  s.Binv = nullptr;
+  s.fidx = nullptr;
+  if (hipMalloc(&s.Binv, Bi.size() * sizeof(double)) != hipSuccess) return false;
+  if (hipMalloc(&s.fidx, fl.size() * sizeof(int)) != hipSuccess) return false;
+  if (hipMemcpy(s.Binv, Bi.data(), Bi.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return false;
+  if (hipMemcpy(s.fidx, fl.data(), fl.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return false;
+  s.nf = nf;
+  s.dense_mask = cm;
+  return true;
+}
+
+void free_solver(Solver* s) {
+  if (!s) return;
+  for (int l = 0; l < s->L; ++l) {
+    if (s->rl[l] && s->rl[l] != (void*)s->r) (void)hipFree(s->rl[l]);
+    if (s->zl[l]) (void)hipFree(s->zl[l]);
+    if (s->bm[l]) (void)hipFree(s->bm[l]);
+  }
+  for (void* p : {(void*)s->r, (void*)s->x, (void*)s->p, (void*)s->Ap, (void*)s->partials, (void*)s->counter,
+                  (void*)s->sc, (void*)s->Binv, (void*)s->fidx})
+    if (p) (void)hipFree(p);
+  if (s->sc_host) (void)hipHostFree(s->sc_host);
+  delete s;
+}
+
+int set_scalars(Solver& s, hipStream_t st, int max_iters, int stop_mode, double abs_tol, double rel_tol) {
+  double v[S_COUNT];
+  for (int i = 0; i < S_COUNT; ++i) v[i] = 0.0;
+  v[S_MAXIT] = max_iters < 0 ? 1e18 : (double)max_iters;
+  v[S_STOPMODE] = stop_mode;
+  v[S_ABSTOL] = abs_tol;
+  v[S_RELTOL] = rel_tol;
+  v[S_SOLID] = s.solid ? 1.0 : 0.0;
+  v[S_NCELL] = (double)s.N;
+  v[S_NEU] = (s.bt == 63) ? 1.0 : 0.0;
+  // pass through pinned host memory (stream-ordered copy, no memset)
+  for (int i = 0; i < S_COUNT; ++i) s.sc_host[32 + i] = v[i];
+  EDGE_CHECK_HIP(hipMemcpyAsync(s.sc, s.sc_host + 32, S_COUNT * sizeof(double), hipMemcpyHostToDevice, st));
+  return EDGE_OK;
+}
+
+int prepare_masks(Solver& s, hipStream_t st) {
+  if (!s.solid) return EDGE_OK;
+  hipLaunchKernelGGL(k_mask0, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N, s.user_bm, s.bm[0]);
+  EDGE_CHECK_LAUNCH();
+  for (int l = 1; l < s.L; ++l) {
+    hipLaunchKernelGGL(k_downsample, dim3(blocks_for(s.nl[l], BS)), dim3(BS), 0, st, s.lv[l - 1], s.lv[l], s.bm[l]);
+    EDGE_CHECK_LAUNCH();
+  }
+  if (s.mode == 1) {
+    // the dense coarse inverse depends on the coarse mask
+    std::vector<int8_t> cm(s.nl[s.L - 1]);
+    EDGE_CHECK_HIP(hipMemcpyAsync(cm.data(), s.bm[s.L - 1], cm.size(), hipMemcpyDeviceToHost, st));
+    EDGE_CHECK_HIP(hipStreamSynchronize(st));
+    if (!s.dense_ok || cm != s.dense_mask) s.dense_ok = build_dense(s, cm);
+  }
+  return EDGE_OK;
+}
+
+template <class S>
+int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* stats) {
+  Red R = red(s);
+  const Lv& L0 = s.lv[0];
+  // initial preconditioner application and p, z.r (sim/mgpcg.py:216-228)
+  Ops<S>::vcycle(s, st, 1);
+  hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(NB), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r, (S*)s.zl[0], R,
+                     (int)F_ZR_INIT, 1);
+  EDGE_CHECK_LAUNCH();
+  int it = 0;
+  int done = 0;
+  const int maxit = max_iters < 0 ? (1 << 30) : max_iters;
+  while (!done) {
+    for (int b = 0; b < check_every; ++b) {
+      hipLaunchKernelGGL(k_p_Ap<S>, dim3(NB), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
+      hipLaunchKernelGGL(k_update_xr<S>, dim3(NB), dim3(BS), 0, st, L0, (const S*)s.zl[0], s.x, s.r, s.p, s.Ap, R);
+      Ops<S>::vcycle(s, st, 0);
+      hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(NB), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r,
+                         (S*)s.zl[0], R, (int)F_ZR, 0);
+      EDGE_CHECK_LAUNCH();
+      ++it;
+      if (it >= maxit) break;
+    }
+    EDGE_CHECK_HIP(hipMemcpyAsync(s.sc_host, s.sc, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st));
+    EDGE_CHECK_HIP(hipStreamSynchronize(st));
+    done = s.sc_host[S_DONE] != 0.0 || it >= maxit;
+  }
+  if (stats) {
+    stats[0] = s.sc_host[S_ITERS];
+    stats[1] = s.sc_host[S_RTR];
+    stats[2] = s.sc_host[S_B2];
+    stats[3] = s.sc_host[S_DSUM];
+    stats[4] = s.sc_host[S_CORR] != 0.0 ? 1.0 : 0.0;
+    stats[5] = s.sc_host[S_DONE] == 1.0 ? 1.0 : 0.0;
+    stats[6] = s.sc_host[S_ZTR0];
+    stats[7] = s.sc_host[S_R02];
+    stats[8] = s.sc_host[S_USEX0];
+  }
+  return s.sc_host[S_DONE] == 1.0 ? EDGE_OK : EDGE_ENOCONV;
+}
+
+int run_cg_any(Solver& s, hipStream_t st, int max_iters, int check_every, double* stats) {
+  if (s.mode == 1) return run_cg<__hip_bfloat16>(s, st, max_iters, check_every, stats);
+  return run_cg<float>(s, st, max_iters, check_every, stats);
+}
+
+// common start: r prepared by k_init; mean in S_MEAN.  Recentre (all-Neumann),
+// ||b||^2, optional warm start.
+int start_solve(Solver& s, hipStream_t st, const float* x0) {
+  Red R = red(s);
+  const Lv& L0 = s.lv[0];
+  bool neu = (s.bt == 63);
+  if (!neu) hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, st, s.sc, (int)S_MEAN, 0.0);
+  hipLaunchKernelGGL(k_recenter, dim3(NB), dim3(BS), 0, st, L0, s.r, R, (int)F_B2);
+  EDGE_CHECK_LAUNCH();
+  if (x0) {
+    hipLaunchKernelGGL(k_residual_x0, dim3(NB), dim3(BS), 0, st, L0, s.bt, s.r, s.x, s.Ap, R);
+    hipLaunchKernelGGL(k_apply_x0, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N, s.r, s.Ap, s.x, s.sc);
+    EDGE_CHECK_LAUNCH();
+    // with walls all round, 1'A = 0, so r = b - A x0 stays mean-free
+  }
+  return EDGE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Create a solver for an nx x ny x nz all-cell Poisson problem.
+// mode 0 plain (FP32, reference V-cycle), 1 mixed (BF16 levels, dense FP64 bottom).
+// bt: host int[6] boundary_types (row-major [axis][side], 2 = wall), or NULL for all walls.
+int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, void** handle) {
+  clear_error();
+  EDGE_REQUIRE(handle, "edge_mg_create: null handle");
+  *handle = nullptr;
+  EDGE_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "edge_mg_create: bad dims");
+  EDGE_REQUIRE(levels >= 2 && levels <= MAXL, "edge_mg_create: levels must be in 2..%d", MAXL);
+  EDGE_REQUIRE(mode == 0 || mode == 1, "edge_mg_create: mode must be 0 (plain) or 1 (mixed)");
+  for (int l = 0; l < levels; ++l) {
+    EDGE_REQUIRE(((nx >> l) << l) == nx && ((ny >> l) << l) == ny && ((nz >> l) << l) == nz,
+                 "edge_mg_create: dims must be divisible by 2^(levels-1)");
+    EDGE_REQUIRE((nz >> l) % 2 == 0 && (nx >> l) >= 1 && (ny >> l) >= 1,
+                 "edge_mg_create: every level needs an even nz (got %d at level %d)", nz >> l, l);
+  }
+  Solver* s = new Solver();
+  s->nx = nx;
+  s->ny = ny;
+  s->nz = nz;
+  s->L = levels;
+  s->mode = mode;
+  s->bt = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int sd = 0; sd < 2; ++sd)
+      if (!bt || bt[2 * a + sd] == 2) s->bt |= 1 << (2 * a + sd);
+  s->N = (int64_t)nx * ny * nz;
+  s->elem = mode == 1 ? 2 : 4;
+  std::memset(s->rl, 0, sizeof(s->rl));
+  std::memset(s->zl, 0, sizeof(s->zl));
+  std::memset(s->bm, 0, sizeof(s->bm));
+  s->r = s->x = s->p = s->Ap = nullptr;
+  s->partials = nullptr;
+  s->counter = nullptr;
+  s->sc = nullptr;
+  s->sc_host = nullptr;
+  auto fail = [&](const char* what) {
+    set_error("edge_mg_create: %s failed", what);
+    free_solver(s);
+    return EDGE_ENOMEM;
+  };
+  for (int l = 0; l < levels; ++l) {
+    s->lv[l] = Lv{nx >> l, ny >> l, nz >> l, nullptr};
+    s->nl[l] = (int64_t)(nx >> l) * (ny >> l) * (nz >> l);
+  }
+  if (hipMalloc(&s->r, s->N * 4) != hipSuccess) return fail("hipMalloc r");
+  if (hipMalloc(&s->x, s->N * 4) != hipSuccess) return fail("hipMalloc x");
+  if (hipMalloc(&s->p, s->N * 4) != hipSuccess) return fail("hipMalloc p");
+  if (hipMalloc(&s->Ap, s->N * 4) != hipSuccess) return fail("hipMalloc Ap");
+  for (int l = 0; l < levels; ++l) {
+    if (l == 0 && mode == 0) {
+      s->rl[0] = s->r;
+    } else if (hipMalloc(&s->rl[l], s->nl[l] * s->elem) != hipSuccess) {
+      return fail("hipMalloc level r");
+    }
+    if (hipMalloc(&s->zl[l], s->nl[l] * s->elem) != hipSuccess) return fail("hipMalloc level z");
+  }
+  if (hipMalloc(&s->partials, NB * 2 * sizeof(double)) != hipSuccess) return fail("hipMalloc partials");
+  if (hipMalloc(&s->counter, sizeof(unsigned int)) != hipSuccess) return fail("hipMalloc counter");
+  if (hipMalloc(&s->sc, 64 * sizeof(double)) != hipSuccess) return fail("hipMalloc scalars");
+  if (hipHostMalloc(&s->sc_host, 128 * sizeof(double)) != hipSuccess) return fail("hipHostMalloc");
+  if (hipMemset(s->counter, 0, sizeof(unsigned int)) != hipSuccess) return fail("hipMemset");
+  if (hipMemset(s->p, 0, s->N * 4) != hipSuccess) return fail("hipMemset");
+  if (s->nl[levels - 1] * 3 * 4 > 160 * 1024 && mode == 0) {
+    // bottom falls back to per-sweep launches; fine
+  }
+  if (mode == 1) {
+    std::vector<int8_t> cm(s->nl[levels - 1], 0);
+    s->dense_ok = build_dense(*s, cm);
+  }
+  if (hipDeviceSynchronize() != hipSuccess) return fail("hipDeviceSynchronize");
+  *handle = s;
+  return EDGE_OK;
+}
+
+int edge_mg_destroy(void* handle) {
+  clear_error();
+  if (handle) {
+    (void)hipDeviceSynchronize();
+    free_solver((Solver*)handle);
+  }
+  return EDGE_OK;
+}
+
+// Solids (sim/mgpcg_solid.py): bm int32 device mask (nx*ny*nz, >0 solid),
+// bvel 3 planes of boundary velocity; both borrowed until the next call.
+// NULL bm clears the mask.
+int edge_mg_set_solid(void* handle, const int* bm, const float* bvel) {
+  clear_error();
+  EDGE_REQUIRE(handle, "edge_mg_set_solid: null handle");
+  Solver& s = *(Solver*)handle;
+  if (!bm) {
+    s.solid = false;
+    for (int l = 0; l < s.L; ++l) s.lv[l].bm = nullptr;
+    s.user_bm = nullptr;
+    s.user_bvel = nullptr;
+    return EDGE_OK;
+  }
+  EDGE_REQUIRE(bvel, "edge_mg_set_solid: boundary velocity required with a mask");
+  for (int l = 0; l < s.L; ++l) {
+    if (!s.bm[l]) EDGE_CHECK_HIP(hipMalloc(&s.bm[l], s.nl[l]));
+    s.lv[l].bm = s.bm[l];
+  }
+  s.solid = true;
+  s.user_bm = bm;
+  s.user_bvel = bvel;
+  return EDGE_OK;
+}
+
+// MGPCG_3.Poisson / MGPCG_Solid_3.Poisson (sim/mgpcg.py:440-486,
+// sim/mgpcg_solid.py:418-427): project u in place; p_out (optional) gets the
+// pressure; f* (optional) get -(grad p) (Poisson_with_buffer).
+// stop_mode 0: the reference criterion; 1: ||r|| <= rel_tol ||b||.
+// x0 (optional): warm start, used only when ||b - A x0|| <= ||b||.
+// check_every: CG iterations between host convergence checks.
+// stats (host double[9]): iters, rTr, ||b||^2, fluid div sum, corrected,
+// converged, z0.r0, ||b - A x0||^2, x0 used.
+int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, float* fx, float* fy, float* fz,
+                 int max_iters, int stop_mode, double abs_tol, double rel_tol, const float* x0, int check_every,
+                 double* stats, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(handle && ux && uy && uz, "edge_poisson: null argument");
+  EDGE_REQUIRE(check_every >= 1, "edge_poisson: check_every must be >= 1");
+  EDGE_REQUIRE(stop_mode == 0 || stop_mode == 1, "edge_poisson: bad stop_mode");
+  Solver& s = *(Solver*)handle;
+  hipStream_t st = (hipStream_t)stream;
+  Faces3 U{{ux, uy, uz}};
+  int64_t nf = (int64_t)(s.nx + 1) * s.ny * s.nz + (int64_t)s.nx * (s.ny + 1) * s.nz + (int64_t)s.nx * s.ny * (s.nz + 1);
+  int rc = set_scalars(s, st, max_iters, stop_mode, abs_tol, rel_tol);
+  if (rc) return rc;
+  rc = prepare_masks(s, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, s.nx, s.ny, s.nz, s.bt, U,
+                     s.solid ? s.user_bm : nullptr, s.solid ? s.user_bvel : nullptr);
+  if (s.solid)
+    hipLaunchKernelGGL(k_div_sum, dim3(NB), dim3(BS), 0, st, s.nx, s.ny, s.nz, U, (const int8_t*)s.bm[0], red(s));
+  hipLaunchKernelGGL(k_init, dim3(NB), dim3(BS), 0, st, s.nx, s.ny, s.nz, U, nullptr,
+                     s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s));
+  EDGE_CHECK_LAUNCH();
+  rc = start_solve(s, st, x0);
+  if (rc) return rc;
+  int cg = run_cg_any(s, st, max_iters, check_every, stats);
+  if (cg != EDGE_OK && cg != EDGE_ENOCONV) return cg;
+  // get_result + subtract_grad_p + apply_bc
+  hipLaunchKernelGGL(k_subtract_grad, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, s.nx, s.ny, s.nz, U,
+                     Faces3{{fx, fy, fz}}, (const float*)s.x);
+  hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, s.nx, s.ny, s.nz, s.bt, U,
+                     s.solid ? s.user_bm : nullptr, s.solid ? s.user_bvel : nullptr);
+  EDGE_CHECK_LAUNCH();
+  if (p_out) EDGE_CHECK_HIP(hipMemcpyAsync(p_out, s.x, s.N * 4, hipMemcpyDeviceToDevice, st));
+  if (cg == EDGE_ENOCONV) set_error("edge_poisson: no convergence within %d iterations", max_iters);
+  return cg;
+}
+
+// Solve A x = b (b recentred first when all walls), same options as edge_poisson.
+int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int stop_mode, double abs_tol,
+                  double rel_tol, const float* x0, int check_every, double* stats, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(handle && b && x, "edge_mg_solve: null argument");
+  EDGE_REQUIRE(check_every >= 1, "edge_mg_solve: check_every must be >= 1");
+  Solver& s = *(Solver*)handle;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = set_scalars(s, st, max_iters, stop_mode, abs_tol, rel_tol);
+  if (rc) return rc;
+  rc = prepare_masks(s, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_init, dim3(NB), dim3(BS), 0, st, s.nx, s.ny, s.nz, Faces3{}, b,
+                     s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s));
+  EDGE_CHECK_LAUNCH();
+  rc = start_solve(s, st, x0);
+  if (rc) return rc;
+  int cg = run_cg_any(s, st, max_iters, check_every, stats);
+  if (cg != EDGE_OK && cg != EDGE_ENOCONV) return cg;
+  EDGE_CHECK_HIP(hipMemcpyAsync(x, s.x, s.N * 4, hipMemcpyDeviceToDevice, st));
+  if (cg == EDGE_ENOCONV) set_error("edge_mg_solve: no convergence within %d iterations", max_iters);
+  return cg;
+}
+
+// One V-cycle z = M^-1 r (r used as given, not recentred) -- test hook.
+int edge_mg_vcycle(void* handle, const float* r, float* z, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(handle && r && z, "edge_mg_vcycle: null argument");
+  Solver& s = *(Solver*)handle;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = set_scalars(s, st, 1, 0, 0, 0);
+  if (rc) return rc;
+  rc = prepare_masks(s, st);
+  if (rc) return rc;
+  EDGE_CHECK_HIP(hipMemcpyAsync(s.r, r, s.N * 4, hipMemcpyDeviceToDevice, st));
+  if (s.mode == 1) {
+    Ops<__hip_bfloat16>::vcycle(s, st, 1);
+    hipLaunchKernelGGL(k_smooth_last_dot<__hip_bfloat16>, dim3(NB), dim3(BS), 0, st, s.lv[0], s.bt,
+                       (const __hip_bfloat16*)s.rl[0], s.r, (__hip_bfloat16*)s.zl[0], red(s), (int)F_NONE, 1);
+    hipLaunchKernelGGL(k_to_float<__hip_bfloat16>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N,
+                       (const __hip_bfloat16*)s.zl[0], z);
+  } else {
+    Ops<float>::vcycle(s, st, 1);
+    hipLaunchKernelGGL(k_smooth_last_dot<float>, dim3(NB), dim3(BS), 0, st, s.lv[0], s.bt, (const float*)s.rl[0], s.r,
+                       (float*)s.zl[0], red(s), (int)F_NONE, 1);
+    EDGE_CHECK_HIP(hipMemcpyAsync(z, s.zl[0], s.N * 4, hipMemcpyDeviceToDevice, st));
+  }
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+// Ap = A p (sim/mgpcg.py:119-124) -- test hook.
+int edge_mg_apply_A(void* handle, const float* p, float* Ap, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(handle && p && Ap, "edge_mg_apply_A: null argument");
+  Solver& s = *(Solver*)handle;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = prepare_masks(s, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_apply_A, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.lv[0], s.bt, p, Ap);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+// 1 when the mixed solver's dense coarsest solve is active (0: red-black bottom).
+int edge_mg_dense_bottom(void* handle) {
+  if (!handle) return 0;
+  return ((Solver*)handle)->dense_ok && ((Solver*)handle)->mode == 1 ? 1 : 0;
+}
+
+}  // extern "C"

@@ -29,7 +29,7 @@ SIGNATURES = {
     "edge_advance_points": "ipppppppliiiffp",
     "edge_compose_points": "pppppliiifp",
     "edge_calc_ed4_garm": "ppppppppppppppplfp",
-    "edge_calc_ed": "ipppppppppppplfp",
+    "edge_calc_ed": "ippppppppppplfp",
     "edge_offset_points": "pplfffp",
     "edge_eval_points": "ipiiipppppiiiplfppp",
     "edge_reset_identity": "pppppppppppiiip",
@@ -50,9 +50,21 @@ SIGNATURES = {
     "edge_fill": "pflp",
     "edge_vortex_ring": "fffpppppppiiiip",
     "edge_split_central_vector": "ppppiiip",
+    # projection
+    "edge_mg_create": "iiiiipp",
+    "edge_mg_destroy": "p",
+    "edge_mg_set_solid": "ppp",
+    "edge_poisson": "ppppppppiiddpipp",
+    "edge_mg_solve": "pppiiddpipp",
+    "edge_mg_vcycle": "pppp",
+    "edge_mg_apply_A": "pppp",
+    "edge_mg_dense_bottom": "p",
 }
 
 _RESTYPE = {"edge_last_error": ctypes.c_char_p}
+
+# statuses that are results rather than failures
+NOCONV = -4
 
 _lib = None
 
@@ -79,10 +91,10 @@ def last_error():
     return msg.decode() if msg else ""
 
 
-def call(name, *args):
-    """Call an exported function and raise RuntimeError on a non-zero status."""
+def call(name, *args, ok=(0,)):
+    """Call an exported function and raise RuntimeError on a status not in `ok`."""
     st = getattr(lib(), name)(*args)
-    if st != 0:
+    if st not in ok:
         raise RuntimeError(f"{name} failed (status {st}): {last_error()}")
     return st
 

@@ -1,0 +1,189 @@
+/*
+ * edge_hip.h -- C ABI of libedge_hip.so, the MI355X (gfx950) EDGE flow-map
+ * time step.
+ *
+ * Conventions (all entry points):
+ *  - Return 0 on success, else a negative status (-1 bad argument, -2 HIP
+ *    error, -3 out of memory, -4 no convergence); edge_last_error() gives the
+ *    thread-local message.  No exception crosses the ABI.
+ *  - Every pointer is a device pointer borrowed for the call, except where a
+ *    comment says "host".  `stream` is a hipStream_t (torch's
+ *    current_stream().cuda_stream); work is enqueued, not synchronised.
+ *  - Grids: nx, ny, nz cells; a field of n components is n contiguous planes
+ *    of nx*ny*nz floats, index (i*ny + j)*nz + k.  MAC faces: u_x
+ *    (nx+1, ny, nz), u_y (nx, ny+1, nz), u_z (nx, ny, nz+1).  A map row T_c
+ *    is grad psi_c (3 planes); dT_c holds [d2/dydz, d2/dxdz, d2/dxdy] psi_c.
+ *  - All arithmetic is FP32 unless a function says otherwise.
+ *
+ * The reference is Python + Taichi; each comment names the @ti.kernel (or
+ * Python function) an entry point replaces, as file:line in
+ * pfm-gatech/EDGE-Flow-Maps.
+ */
+#ifndef EDGE_HIP_H
+#define EDGE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library ------------------------------------------------------------ */
+const char* edge_last_error(void);
+int edge_abi_version(void);
+/* ti.sync() (util/timer.py:19,27) */
+int edge_stream_sync(void* stream);
+
+/* ---- Hermite tables and finite differences ------------------------------ */
+/* Packs the per-cell Hermite coefficients of a map (3*nx*ny*nz*8 floats,
+ * 16-byte aligned).  kind 0: garm, d2T derived from dT
+ * (sim/hermite.py:298-456 + 1258-1287); kind 1: full, d2T given
+ * (sim/hermite.py:459-607).  d2T* may be NULL for kind 0. */
+int edge_pack_hermite_table(int kind, const float* psi, const float* Tx, const float* Ty, const float* Tz,
+                            const float* dTx, const float* dTy, const float* dTz, const float* d2Tx,
+                            const float* d2Ty, const float* d2Tz, float* tab, int nx, int ny, int nz, float dx,
+                            void* stream);
+/* want_dT=1: calc_fd_3_from_grad (sim/hermite.py:1180-1255);
+ * want_dT=0: calc_fd_3_from_grad_grad (sim/hermite.py:1258-1287), T* ignored. */
+int edge_calc_fd3(int want_dT, const float* Tx, const float* Ty, const float* Tz, float* dTx, float* dTy,
+                  float* dTz, float* d2Tx, float* d2Ty, float* d2Tz, int nx, int ny, int nz, float dx,
+                  void* stream);
+
+/* ---- backward-map marches ----------------------------------------------- */
+/* advect_psi_garm_ed4_fusion_unroll (sim/hermite.py:1426-1521): reads the old
+ * map through its garm table, writes the new psi, T, dT (may alias the fields
+ * the table was built from). */
+int edge_march_edge(const float* ux, const float* uy, const float* uz, const float* tab_old, const float* X,
+                    float* psi, float* Tx, float* Ty, float* Tz, float* dTx, float* dTy, float* dTz, int nx, int ny,
+                    int nz, float dx, float dt, float eps, void* stream);
+/* advect_ps_hermite_ed8 (sim/hermite.py:1290-1338), fused; tab_old is a full table. */
+int edge_march_ed8(const float* ux, const float* uy, const float* uz, const float* tab_old, const float* X,
+                   float* psi, float* Tx, float* Ty, float* Tz, float* dTx, float* dTy, float* dTz, float* d2Tx,
+                   float* d2Ty, float* d2Tz, int nx, int ny, int nz, float dx, float dt, float eps, void* stream);
+/* advect_ps_garm (sim/hermite.py:1398-1421) up to calc_fd_3_from_grad; full table. */
+int edge_march_ge(const float* ux, const float* uy, const float* uz, const float* tab_old, const float* X,
+                  float* psi, float* Tx, float* Ty, float* Tz, int nx, int ny, int nz, float dx, float dt,
+                  void* stream);
+/* advect_ps_hermite_ed4_fd_unroll (sim/hermite.py:1375-1395), fused; table
+ * built from T by calc_fd_3_from_grad (kind 1 after edge_calc_fd3). */
+int edge_march_ed4(const float* ux, const float* uy, const float* uz, const float* tab_old, const float* X,
+                   float* psi, float* Tx, float* Ty, float* Tz, int nx, int ny, int nz, float dx, float dt,
+                   float eps, void* stream);
+/* march_phi_grid -> advance_map on the three face grids (run_edge.py:330-357). */
+int edge_advance_map_faces(const float* ux, const float* uy, const float* uz, float* phix, float* phiy, float* phiz,
+                           float* Fx, float* Fy, float* Fz, int nx, int ny, int nz, float dx, float dt, void* stream);
+/* mode 0: advance_map_0 (sim/hermite.py:786-803); mode 1:
+ * advance_map_1_transpose (sim/hermite.py:863-918).  pts: 3 planes of n. */
+int edge_advance_points(int mode, const float* ux, const float* uy, const float* uz, float* pts, float* Tx,
+                        float* Ty, float* Tz, long n, int nx, int ny, int nz, float dx, float dt, void* stream);
+/* interp_psi_hermite(_garm)(_fd_unroll) (sim/hermite.py:921-1046): compose n
+ * points (and their rows, if T* given) with a map table. */
+int edge_compose_points(const float* tab, float* pts, float* Tx, float* Ty, float* Tz, long n, int nx, int ny,
+                        int nz, float dx, void* stream);
+/* calc_ed4_garm (sim/hermite.py:1078-1095) on stored point rows. */
+int edge_calc_ed4_garm(const float* T0x, const float* T0y, const float* T0z, const float* T1x, const float* T1y,
+                       const float* T1z, const float* T2x, const float* T2y, const float* T2z, const float* T3x,
+                       const float* T3y, const float* T3z, float* dTx, float* dTy, float* dTz, long n, float eps,
+                       void* stream);
+/* np=8: calc_ed8_3 (sim/hermite.py:1098-1151); np=4: calc_ed4_1
+ * (sim/hermite.py:1154-1177).  pts: host array of np device pointers. */
+int edge_calc_ed(int np, const float* const* pts, float* psi, float* Tx, float* Ty, float* Tz, float* dTx, float* dTy,
+                 float* dTz, float* d2Tx, float* d2Ty, float* d2Tz, long n, float eps, void* stream);
+/* dst = src + (ox, oy, oz): the eps-point resets (sim/hermite.py:704-783). */
+int edge_offset_points(const float* src, float* dst, long n, float ox, float oy, float oz, void* stream);
+/* Test hooks for the @ti.func interpolators: what 0 interp_(grad_)2 of one
+ * field with BL (host float[3]) (util/taichi_utils.py:254-319); 1
+ * interp_u_MAC(_grad) (322-341); 2 a map table's value (+rows).  pts: n x 3
+ * interleaved. */
+int edge_eval_points(int what, const float* f, int d0, int d1, int d2, const float* BL, const float* ux,
+                     const float* uy, const float* uz, const float* tab, int nx, int ny, int nz, const float* pts,
+                     long n, float dx, float* val, float* grad, void* stream);
+/* reset_to_identity_1/2/3 (sim/hermite.py:642-701); NULL outputs are skipped. */
+int edge_reset_identity(const float* X, float* psi, float* Tx, float* Ty, float* Tz, float* dTx, float* dTy,
+                        float* dTz, float* d2Tx, float* d2Ty, float* d2Tz, int nx, int ny, int nz, void* stream);
+/* reset_to_identity on the faces (run_edge.py:306-320). */
+int edge_reset_faces(float* phix, float* phiy, float* phiz, float* Fx, float* Fy, float* Fz, int nx, int ny, int nz,
+                     float dx, void* stream);
+/* center/x/y/z_coords_func (util/taichi_utils.py:23-44): mode 0 cells, 1..3 faces. */
+int edge_coords(int mode, float* out, int nx, int ny, int nz, float dx, void* stream);
+
+/* ---- step-loop driver kernels ------------------------------------------- */
+/* map_u (run_edge.py:360-373). */
+int edge_map_u(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+               const float* Fx, const float* Fy, const float* Fz, const float* phix, const float* phiy,
+               const float* phiz, int nx, int ny, int nz, float dx, void* stream);
+/* map_u_psi (run_edge.py:376-417); tab: garm table of the cell map. */
+int edge_map_u_psi(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+                   const float* tab, int nx, int ny, int nz, float dx, void* stream);
+/* split_psi_hermite (run_ed8.py:190-223, run_ge.py:184-217); tab: full table. */
+int edge_split_psi_hermite(const float* tab, float* phix, float* phiy, float* phiz, float* Fx, float* Fy, float* Fz,
+                           int nx, int ny, int nz, float dx, void* stream);
+/* mode 0 map_smoke (run_edge.py:420-427), 1 clamp_smoke (469-477); nc 1 or 4. */
+int edge_map_smoke(int mode, const float* s0, float* s1, int nc, const float* phix, const float* phiy,
+                   const float* phiz, int nx, int ny, int nz, float dx, void* stream);
+/* mode 0 map_smoke_psi (run_edge.py:443-466), 1 clamp_smoke_psi (480-505). */
+int edge_map_smoke_psi(int mode, const float* s0, float* s1, int nc, const float* tab, int nx, int ny, int nz,
+                       float dx, void* stream);
+/* clamp_u (run_edge.py:508-519) on the three face grids. */
+int edge_clamp_u(const float* ux, const float* uy, const float* uz, const float* u1x, const float* u1y,
+                 const float* u1z, float* u2x, float* u2y, float* u2z, int nx, int ny, int nz, void* stream);
+/* advect_u_semi_lagrangian (sim/hermite.py:1578-1621); dt is the full argument. */
+int edge_advect_u_sl(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+                     int nx, int ny, int nz, float dx, float dt, void* stream);
+/* calc_max_speed (run_edge.py:295-303) into one device float. */
+int edge_max_speed(const float* ux, const float* uy, const float* uz, float* out_dev, int nx, int ny, int nz,
+                   void* stream);
+/* add_fields (util/taichi_utils.py:12-15): dest = f1 + m f2. */
+int edge_axpy(const float* f1, const float* f2, float* dest, float m, long n, void* stream);
+/* scale_field (util/taichi_utils.py:6-9). */
+int edge_scale(const float* a, float alpha, float* r, long n, void* stream);
+/* Taichi Field.copy_from / fill (run_edge.py:621-624, 644). */
+int edge_copy(const float* src, float* dst, long n, void* stream);
+int edge_fill(float* p, float v, long n, void* stream);
+/* add_vortex_ring_and_smoke (sim/init_conditions.py:13-36); c, ex, ey, color: host float[3]. */
+int edge_vortex_ring(float w, float rad, float delta, const float* c, const float* ex, const float* ey,
+                     const float* pf, float* vf, float* smokef, const float* color, int num_samples, int nx, int ny,
+                     int nz, void* stream);
+/* split_central_vector (util/taichi_utils.py:103-116). */
+int edge_split_central_vector(const float* vc, float* ux, float* uy, float* uz, int nx, int ny, int nz,
+                              void* stream);
+
+/* ---- pressure projection (MGPCG) ----------------------------------------- */
+/* Solver for an nx*ny*nz cell grid, `levels` multigrid levels (dims divisible
+ * by 2^(levels-1), nz even on every level).  mode 0 plain: FP32 storage and
+ * the reference V-cycle (sim/mgpcg.py:23-57,147-195); mode 1 mixed: BF16
+ * level storage, FP32 arithmetic, dense FP64 coarsest solve.  bt: host
+ * int[6] boundary_types ([axis][side], 2 = wall) or NULL for all walls.
+ * Replaces MGPCG_3.__init__ (sim/mgpcg.py:366-371) and MGPCG_Solid_3.__init__
+ * (sim/mgpcg_solid.py:315-324).  The handle owns its hierarchy. */
+int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, void** handle);
+int edge_mg_destroy(void* handle);
+/* Solid cells (sim/mgpcg_solid.py): int32 mask (>0 solid) and boundary
+ * velocity (3 planes), borrowed until changed; NULL mask clears. */
+int edge_mg_set_solid(void* handle, const int* bm, const float* bvel);
+/* MGPCG_3.Poisson / Poisson_with_buffer (sim/mgpcg.py:440-486) and
+ * MGPCG_Solid_3.Poisson (sim/mgpcg_solid.py:418-427): u projected in place;
+ * p_out, f* optional.  stop_mode 0 = the reference's test (plain rTr <
+ * max(abs_tol, rel_tol z0'r0), sim/mgpcg.py:228,254; solid rTr < abs_tol,
+ * sim/mgpcg_solid.py:281), 1 = ||r|| <= rel_tol ||b|| (FP64 norms).  x0
+ * optional warm start (kept only if ||b - A x0|| <= ||b||).  check_every:
+ * iterations between host convergence checks.  stats: host double[9]
+ * {iters, rTr, ||b||^2, fluid div sum, corrected, converged, z0'r0,
+ * ||b - A x0||^2, x0 used}.  Returns -4 (results written) when max_iters ran
+ * out. */
+int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, float* fx, float* fy, float* fz,
+                 int max_iters, int stop_mode, double abs_tol, double rel_tol, const float* x0, int check_every,
+                 double* stats, void* stream);
+/* A x = b with the same solver (MGPCG.solve, sim/mgpcg.py:197-283, after init). */
+int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int stop_mode, double abs_tol,
+                  double rel_tol, const float* x0, int check_every, double* stats, void* stream);
+/* Test hooks: one V-cycle z = M^-1 r (apply_preconditioner, sim/mgpcg.py:177-195)
+ * and Ap = A p (get_Ap, sim/mgpcg.py:119-124). */
+int edge_mg_vcycle(void* handle, const float* r, float* z, void* stream);
+int edge_mg_apply_A(void* handle, const float* p, float* Ap, void* stream);
+/* 1 if the mixed solver uses the dense coarsest solve (0: red-black bottom). */
+int edge_mg_dense_bottom(void* handle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDGE_HIP_H */

@@ -1,0 +1,261 @@
+"""GPU parity of the advection kernels against the CPU oracle (FP32, and FP64
+where a quantity is ill-conditioned).  Tolerances are written per test:
+  * 1e-5 of the field's scale for FP32 quantities computed in a different
+    summation order (the reference's 64-tap loops vs the 27-tap kernels);
+  * for eps-differences (dT), the GPU's error against the FP64 oracle must be
+    within a small factor of the FP32 oracle's own error against FP64.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import (any_points, cell_coords, identity_plus, interior_points, linear_velocity, smooth_velocity)
+from oracle.oracle import HK_FD, HK_FULL, HK_GARM_FD
+
+pytestmark = pytest.mark.gpu
+
+N3 = (24, 20, 16)
+DX = 1.0 / 20
+
+
+def T_(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+
+
+def N_(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def scale_err(gpu, ref):
+    ref = np.asarray(ref, np.float64)
+    return float(np.max(np.abs(np.asarray(gpu, np.float64) - ref)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+def realistic_map(orc, n, dx, steps=2, seed=0, amp=1.0):
+    """A map produced by the oracle's own EDGE march from identity."""
+    u = smooth_velocity(*n, dx, seed=seed, amp=amp)
+    X = cell_coords(*n, dx)
+    psi = X.copy()
+    T = np.zeros((3, 3) + n)
+    for c in range(3):
+        T[c, c] = 1
+    dT = np.zeros_like(T)
+    for _ in range(steps):
+        psi, T, dT = orc.march_edge(psi, T, dT, u, X, dx, 0.5 * dx / amp, 0.025 * dx)
+    return u, X, psi, T, dT
+
+
+def test_bspline_points(orc32, dev):
+    from edge_hip import ops
+    rng = np.random.default_rng(0)
+    f = rng.normal(size=(N3[0] + 1, N3[1], N3[2]))
+    pts = any_points(*N3, DX, 5000, seed=1)
+    for BL in [(0, .5, .5), (.5, .5, .5)]:
+        v, g = ops.eval_bspline(T_(f, dev), BL, T_(pts, dev), DX, grad=True)
+        rv, rg = orc32.eval_bspline(f, BL, pts, DX, grad=True)
+        assert scale_err(N_(v), rv) < 1e-5
+        assert scale_err(N_(g), rg) < 1e-5
+
+
+def test_umac_points(orc32, dev):
+    from edge_hip import ops
+    u = smooth_velocity(*N3, DX, seed=2)
+    pts = any_points(*N3, DX, 5000, seed=3)
+    v, J = ops.eval_umac([T_(c, dev) for c in u], T_(pts, dev), DX, grad=True)
+    rv, rJ = orc32.eval_umac(u, pts, DX, grad=True)
+    assert scale_err(N_(v), rv) < 1e-5
+    assert scale_err(N_(J), rJ) < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["garm", "full", "fd"])
+def test_hermite_table_points(orc32, dev, kind):
+    from edge_hip import ops
+    pm = identity_plus(*N3, DX, amp=0.3, seed=4)
+    psi, T, dT, d2T = pm.tables(*N3, DX)
+    rng = np.random.default_rng(5)
+    dT = dT + 0.1 * rng.normal(size=dT.shape)  # arbitrary derivative data
+    d2T = d2T + 0.1 * rng.normal(size=d2T.shape)
+    pts = any_points(*N3, DX, 5000, seed=6)
+    tab = ops.pack_table(kind, T_(psi, dev), T_(T, dev), T_(dT, dev), T_(d2T, dev), DX)
+    v, G = ops.eval_map(tab, N3, T_(pts, dev), DX, grad=True)
+    hk = {"garm": HK_GARM_FD, "full": HK_FULL, "fd": HK_FD}[kind]
+    rv, rG = orc32.eval_hermite(hk, psi, T, dT, d2T, pts, DX, grad=True)
+    assert scale_err(N_(v), rv) < 2e-6
+    assert scale_err(N_(G), rG) < 1e-5
+
+
+def test_point_marches(orc32, dev):
+    from edge_hip import ops
+    u = smooth_velocity(*N3, DX, seed=7)
+    pts = any_points(*N3, DX, 4000, seed=8).T.copy()
+    gu = [T_(c, dev) for c in u]
+    p0 = T_(pts, dev)
+    ops.advance_points(0, gu, p0, None, DX, -0.03)
+    ref = orc32.advance_map_0(pts, u, DX, -0.03)
+    assert scale_err(N_(p0), ref) < 1e-6
+    Tm = np.broadcast_to(np.eye(3)[:, :, None], (3, 3, pts.shape[1])).copy()
+    p1, t1 = T_(pts, dev), T_(Tm, dev)
+    ops.advance_points(1, gu, p1, t1, DX, -0.03)
+    rp, rT = orc32.advance_map_1_transpose(pts, Tm, u, DX, -0.03)
+    assert scale_err(N_(p1), rp) < 1e-6
+    assert np.max(np.abs(N_(t1) - rT)) < 1e-5
+
+
+def test_face_march(orc32, dev):
+    from edge_hip import ops
+    from helpers import face_coords
+    u = smooth_velocity(*N3, DX, seed=9)
+    gu = [T_(c, dev) for c in u]
+    phi = [face_coords(a, *N3, DX) for a in range(3)]
+    F = [np.broadcast_to(np.eye(3)[a][:, None, None, None], phi[a].shape).copy() for a in range(3)]
+    gphi, gF = [T_(p, dev) for p in phi], [T_(f, dev) for f in F]
+    ops.advance_faces(gu, gphi, gF, DX, 0.02)
+    for a in range(3):
+        rphi, rF = orc32.advance_map_col(phi[a], F[a], u, DX, 0.02)
+        assert scale_err(N_(gphi[a]), rphi) < 1e-6
+        assert np.max(np.abs(N_(gF[a]) - rF)) < 1e-5
+
+
+def test_edge_march_parity(orc32, orc64, dev):
+    from edge_hip import ops
+    n, dx = (32, 24, 20), 1.0 / 24
+    u, X, psi, T, dT = realistic_map(orc64, n, dx, steps=2, seed=10)
+    dt, eps = 0.5 * dx, 0.025 * dx
+    r32 = orc32.march_edge(psi, T, dT, u, X, dx, dt, eps)
+    r64 = orc64.march_edge(psi, T, dT, u, X, dx, dt, eps)
+    tab = ops.pack_table("garm", T_(psi, dev), T_(T, dev), T_(dT, dev), None, dx)
+    gp, gT, gdT = torch.empty((3,) + n, device=dev), torch.empty((3, 3) + n, device=dev), \
+        torch.empty((3, 3) + n, device=dev)
+    ops.march_edge([T_(c, dev) for c in u], tab, T_(X, dev), gp, gT, gdT, dx, dt, eps)
+    assert scale_err(N_(gp), r32[0]) < 1e-6
+    assert np.max(np.abs(N_(gT) - r32[1])) < 2e-5
+    # dT is an eps-difference: judge against FP64 with the FP32 oracle's own error as the yardstick
+    e_orc = np.max(np.abs(r32[2] - r64[2]))
+    e_gpu = np.max(np.abs(N_(gdT) - r64[2]))
+    assert e_gpu <= 4 * e_orc + 1e-3 * np.max(np.abs(r64[2])), (e_gpu, e_orc)
+
+
+def test_driver_maps_parity(orc32, orc64, dev):
+    from edge_hip import ops
+    n, dx = (32, 24, 20), 1.0 / 24
+    u, X, psi, T, dT = realistic_map(orc64, n, dx, steps=2, seed=11)
+    gu = [T_(c, dev) for c in u]
+    tab = ops.pack_table("garm", T_(psi, dev), T_(T, dev), T_(dT, dev), None, dx)
+    # map_u_psi
+    out = ops.map_u_psi(gu, tab, dx)
+    ref = orc32.map_u_psi(u, psi, T, dT, dx)
+    for a in range(3):
+        assert scale_err(N_(out[a]), ref[a]) < 1e-5
+    # smoke maps and clamps through the map
+    rng = np.random.default_rng(12)
+    s0 = rng.uniform(0, 1, size=(4,) + n)
+    s1 = rng.uniform(-0.5, 1.5, size=(4,) + n)
+    for mode in (0, 1):
+        g = ops.map_smoke_psi(mode, T_(s0, dev), T_(s1, dev) if mode else None, tab, dx)
+        r = orc32.map_smoke_psi(mode, s0, s1 if mode else None, psi, T, dT, dx)
+        assert scale_err(N_(g), r) < 1e-5
+    # face maps phi/F from an oracle face march, then map_u / map_smoke / clamp_smoke
+    from helpers import face_coords
+    phi = [face_coords(a, *n, dx) for a in range(3)]
+    F = [np.broadcast_to(np.eye(3)[a][:, None, None, None], phi[a].shape).copy() for a in range(3)]
+    for a in range(3):
+        phi[a], F[a] = orc64.advance_map_col(phi[a], F[a], u, dx, 0.3 * dx)
+    gphi, gF = [T_(p, dev) for p in phi], [T_(f, dev) for f in F]
+    out = ops.map_u(gu, gF, gphi, dx)
+    ref = orc32.map_u(u, F, phi, dx)
+    for a in range(3):
+        assert scale_err(N_(out[a]), ref[a]) < 1e-5
+    for mode in (0, 1):
+        g = ops.map_smoke(mode, T_(s0, dev), T_(s1, dev) if mode else None, gphi, dx)
+        r = orc32.map_smoke(mode, s0, s1 if mode else None, phi, dx)
+        assert scale_err(N_(g), r) < 1e-5
+    # clamp_u, semi-Lagrangian, max speed
+    u1 = smooth_velocity(*n, dx, seed=13)
+    gc = ops.clamp_u(gu, [T_(c, dev) for c in u1])
+    for a in range(3):
+        assert np.array_equal(N_(gc[a]).astype(np.float32), orc32.clamp_u(u[a], u1[a]))
+    sl = ops.advect_u_sl(gu, dx, 0.25 * dx)
+    rsl = orc32.advect_u_sl(u, dx, 0.25 * dx)
+    for a in range(3):
+        assert scale_err(N_(sl[a]), rsl[a]) < 1e-5
+    assert ops.max_speed(gu) == pytest.approx(orc32.max_speed(u), rel=1e-6)
+
+
+def test_split_psi_and_variant_marches(orc32, orc64, dev):
+    """ED8 (sim/hermite.py:1290-1338), GE (1398-1424), ED4 (1375-1395) and
+    split_psi_hermite (run_ed8.py:190-223) against the oracle's unfused sequence."""
+    from edge_hip import ops, _lib
+    from edge_hip.field import stream_handle
+    n, dx = (24, 20, 16), 1.0 / 20
+    u = smooth_velocity(*n, dx, seed=14)
+    pm = identity_plus(*n, dx, amp=0.2, seed=15)
+    psi, T, dT, d2T = pm.tables(*n, dx)
+    X = cell_coords(*n, dx)
+    dt, eps = 0.4 * dx, 0.025 * dx
+    gu = [T_(c, dev) for c in u]
+    s = stream_handle()
+    # --- ED8 fused vs oracle: reset, 8x advance_map_0, 8x interp_psi_hermite, calc_ed8_3
+    P8 = []
+    for e in range(8):
+        sg = np.array([1 if e & 1 else -1, 1 if e & 2 else -1, 1 if e & 4 else -1], float)
+        p = (X + eps * sg.reshape(3, 1, 1, 1)).reshape(3, -1)
+        p = orc64.advance_map_0(p, u, dx, -dt)
+        p, _ = orc64.compose(HK_FULL, psi, T, dT, d2T, p, None, dx)
+        P8.append(p.reshape((3,) + n))
+    r = orc64.calc_ed8(np.stack(P8), eps)
+    tab = ops.pack_table("full", T_(psi, dev), T_(T, dev), T_(dT, dev), T_(d2T, dev), dx)
+    out = [torch.empty((3,) + n, device=dev), torch.empty((3, 3) + n, device=dev),
+           torch.empty((3, 3) + n, device=dev), torch.empty((3,) + n, device=dev)]
+    _lib.call("edge_march_ed8", *[c.data_ptr() for c in gu], tab.data_ptr(), T_(X, dev).data_ptr(),
+              out[0].data_ptr(), *[out[1][c].data_ptr() for c in range(3)], *[out[2][c].data_ptr() for c in range(3)],
+              *[out[3][c].data_ptr() for c in range(3)], *n, float(dx), float(dt), float(eps), s)
+    assert scale_err(N_(out[0]), r[0]) < 1e-6
+    assert np.max(np.abs(N_(out[1]) - r[1])) < 5e-3  # /eps of FP32 positions
+    # --- GE: march + compose (full) then FD derivatives
+    p = X.reshape(3, -1).copy()
+    Tm = np.broadcast_to(np.eye(3)[:, :, None], (3, 3, p.shape[1])).copy()
+    p, Tm = orc64.advance_map_1_transpose(p, Tm, u, dx, -dt)
+    p, Tm = orc64.compose(HK_FULL, psi, T, dT, d2T, p, Tm, dx)
+    rp, rT = p.reshape((3,) + n), Tm.reshape((3, 3) + n)
+    rdT, rd2T = orc64.fd3(rT, None, dx, want_dT=True)
+    gp, gT = torch.empty((3,) + n, device=dev), torch.empty((3, 3) + n, device=dev)
+    _lib.call("edge_march_ge", *[c.data_ptr() for c in gu], tab.data_ptr(), T_(X, dev).data_ptr(), gp.data_ptr(),
+              *[gT[c].data_ptr() for c in range(3)], *n, float(dx), float(dt), s)
+    gdT, gd2T = ops.calc_fd3(gT, dx)
+    assert scale_err(N_(gp), rp) < 1e-6
+    assert np.max(np.abs(N_(gT) - rT)) < 2e-5
+    assert scale_err(N_(gdT), rdT) < 1e-3
+    # --- ED4 (fd variant)
+    S4 = [(-1, -1, -1), (1, 1, -1), (-1, 1, 1), (1, -1, 1)]
+    P4 = []
+    for sg in S4:
+        q = (X + eps * np.array(sg, float).reshape(3, 1, 1, 1)).reshape(3, -1)
+        q = orc64.advance_map_0(q, u, dx, -dt)
+        q, _ = orc64.compose(HK_FD, psi, T, None, None, q, None, dx)
+        P4.append(q.reshape((3,) + n))
+    r4 = orc64.calc_ed4_1(np.stack(P4), eps)
+    tab4 = ops.pack_table("fd", T_(psi, dev), T_(T, dev), None, None, dx)
+    gp4, gT4 = torch.empty((3,) + n, device=dev), torch.empty((3, 3) + n, device=dev)
+    _lib.call("edge_march_ed4", *[c.data_ptr() for c in gu], tab4.data_ptr(), T_(X, dev).data_ptr(),
+              gp4.data_ptr(), *[gT4[c].data_ptr() for c in range(3)], *n, float(dx), float(dt), float(eps), s)
+    assert scale_err(N_(gp4), r4[0]) < 1e-6
+    assert np.max(np.abs(N_(gT4) - r4[1])) < 5e-3
+    # --- split_psi_hermite (full)
+    phi, F = ops.split_psi_hermite(tab, n, dx, dev)
+    rphi, rF = orc32.split_psi_hermite(HK_FULL, psi, T, dT, d2T, dx)
+    for a in range(3):
+        assert scale_err(N_(phi[a]), rphi[a]) < 1e-6
+        assert np.max(np.abs(N_(F[a]) - rF[a])) < 2e-5
+
+
+def test_vortex_ring_init(orc32, dev):
+    from edge_hip import ops
+    n, dx = (32, 16, 16), 1.0 / 16
+    X = cell_coords(*n, dx)
+    vf, sm = np.zeros((3,) + n, np.float32), np.zeros((4,) + n, np.float32)
+    args = (0.02, 0.2, 0.03, (0.6, 0.5, 0.5), (0, 0, -1), (0, 1, 0))
+    orc32.vortex_ring(*args, X, vf, sm, (1, 0, 0), 500)
+    gv, gs = torch.zeros((3,) + n, device=dev), torch.zeros((4,) + n, device=dev)
+    ops.vortex_ring(*args, T_(X, dev), gv, gs, (1, 0, 0), 500)
+    assert scale_err(N_(gv), vf) < 1e-4
+    assert np.mean(N_(gs) != sm) < 1e-3  # threshold flips only at |s - 0.002| ~ rounding
