@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark of the EDGE flow-map time step on one MI355X.
+
+Workload: 256x128x128 cells, four seeded vortex rings, reinit every 12 steps,
+CFL 0.5, BFECC clamp, projection to ||r|| <= 1e-6 ||b|| (FP64 norms).  A step
+is one iteration of the reference's loop (run_edge.py:588-711): semi-Lagrangian
+half step + projection, EDGE march, face march, BFECC pull-back, projection.
+
+Prints ONE JSON line: ms/step of the mixed-precision path (value), the plain
+FP32 path beside it, Poisson ms and iterations of both on the same right-hand
+sides, per-phase times, the roofline of the dominant kernel (HIP events on the
+launch stream) and the CPU baseline (the oracle restatement, timed on this
+host's cores on one 64x32x32 step and scaled by the cell count).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--res NX NY NZ]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "edge-flow-maps_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+# algorithmic work per unit (minimal separable counts; FMA = 2 flops; bytes =
+# compulsory FP32 reads + writes)
+MARCH_FLOP_PER_CELL = 33600.0      # fused EDGE march: 5 points x (RK4 with Jacobian + Hermite compose)
+MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0}  # one CG iteration
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--res", type=int, nargs=3, default=[256, 128, 128])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
+    ap.add_argument("--poisson-reps", type=int, default=3)
+    return ap.parse_args()
+
+
+def phase_times(sim):
+    """Per-phase ms of one step from its recorded marks."""
+    marks = sim.timing.pop("_marks")
+    out = {}
+    for (n0, e0), (n1, e1) in zip(marks[:-1], marks[1:]):
+        out[n0] = out.get(n0, 0.0) + e0.elapsed_time(e1)
+    return out
+
+
+def run_steps(sim, k, torch):
+    """k timed steps; returns wall ms per step (list) and phase ms (list of dicts)."""
+    walls, phases = [], []
+    for _ in range(k):
+        sim.timing = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.substep()
+        torch.cuda.synchronize()
+        walls.append((time.perf_counter() - t0) * 1e3)
+        phases.append(phase_times(sim))
+        sim.timing = None
+    return walls, phases
+
+
+def poisson_compare(sim, reps, torch):
+    """Plain vs mixed projection of the same velocity (copies), to 1e-6 relative."""
+    from sim.mgpcg import MGPCG_3
+    import numpy as np
+    res = {}
+    base = [sim.u_x.data.clone(), sim.u_y.data.clone(), sim.u_z.data.clone()]
+    bt = np.full((3, 2), 2, np.int32)
+    N = [sim.res_x, sim.res_y, sim.res_z]
+    for mode in ("plain", "mixed"):
+        solver = MGPCG_3(bt, N, base_level=3, mode=mode, stop="relative", rel_tol=1e-6)
+        times, iters = [], []
+        for r in range(reps + 1):
+            for f, b in zip((sim.tmp_u_x, sim.tmp_u_y, sim.tmp_u_z), base):
+                f.data.copy_(b)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            solver.Poisson(sim.tmp_u_x, sim.tmp_u_y, sim.tmp_u_z)
+            e1.record()
+            torch.cuda.synchronize()
+            if r > 0:
+                times.append(e0.elapsed_time(e1))
+                iters.append(solver.stats["iters"])
+        res[mode] = dict(ms=statistics.median(times), iters=int(statistics.median(iters)),
+                         converged=bool(solver.stats["converged"]))
+        del solver
+    return res
+
+
+def cpu_baseline(torch):
+    """The oracle (CPU restatement) of one full step at 64x32x32, scaled x(cells) to 256x128x128."""
+    from run_edge import Simulation
+    from oracle.step import OracleStep
+    import numpy as np
+    small = Simulation("cfg2", solver_mode="plain", res=(64, 32, 32), num_samples=500)
+    small.initialize()
+    u = small.velocity_numpy()
+    smoke = np.moveaxis(small.smoke.to_numpy(), -1, 0)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    st = OracleStep(u, smoke, small.dx, 12, 0.5, small.ed_eps, levels=3, BFECC_clamp=True)
+    t0 = time.perf_counter()
+    st.substep()
+    dt = time.perf_counter() - t0
+    scale = (256 * 128 * 128) / (64 * 32 * 32)
+    return dict(value=round(dt * 1e3 * scale, 1), unit="ms/step", cores=threads, kind="port",
+                sample=f"one full oracle step (advection + 2 projections, FP32, OpenMP) at 64x32x32 = "
+                       f"{dt:.2f} s, scaled x{scale:.0f} by cell count to 256x128x128")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
+                                   "(multi-GPU slab decomposition is a design note only)"}))
+        sys.exit(2)
+    import torch
+    from run_edge import Simulation
+
+    sim = Simulation("cfg2", solver_mode="mixed", stop="relative", rel_tol=1e-6, res=tuple(a.res), check_every=4)
+    sim.initialize()
+    for _ in range(a.warmup):
+        sim.substep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    walls, phases = run_steps(sim, a.steps, torch)
+    torch.cuda.synchronize()
+    total_ms = (time.perf_counter() - t0) * 1e3
+    ms_step = total_ms / a.steps
+    ph = {k: statistics.median(p.get(k, 0.0) for p in phases) for k in phases[0]}
+    pstats = sim.poisson_stats[-2 * a.steps:]
+    mixed_iters = [s["iters"] for s in pstats]
+
+    plain = None
+    if not a.no_plain:
+        from sim.mgpcg import MGPCG_3
+        import numpy as np
+        mixed_solver = sim.solver
+        sim.solver = MGPCG_3(np.full((3, 2), 2, np.int32), [sim.res_x, sim.res_y, sim.res_z], base_level=3,
+                             mode="plain", stop="relative", rel_tol=1e-6)
+        run_steps(sim, 1, torch)  # warm the plain solver
+        pw, pph = run_steps(sim, a.steps, torch)
+        plain = dict(ms_per_step=round(statistics.mean(pw), 3),
+                     phases_ms={k: round(statistics.median(p.get(k, 0.0) for p in pph), 3) for k in pph[0]},
+                     poisson_iters=[s["iters"] for s in sim.poisson_stats[-2 * a.steps:]])
+        sim.solver = mixed_solver
+
+    pcmp = poisson_compare(sim, a.poisson_reps, torch)
+
+    ncell = sim.res_x * sim.res_y * sim.res_z
+    march_ms = ph.get("march", float("nan"))
+    march_tflops = MARCH_FLOP_PER_CELL * ncell / (march_ms * 1e-3) / 1e12
+    # the projection's CG iteration, HBM-bound: bytes per iteration / ms per iteration
+    it_ms = pcmp["mixed"]["ms"] / max(pcmp["mixed"]["iters"], 1)
+    mg_gbs = MG_BYTES_PER_CELL["mixed"] * ncell / (it_ms * 1e-3) / 1e9
+    roof_march = dict(bound="valu", achieved=round(march_tflops, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                      frac=round(march_tflops / FP32_PEAK_TFLOPS, 4), traffic=None, kernel="k_march_edge",
+                      time_ms=round(march_ms, 3))
+    roof_mg = dict(bound="hbm", achieved=round(mg_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                   frac=round(mg_gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="mixed CG iteration",
+                   time_ms=round(it_ms, 4))
+
+    cpu = None if a.no_cpu else cpu_baseline(torch)
+
+    line = {
+        "metric": "ms/step @256x128x128; Poisson ms+iters to 1e-6 rel, learned/mixed vs plain",
+        "value": round(ms_step, 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: four seeded vortex rings (Biot-Savart init on the GPU)",
+        "config": {"workload": f"EDGE step {a.res[0]}x{a.res[1]}x{a.res[2]} four rings, mixed projection",
+                   "cells": ncell, "reinit_every": sim.reinit_every, "poisson_stop": "||r|| <= 1e-6 ||b||",
+                   "solver": "mixed (BF16 multigrid, FP64 dots, dense FP64 coarsest)"},
+        "phases_ms": {k: round(v, 3) for k, v in ph.items()},
+        "poisson_iters_mixed": mixed_iters,
+        "plain": plain,
+        "poisson_same_rhs": pcmp,
+        "roofline": roof_march if march_ms >= it_ms * pcmp["mixed"]["iters"] else roof_mg,
+        "roofline_march": roof_march,
+        "roofline_cg_iteration": roof_mg,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,253 @@
+"""Drop-in for the reference's run_edge.py: the EDGE step loop on MI355X.
+
+`Simulation.substep()` is one iteration of the reference's `while True` loop
+(run_edge.py:588-711): max speed -> dt, reinit every `reinit_every` steps,
+semi-Lagrangian half-step + projection, EDGE march of the backward map,
+forward face march, impulse/smoke pull-back with BFECC, clamps, projection.
+Every kernel is a libedge_hip.so kernel; this file only sequences them.
+
+The case table keeps the reference's cases 0 and 1 (run_edge.py:25-60) and
+adds the benchmark configurations ("cfg1": 64x32x32 leapfrogging rings,
+"cfg2": 256x128x128 four seeded rings).  Where the reference copies a field
+whose source is dead afterwards (u <- tmp_u), the storages are exchanged
+instead (Field.swap_storage); the values are identical.
+
+Usage: python run_edge.py -c cfg1 [--steps N] [--solver plain|mixed]
+"""
+import argparse
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+if _HERE not in sys.path:
+    sys.path.insert(0, _HERE)
+
+from edge_hip.field import Field, ti  # noqa: E402
+from sim.edge_driver import (calc_max_speed, clamp_smoke_psi, clamp_u3, map_smoke, map_smoke_psi, map_u,  # noqa: E402
+                             map_u_psi, march_phi_grid, reset_to_identity)
+from sim.hermite import advect_psi_garm_ed4_fusion_unroll, advect_u_semi_lagrangian, reset_to_identity_2  # noqa
+from sim.init_conditions import init_four_vorts, init_rings, init_vorts_leapfrog, seeded_rings  # noqa: E402
+from sim.mgpcg import MGPCG_3  # noqa: E402
+from util.taichi_utils import (add_fields, center_coords_func, scale_field, split_central_vector,  # noqa: E402
+                               x_coords_func, y_coords_func, z_coords_func)
+
+CASES = {
+    # reference case 0 (run_edge.py:25-42)
+    0: dict(init="leapfrog", res=(384, 128, 128), visualize_dt=0.1, reinit_every=20, CFL=0.5, BFECC_clamp=False,
+            eps_frac=0.01, base_level=3),
+    # reference case 1 (run_edge.py:43-60)
+    1: dict(init="fourvorts", res=(128, 128, 256), visualize_dt=0.1, reinit_every=20, CFL=0.5, BFECC_clamp=True,
+            eps_frac=0.025, base_level=3),
+    # 64x32x32, dx = 1/32, two coaxial rings R=0.2, core 0.03, strength 1, gap 0.15, reinit 12, 4 MG levels
+    "cfg1": dict(init="rings2", res=(64, 32, 32), visualize_dt=1e9, reinit_every=12, CFL=0.5, BFECC_clamp=False,
+                 eps_frac=0.01, levels=4),
+    # 256x128x128, dx = 1/128, four seeded rings R=0.15, core 0.025, strength 0.02, reinit 12
+    "cfg2": dict(init="rings4", res=(256, 128, 128), visualize_dt=1e9, reinit_every=12, CFL=0.5, BFECC_clamp=True,
+                 eps_frac=0.025, base_level=3),
+}
+
+
+class Simulation:
+    """The fields of run_edge.py:174-254 and one step of its loop."""
+
+    def __init__(self, case="cfg1", solver_mode="plain", stop="relative", rel_tol=1e-6, res=None,
+                 reinit_every=None, check_every=None, num_samples=2000, seed=0):
+        cfg = dict(CASES[case])
+        if res is not None:
+            cfg["res"] = tuple(res)
+        if reinit_every is not None:
+            cfg["reinit_every"] = reinit_every
+        self.cfg = cfg
+        ti.init()
+        self.res_x, self.res_y, self.res_z = rx, ry, rz = cfg["res"]
+        self.dx = dx = 1.0 / ry
+        self.CFL = cfg["CFL"]
+        self.reinit_every = cfg["reinit_every"]
+        self.visualize_dt = cfg["visualize_dt"]
+        self.BFECC_clamp = cfg["BFECC_clamp"]
+        self.ed_eps = cfg["eps_frac"] * dx
+        boundary_types = np.full((3, 2), 2, np.int32)
+        kw = dict(mode=solver_mode, stop=stop, check_every=check_every)
+        if "levels" in cfg:
+            kw["levels"] = cfg["levels"]
+        self.solver = MGPCG_3(boundary_types=boundary_types, N=[rx, ry, rz], base_level=cfg.get("base_level", 3),
+                              rel_tol=rel_tol if stop == "relative" else 1e-12, **kw)
+        self.poisson_stats = []
+        self.poisson_max_iters = 400   # sim/mgpcg.py:436
+        V = ti.Vector.field
+        F = ti.field
+        cell, fx, fy, fz = (rx, ry, rz), (rx + 1, ry, rz), (rx, ry + 1, rz), (rx, ry, rz + 1)
+        self.X = V(3, float, cell)
+        self.X_x, self.X_y, self.X_z = V(3, float, fx), V(3, float, fy), V(3, float, fz)
+        center_coords_func(self.X, dx)
+        x_coords_func(self.X_x, dx)
+        y_coords_func(self.X_y, dx)
+        z_coords_func(self.X_z, dx)
+        self.F_x, self.F_y, self.F_z = V(3, float, fx), V(3, float, fy), V(3, float, fz)
+        self.phi_x, self.phi_y, self.phi_z = V(3, float, fx), V(3, float, fy), V(3, float, fz)
+        self.u = V(3, float, cell)
+        self.u_x, self.u_y, self.u_z = F(float, fx), F(float, fy), F(float, fz)
+        self.init_u_x, self.init_u_y, self.init_u_z = F(float, fx), F(float, fy), F(float, fz)
+        self.err_u_x, self.err_u_y, self.err_u_z = F(float, fx), F(float, fy), F(float, fz)
+        self.tmp_u_x, self.tmp_u_y, self.tmp_u_z = F(float, fx), F(float, fy), F(float, fz)
+        self.init_smoke = V(4, float, cell)
+        self.smoke = V(4, float, cell)
+        self.tmp_smoke = V(4, float, cell)
+        self.err_smoke = V(4, float, cell)
+        self.psi_garm = V(3, float, cell)
+        self.T_x_garm, self.T_y_garm, self.T_z_garm = V(3, float, cell), V(3, float, cell), V(3, float, cell)
+        self.dT_x_garm, self.dT_y_garm, self.dT_z_garm = V(3, float, cell), V(3, float, cell), V(3, float, cell)
+        # scratch of the reference's fused march (accepted, unused by the in-place kernel)
+        self.psi_tmp = self.T_x_tmp = self.T_y_tmp = self.T_z_tmp = None
+        self.dT_x_tmp = self.dT_y_tmp = self.dT_z_tmp = None
+        self.num_samples = num_samples
+        self.seed = seed
+        self.i = -1
+        self.sub_t = 0.0
+        self.timing = None   # dict name -> list of (start, end) torch.cuda.Event pairs, when enabled
+
+    def _mark(self, name):
+        """Record a HIP event on the current stream at a phase boundary (timing only)."""
+        if self.timing is None:
+            return
+        import torch
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self.timing.setdefault("_marks", []).append((name, ev))
+
+    # -- initial condition (run_edge.py:538-557) ---------------------------------
+    def initialize(self):
+        init = self.cfg["init"]
+        self.u.fill(0.0)
+        if init == "leapfrog":
+            init_vorts_leapfrog(self.X, self.u, self.smoke, self.tmp_smoke)
+        elif init == "fourvorts":
+            init_four_vorts(self.X, self.u, self.smoke, self.tmp_smoke)
+        elif init == "rings2":
+            init_vorts_leapfrog(self.X, self.u, self.smoke, self.tmp_smoke, radius=0.2, x_gap=0.15, x_start=0.35,
+                                delta=0.03, w=1.0, num_samples=self.num_samples)
+        elif init == "rings4":
+            dom = (self.res_x * self.dx, self.res_y * self.dx, self.res_z * self.dx)
+            init_rings(self.X, self.u, self.smoke, self.tmp_smoke, seeded_rings(4, dom, seed=self.seed),
+                       num_samples=self.num_samples)
+        else:
+            raise ValueError(init)
+        split_central_vector(self.u, self.u_x, self.u_y, self.u_z)
+        self.solver.Poisson(self.u_x, self.u_y, self.u_z)
+
+    @property
+    def garm_map(self):
+        return (self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm, self.dT_y_garm,
+                self.dT_z_garm)
+
+    def poisson(self):
+        self.solver.Poisson(self.u_x, self.u_y, self.u_z, max_iters=self.poisson_max_iters)
+        self.poisson_stats.append(dict(self.solver.stats, time=self.solver.last_time))
+
+    # -- one step (run_edge.py:588-711) --------------------------------------------
+    def substep(self):
+        self.i += 1
+        j = self.i % self.reinit_every
+        dx = self.dx
+        max_speed = calc_max_speed(self.u_x, self.u_y, self.u_z)
+        curr_dt = self.CFL * dx / max_speed
+        if self.sub_t + curr_dt >= self.visualize_dt:
+            curr_dt = self.visualize_dt - self.sub_t
+            self.sub_t = 0.
+        elif self.sub_t + curr_dt * 2 >= self.visualize_dt:
+            curr_dt = (self.visualize_dt - self.sub_t) / 2
+            self.sub_t += curr_dt
+        else:
+            self.sub_t += curr_dt
+        self.curr_dt = curr_dt
+
+        if j == 0:
+            reset_to_identity(self.phi_x, self.phi_y, self.phi_z, self.F_x, self.F_y, self.F_z, dx)
+            reset_to_identity_2(self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm,
+                                self.dT_y_garm, self.dT_z_garm, self.X)
+            self.init_u_x.copy_from(self.u_x)
+            self.init_u_y.copy_from(self.u_y)
+            self.init_u_z.copy_from(self.u_z)
+            self.init_smoke.copy_from(self.smoke)
+
+        self._mark("sl")
+        advect_u_semi_lagrangian(self.u_x, self.u_y, self.u_z, self.tmp_u_x, self.tmp_u_y, self.tmp_u_z, dx,
+                                 0.5 * curr_dt, self.X_x, self.X_y, self.X_z)
+        self.u_x.swap_storage(self.tmp_u_x)   # u.copy_from(tmp_u): tmp_u is rewritten before it is read
+        self.u_y.swap_storage(self.tmp_u_y)
+        self.u_z.swap_storage(self.tmp_u_z)
+        self._mark("poisson1")
+        self.poisson()
+
+        self._mark("march")
+        advect_psi_garm_ed4_fusion_unroll(self.psi_tmp, self.T_x_tmp, self.T_y_tmp, self.T_z_tmp, self.dT_x_tmp,
+                                          self.dT_y_tmp, self.dT_z_tmp, self.psi_garm, self.T_x_garm, self.T_y_garm,
+                                          self.T_z_garm, self.dT_x_garm, self.dT_y_garm, self.dT_z_garm, self.u_x,
+                                          self.u_y, self.u_z, self.X, dx, curr_dt, self.ed_eps)
+        self._mark("phi_march")
+        march_phi_grid(self.phi_x, self.phi_y, self.phi_z, self.F_x, self.F_y, self.F_z, self.u_x, self.u_y,
+                       self.u_z, dx, curr_dt)
+
+        self._mark("maps")
+        gm = self.garm_map
+        map_u_psi(self.init_u_x, self.init_u_y, self.init_u_z, self.u_x, self.u_y, self.u_z, dx, gm)
+        map_smoke_psi(self.init_smoke, self.smoke, dx, gm)
+        map_u(self.u_x, self.u_y, self.u_z, self.err_u_x, self.err_u_y, self.err_u_z, self.F_x, self.F_y, self.F_z,
+              self.phi_x, self.phi_y, self.phi_z, dx)
+        map_smoke(self.smoke, self.err_smoke, self.phi_x, self.phi_y, self.phi_z, dx)
+        for e, i0 in ((self.err_u_x, self.init_u_x), (self.err_u_y, self.init_u_y), (self.err_u_z, self.init_u_z),
+                      (self.err_smoke, self.init_smoke)):
+            add_fields(e, i0, e, -1.)
+        for e in (self.err_u_x, self.err_u_y, self.err_u_z, self.err_smoke):
+            scale_field(e, 0.5, e)
+        map_u_psi(self.err_u_x, self.err_u_y, self.err_u_z, self.tmp_u_x, self.tmp_u_y, self.tmp_u_z, dx, gm)
+        map_smoke_psi(self.err_smoke, self.tmp_smoke, dx, gm)
+        add_fields(self.u_x, self.tmp_u_x, self.err_u_x, -1.)
+        add_fields(self.u_y, self.tmp_u_y, self.err_u_y, -1.)
+        add_fields(self.u_z, self.tmp_u_z, self.err_u_z, -1.)
+        add_fields(self.smoke, self.tmp_smoke, self.smoke, -1.)
+        clamp_smoke_psi(self.init_smoke, self.smoke, dx, gm)
+        if self.BFECC_clamp:
+            clamp_u3(self.err_u_x, self.err_u_y, self.err_u_z, self.u_x, self.u_y, self.u_z, self.tmp_u_x,
+                     self.tmp_u_y, self.tmp_u_z)
+            self.u_x.swap_storage(self.tmp_u_x)
+            self.u_y.swap_storage(self.tmp_u_y)
+            self.u_z.swap_storage(self.tmp_u_z)
+        else:
+            self.u_x.swap_storage(self.err_u_x)
+            self.u_y.swap_storage(self.err_u_y)
+            self.u_z.swap_storage(self.err_u_z)
+        self._mark("poisson2")
+        self.poisson()
+        self._mark("end")
+        return curr_dt
+
+    def velocity_numpy(self):
+        return [self.u_x.to_numpy(), self.u_y.to_numpy(), self.u_z.to_numpy()]
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-c", "--case", default="cfg1")
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--solver", default="plain", choices=["plain", "mixed"])
+    a = ap.parse_args(argv)
+    case = int(a.case) if a.case.isdigit() else a.case
+    sim = Simulation(case, solver_mode=a.solver)
+    sim.initialize()
+    ti.sync()
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        dt = sim.substep()
+        st = sim.poisson_stats[-1]
+        print(f"[Simulate] step {s} dt {dt:.5g} poisson iters {st['iters']}", flush=True)
+    ti.sync()
+    print(f"{a.steps} steps in {time.perf_counter() - t0:.2f}s")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,84 @@
+"""The EDGE step loop of run_edge.py:588-711 composed from oracle kernels.
+
+TEST INFRASTRUCTURE ONLY: the CPU restatement used for trajectory parity and
+as the timed CPU baseline.  State is numpy arrays in the shared SoA layout;
+every kernel is the oracle's literal restatement (edge_oracle*.cpp).
+"""
+import numpy as np
+
+from .oracle import Oracle
+
+
+def face_coords(a, nx, ny, nz, dx):
+    d = [nx + (a == 0), ny + (a == 1), nz + (a == 2)]
+    i, j, k = np.meshgrid(np.arange(d[0]), np.arange(d[1]), np.arange(d[2]), indexing="ij")
+    o = [0.0 if a == 0 else 0.5, 0.0 if a == 1 else 0.5, 0.0 if a == 2 else 0.5]
+    return np.stack([(i + o[0]) * dx, (j + o[1]) * dx, (k + o[2]) * dx])
+
+
+class OracleStep:
+    """Fields of run_edge.py:174-254 as numpy arrays; `substep()` = one loop iteration."""
+
+    def __init__(self, u, smoke, dx, reinit_every, CFL, eps, levels, BFECC_clamp, dtype=np.float32,
+                 stop_mode=1, rel_tol=1e-6, max_iters=400):
+        self.o = Oracle(dtype)
+        self.dt_ = np.dtype(dtype)
+        self.u = [np.ascontiguousarray(c, dtype) for c in u]
+        self.smoke = np.ascontiguousarray(smoke, dtype)
+        self.nx, self.ny, self.nz = self.u[0].shape[0] - 1, self.u[0].shape[1], self.u[0].shape[2]
+        self.dx, self.reinit_every, self.CFL, self.eps = dx, reinit_every, CFL, eps
+        self.levels, self.BFECC_clamp = levels, BFECC_clamp
+        self.stop_mode, self.rel_tol, self.max_iters = stop_mode, rel_tol, max_iters
+        n = (self.nx, self.ny, self.nz)
+        i, j, k = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+        self.X = np.stack([(i + .5) * dx, (j + .5) * dx, (k + .5) * dx]).astype(dtype)
+        self.i = -1
+        self.iters = []
+
+    def poisson(self):
+        self.u, p, st = self.o.poisson(self.u, self.levels, stop_mode=self.stop_mode, rel_tol=self.rel_tol,
+                                       abs_tol=1e-12, max_iters=self.max_iters)
+        self.iters.append(int(st[0]))
+        self.p = p
+
+    def substep(self, dt=None):
+        o, dx = self.o, self.dx
+        self.i += 1
+        j = self.i % self.reinit_every
+        if dt is None:
+            dt = self.CFL * dx / o.max_speed(self.u)
+        self.curr_dt = dt
+        n = (self.nx, self.ny, self.nz)
+        if j == 0:
+            self.phi = [face_coords(a, *n, dx).astype(self.dt_) for a in range(3)]
+            self.F = [np.broadcast_to(np.eye(3)[a][:, None, None, None], self.phi[a].shape).astype(self.dt_)
+                      for a in range(3)]
+            self.psi = self.X.copy()
+            self.T = np.zeros((3, 3) + n, self.dt_)
+            for c in range(3):
+                self.T[c, c] = 1
+            self.dT = np.zeros((3, 3) + n, self.dt_)
+            self.init_u = [c.copy() for c in self.u]
+            self.init_smoke = self.smoke.copy()
+        self.u = o.advect_u_sl(self.u, dx, 0.5 * dt)
+        self.poisson()
+        self.psi, self.T, self.dT = o.march_edge(self.psi, self.T, self.dT, self.u, self.X, dx, dt, self.eps)
+        for a in range(3):
+            self.phi[a], self.F[a] = o.advance_map_col(self.phi[a], self.F[a], self.u, dx, dt)
+        self.u = o.map_u_psi(self.init_u, self.psi, self.T, self.dT, dx)
+        self.smoke = o.map_smoke_psi(0, self.init_smoke, None, self.psi, self.T, self.dT, dx)
+        err_u = o.map_u(self.u, self.F, self.phi, dx)
+        err_s = o.map_smoke(0, self.smoke, None, self.phi, dx)
+        err_u = [((e + -1. * i0) * 0.5).astype(self.dt_) for e, i0 in zip(err_u, self.init_u)]
+        err_s = ((err_s + -1. * self.init_smoke) * 0.5).astype(self.dt_)
+        tmp_u = o.map_u_psi(err_u, self.psi, self.T, self.dT, dx)
+        tmp_s = o.map_smoke_psi(0, err_s, None, self.psi, self.T, self.dT, dx)
+        err_u = [(u + -1. * t).astype(self.dt_) for u, t in zip(self.u, tmp_u)]
+        self.smoke = (self.smoke + -1. * tmp_s).astype(self.dt_)
+        self.smoke = o.map_smoke_psi(1, self.init_smoke, self.smoke, self.psi, self.T, self.dT, dx)
+        if self.BFECC_clamp:
+            self.u = [o.clamp_u(e, u) for e, u in zip(err_u, self.u)]
+        else:
+            self.u = err_u
+        self.poisson()
+        return dt

@@ -1,0 +1,221 @@
+"""GPU parity of the MGPCG projection (plain and mixed) against the CPU oracle.
+
+Tolerances (FP32): operator application and one V-cycle agree with the FP32
+oracle to 1e-5 of scale; a fixed number of CG iterations to 1e-4 (different
+FP64 reduction order changes alpha/beta in the last bits); converged solves are
+judged against the FP64 oracle through the true residual ||b - A p|| <= 1e-6
+||b|| (the north-star stop rule) and the pressure difference.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import smooth_velocity
+
+pytestmark = pytest.mark.gpu
+
+
+def T_(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+
+
+def N_(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def scale_err(gpu, ref):
+    ref = np.asarray(ref, np.float64)
+    return float(np.max(np.abs(np.asarray(gpu, np.float64) - ref)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+class Solver:
+    def __init__(self, n, levels, mode):
+        from edge_hip import _lib
+        self.lib = _lib
+        self.n = n
+        h = ctypes.c_void_p()
+        _lib.call("edge_mg_create", *n, levels, mode, None, ctypes.byref(h))
+        self.h = h
+
+    def __del__(self):
+        self.lib.lib().edge_mg_destroy(self.h)
+
+    def set_solid(self, bm, bvel):
+        self.bm, self.bvel = bm, bvel
+        self.lib.call("edge_mg_set_solid", self.h, bm.data_ptr() if bm is not None else None,
+                      bvel.data_ptr() if bvel is not None else None)
+
+    def poisson(self, u, max_iters=400, stop_mode=1, rel_tol=1e-6, abs_tol=1e-12, x0=None, check_every=1):
+        from edge_hip.field import stream_handle
+        p = torch.zeros(self.n, device=u[0].device)
+        stats = (ctypes.c_double * 9)()
+        st = self.lib.call("edge_poisson", self.h, *[c.data_ptr() for c in u], p.data_ptr(), None, None, None,
+                           max_iters, stop_mode, abs_tol, rel_tol, x0.data_ptr() if x0 is not None else None,
+                           check_every, stats, stream_handle(), ok=(0, -4))
+        torch.cuda.synchronize()
+        return p, list(stats), st
+
+    def vcycle(self, r):
+        from edge_hip.field import stream_handle
+        z = torch.empty_like(r)
+        self.lib.call("edge_mg_vcycle", self.h, r.data_ptr(), z.data_ptr(), stream_handle())
+        return z
+
+    def apply_A(self, p):
+        from edge_hip.field import stream_handle
+        Ap = torch.empty_like(p)
+        self.lib.call("edge_mg_apply_A", self.h, p.data_ptr(), Ap.data_ptr(), stream_handle())
+        return Ap
+
+
+N3 = (32, 16, 16)
+DX = 1.0 / 16
+
+
+def divergence_rhs(u):
+    """b = -div u after the wall BC, recentred (sim/mgpcg.py:373-404,169-175)."""
+    ux, uy, uz = [c.copy() for c in u]
+    ux[0], ux[-1] = 0, 0
+    uy[:, 0], uy[:, -1] = 0, 0
+    uz[:, :, 0], uz[:, :, -1] = 0, 0
+    div = (ux[1:] - ux[:-1]) + (uy[:, 1:] - uy[:, :-1]) + (uz[:, :, 1:] - uz[:, :, :-1])
+    b = -div
+    return b - b.mean()
+
+
+def test_operator_and_vcycle_parity(orc32, dev):
+    rng = np.random.default_rng(0)
+    p = rng.normal(size=N3)
+    S = Solver(N3, 3, 0)
+    assert scale_err(N_(S.apply_A(T_(p, dev))), orc32.apply_A(p)) < 1e-6
+    r = rng.normal(size=N3)
+    r -= r.mean()
+    assert scale_err(N_(S.vcycle(T_(r, dev))), orc32.vcycle(r, 3)) < 1e-5
+    # with solids
+    bm = np.zeros(N3, np.int32)
+    bm[10:16, 4:10, 3:9] = 1
+    bvel = np.zeros((3,) + N3, np.float32)
+    gbm, gbv = torch.from_numpy(bm).to(dev), T_(bvel, dev)
+    S.set_solid(gbm, gbv)
+    fl = bm == 0
+    assert scale_err(N_(S.apply_A(T_(p, dev)))[fl], orc32.apply_A(p, bm=bm)[fl]) < 1e-6
+    rr = r * fl
+    assert scale_err(N_(S.vcycle(T_(rr, dev)))[fl], orc32.vcycle(rr, 3, bm=bm)[fl]) < 1e-5
+
+
+def test_mixed_vcycle_is_a_good_preconditioner(orc64, dev):
+    rng = np.random.default_rng(1)
+    a, b = rng.normal(size=N3), rng.normal(size=N3)
+    a -= a.mean()
+    b -= b.mean()
+    S = Solver(N3, 3, 1)
+    from edge_hip import _lib
+    assert _lib.lib().edge_mg_dense_bottom(S.h) == 1
+    Ma, Mb = N_(S.vcycle(T_(a, dev))), N_(S.vcycle(T_(b, dev)))
+    assert np.dot(Ma.ravel(), a.ravel()) > 0
+    assert abs(np.dot(Ma.ravel(), b.ravel()) - np.dot(a.ravel(), Mb.ravel())) < 2e-2 * np.dot(Ma.ravel(), a.ravel())
+    # close to the reference V-cycle (its 20 bottom sweeps vs the exact bottom, BF16 storage)
+    ref = orc64.vcycle(a, 3)
+    assert np.linalg.norm(Ma - ref) < 0.05 * np.linalg.norm(ref)
+
+
+def test_plain_poisson_fixed_iterations(orc32, dev):
+    u = smooth_velocity(*N3, DX, seed=2)
+    for k in (1, 4):
+        ru, rp, rst = orc32.poisson(u, levels=3, max_iters=k, stop_mode=1, rel_tol=0.0)
+        S = Solver(N3, 3, 0)
+        gu = [T_(c, dev) for c in u]
+        p, st, code = S.poisson(gu, max_iters=k, stop_mode=1, rel_tol=0.0)
+        assert code == -4 and st[0] == k
+        assert scale_err(N_(p), rp) < 1e-4
+        for a in range(3):
+            assert scale_err(N_(gu[a]), ru[a]) < 1e-4
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_poisson_converges_to_relative_tolerance(orc64, dev, mode):
+    u = smooth_velocity(*N3, DX, seed=3)
+    b = divergence_rhs(u)
+    ru, rp, rst = orc64.poisson(u, levels=3, stop_mode=1, rel_tol=1e-8)
+    S = Solver(N3, 3, mode)
+    gu = [T_(c, dev) for c in u]
+    p, st, code = S.poisson(gu, stop_mode=1, rel_tol=1e-6, check_every=1 if mode == 0 else 4)
+    assert code == 0 and st[5] == 1.0
+    P = N_(p)
+    res = b - orc64.apply_A(P)
+    assert np.linalg.norm(res) <= 1.5e-6 * np.linalg.norm(b)   # FP32 true residual vs FP64 norm
+    # pressure against the FP64 oracle (pressure defined up to a constant)
+    assert scale_err(P - P.mean(), rp - rp.mean()) < 1e-4
+    div = (N_(gu[0])[1:] - N_(gu[0])[:-1]) + (N_(gu[1])[:, 1:] - N_(gu[1])[:, :-1]) + \
+        (N_(gu[2])[:, :, 1:] - N_(gu[2])[:, :, :-1])
+    assert np.abs(div).max() < 1e-4 * np.abs(b).max()
+    if mode == 0:
+        _, _, rst32 = orc64.poisson(u, levels=3, stop_mode=1, rel_tol=1e-6)
+        assert abs(st[0] - rst32[0]) <= 1
+
+
+def test_reference_stop_rule_matches_oracle(orc32, dev):
+    u = smooth_velocity(*N3, DX, seed=4)
+    _, rp, rst = orc32.poisson(u, levels=3, stop_mode=0)
+    S = Solver(N3, 3, 0)
+    gu = [T_(c, dev) for c in u]
+    p, st, code = S.poisson(gu, stop_mode=0, abs_tol=1e-12, rel_tol=1e-12)
+    assert code == 0
+    assert abs(st[0] - rst[0]) <= 1
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_solid_poisson(orc64, dev, mode):
+    n = (32, 16, 16)
+    bm = np.zeros(n, np.int32)
+    bm[10:14, 5:11, 4:12] = 1
+    bvel = np.zeros((3,) + n, np.float32)
+    bvel[0][bm > 0] = 0.25
+    u = smooth_velocity(*n, DX, seed=5)
+    ru, rp, rst = orc64.poisson(u, levels=3, bm=bm, bvel=bvel, max_iters=200, stop_mode=1, rel_tol=1e-8)
+    S = Solver(n, 3, mode)
+    S.set_solid(torch.from_numpy(bm).to(dev), T_(bvel, dev))
+    gu = [T_(c, dev) for c in u]
+    p, st, code = S.poisson(gu, max_iters=200, stop_mode=1, rel_tol=1e-6)
+    assert code == 0
+    fl = bm == 0
+    P = N_(p)
+    assert scale_err((P - P[fl].mean())[fl], (rp - rp[fl].mean())[fl]) < 2e-4
+    assert st[3] == pytest.approx(rst[3], rel=1e-4, abs=1e-6)  # fluid divergence sum
+    for a in range(3):
+        assert scale_err(N_(gu[a]), ru[a]) < 2e-4
+
+
+def test_warm_start_and_fallback(orc64, dev):
+    u = smooth_velocity(*N3, DX, seed=6)
+    S = Solver(N3, 3, 1)
+    gu = [T_(c, dev) for c in u]
+    p, st, _ = S.poisson([c.clone() for c in gu], stop_mode=1, rel_tol=1e-6, check_every=1)
+    cold = st[0]
+    # warm start from the converged pressure: (almost) nothing left to do
+    p2, st2, code = S.poisson([c.clone() for c in gu], stop_mode=1, rel_tol=1e-6, x0=p, check_every=1)
+    assert code == 0 and st2[8] == 1.0 and st2[0] <= 2
+    # a guess worse than zero is rejected and the solve still converges
+    bad = torch.full_like(p, 1e3) * torch.randn_like(p)
+    p3, st3, code = S.poisson([c.clone() for c in gu], stop_mode=1, rel_tol=1e-6, x0=bad, check_every=1)
+    assert code == 0 and st3[8] == 0.0 and st3[0] == cold
+
+
+def test_drop_in_classes(orc32, dev):
+    from edge_hip import Field
+    from sim.mgpcg import MGPCG_3
+    u = smooth_velocity(*N3, DX, seed=7)
+    fields = []
+    for c in u:
+        f = Field(None, float, c.shape)
+        f.from_numpy(c.astype(np.float32))
+        fields.append(f)
+    solver = MGPCG_3(boundary_types=np.full((3, 2), 2), N=list(N3), base_level=2)
+    assert solver.n_mg_levels == 3
+    solver.Poisson(*fields)
+    ru, rp, rst = orc32.poisson(u, levels=3, stop_mode=0)
+    assert abs(solver.stats["iters"] - rst[0]) <= 1
+    for a in range(3):
+        assert scale_err(fields[a].to_numpy(), ru[a]) < 1e-4

@@ -1,0 +1,98 @@
+"""Whole-step parity: the GPU step loop (edge-flow-maps_amd/run_edge.py) against
+the oracle's composition of the same step (oracle/step.py).
+
+* one step with a fixed number of CG iterations on both sides: velocity,
+  smoke and map within 1e-4 of scale (FP32, different summation orders,
+  compounded over ~60 kernels);
+* a 10-step trajectory with converged projections (1e-6 relative on both
+  sides): velocity within 1e-3 of scale (the north-star trajectory bound).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def scale_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def make(res, fixed_iters=None, case="cfg1", clamp=None):
+    from run_edge import Simulation
+    from oracle.step import OracleStep
+    sim = Simulation(case, solver_mode="plain", res=res, num_samples=400)
+    if clamp is not None:
+        sim.BFECC_clamp = clamp
+    sim.initialize()
+    u0 = sim.velocity_numpy()
+    s0 = np.moveaxis(sim.smoke.to_numpy(), -1, 0)
+    kw = dict(stop_mode=1, rel_tol=1e-6)
+    if fixed_iters:
+        sim.poisson_max_iters = fixed_iters
+        sim.solver.rel_tol = 0.0
+        kw = dict(stop_mode=1, rel_tol=0.0, max_iters=fixed_iters)
+    ref = OracleStep(u0, s0, sim.dx, sim.reinit_every, sim.CFL, sim.ed_eps, levels=sim.solver.n_mg_levels,
+                     BFECC_clamp=sim.BFECC_clamp, **kw)
+    return sim, ref
+
+
+@pytest.mark.parametrize("clamp", [False, True])
+def test_one_step_fixed_iterations(dev, clamp):
+    sim, ref = make((32, 16, 16), fixed_iters=6, clamp=clamp)
+    for _ in range(2):
+        dt = sim.substep()
+        ref.substep(dt=dt)
+    torch.cuda.synchronize()
+    for g, r in zip(sim.velocity_numpy(), ref.u):
+        assert scale_err(g, r) < 1e-4
+    assert scale_err(np.moveaxis(sim.smoke.to_numpy(), -1, 0), ref.smoke) < 1e-4
+    assert scale_err(np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref.psi) < 1e-5
+
+
+def test_ten_step_trajectory(dev):
+    sim, ref = make((32, 16, 16))
+    for s in range(10):
+        dt = sim.substep()
+        ref.substep(dt=dt)
+    torch.cuda.synchronize()
+    for g, r in zip(sim.velocity_numpy(), ref.u):
+        assert scale_err(g, r) < 1e-3
+    # projections converged on both sides
+    assert all(st["converged"] for st in sim.poisson_stats)
+
+
+def test_drop_in_hermite_api(dev):
+    """sim/hermite.py's fused EDGE march through Fields equals the tensor-level path."""
+    from edge_hip import Field
+    from sim import hermite
+    from helpers import cell_coords, smooth_velocity
+    from oracle.oracle import Oracle
+    n, dx = (24, 16, 16), 1 / 16
+    u = smooth_velocity(*n, dx, seed=3)
+    fu = []
+    for c in u:
+        f = Field(None, float, c.shape)
+        f.from_numpy(c.astype(np.float32))
+        fu.append(f)
+    X = Field(3, float, n)
+    X.from_numpy(np.moveaxis(cell_coords(*n, dx), 0, -1).astype(np.float32))
+    psi = Field(3, float, n)
+    T = [Field(3, float, n) for _ in range(3)]
+    dT = [Field(3, float, n) for _ in range(3)]
+    hermite.reset_to_identity_2(psi, *T, *dT, X)
+    for _ in range(2):
+        hermite.advect_psi_garm_ed4_fusion_unroll(None, None, None, None, None, None, None, psi, *T, *dT, *fu, X,
+                                                  dx, 0.4 * dx, 0.025 * dx)
+    o = Oracle(np.float32)
+    Xn = cell_coords(*n, dx)
+    Tn = np.zeros((3, 3) + n)
+    for c in range(3):
+        Tn[c, c] = 1
+    p_, T_, dT_ = Xn, Tn, np.zeros_like(Tn)
+    for _ in range(2):
+        p_, T_, dT_ = o.march_edge(p_, T_, dT_, u, Xn, dx, 0.4 * dx, 0.025 * dx)
+    assert scale_err(np.moveaxis(psi.to_numpy(), -1, 0), p_) < 1e-6
+    assert scale_err(np.stack([np.moveaxis(t.to_numpy(), -1, 0) for t in T]), T_) < 1e-5
