@@ -87,10 +87,36 @@ __device__ __forceinline__ void st<__hip_bfloat16>(__hip_bfloat16* p, int64_t i,
   p[i] = __float2bfloat16(v);
 }
 
+// q / d for 0 <= q < 2^31 by a multiply-high (round-up magic number):
+// q / d = (umulhi(q, m) + q) >> l with l = ceil(log2 d), m = 2^32 (2^l - d) / d + 1.
+struct FDiv {
+  uint32_t d, m, l;
+};
+inline FDiv make_fdiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+  return FDiv{d, (uint32_t)m, l};
+}
+__device__ __forceinline__ uint32_t fdiv(uint32_t q, const FDiv& f) { return (__umulhi(q, f.m) + q) >> f.l; }
+
 struct Lv {
   int n0, n1, n2;
   const int8_t* bm;   // 1 = solid (null when no solids)
+  FDiv f1, f2, fh;    // divisions by n1, n2 and n2/2
 };
+
+inline Lv make_lv(int n0, int n1, int n2) {
+  Lv L;
+  L.n0 = n0;
+  L.n1 = n1;
+  L.n2 = n2;
+  L.bm = nullptr;
+  L.f1 = make_fdiv((uint32_t)n1);
+  L.f2 = make_fdiv((uint32_t)n2);
+  L.fh = make_fdiv((uint32_t)(n2 > 1 ? n2 / 2 : 1));
+  return L;
+}
 
 __device__ __forceinline__ int64_t lid(const Lv& L, int i, int j, int k) {
   return ((int64_t)i * L.n1 + j) * L.n2 + k;
@@ -136,28 +162,40 @@ __device__ __forceinline__ float neighbor_sum(const Lv& L, int i, int j, int k, 
   return ret;
 }
 
-__device__ __forceinline__ void decode(const Lv& L, int64_t q, int& i, int& j, int& k) {
-  k = (int)(q % L.n2);
-  j = (int)((q / L.n2) % L.n1);
-  i = (int)(q / ((int64_t)L.n1 * L.n2));
+__device__ __forceinline__ void decode(const Lv& L, int64_t q64, int& i, int& j, int& k) {
+  uint32_t q = (uint32_t)q64;
+  uint32_t row = fdiv(q, L.f2);
+  k = (int)(q - row * (uint32_t)L.n2);
+  uint32_t ii = fdiv(row, L.f1);
+  j = (int)(row - ii * (uint32_t)L.n1);
+  i = (int)ii;
 }
 
 // cell of colour `phase` number t (n2 even)
-__device__ __forceinline__ void colour_cell(const Lv& L, int64_t t, int phase, int& i, int& j, int& k) {
-  int h = L.n2 >> 1;
-  int kk = (int)(t % h);
-  int64_t row = t / h;
-  j = (int)(row % L.n1);
-  i = (int)(row / L.n1);
+__device__ __forceinline__ void colour_cell(const Lv& L, int64_t t64, int phase, int& i, int& j, int& k) {
+  uint32_t t = (uint32_t)t64;
+  uint32_t row = fdiv(t, L.fh);
+  int kk = (int)(t - row * (uint32_t)(L.n2 >> 1));
+  uint32_t ii = fdiv(row, L.f1);
+  j = (int)(row - ii * (uint32_t)L.n1);
+  i = (int)ii;
   k = 2 * kk + ((i + j + phase) & 1);
 }
+
+// Reduction kernels give each thread CPT cells (blockDim apart), so a block
+// covers BS*CPT consecutive cells.
+constexpr int CPT = 4;
+inline int red_blocks(int64_t n) { return (int)((n + (int64_t)BS * CPT - 1) / ((int64_t)BS * CPT)); }
+#define FOR_CELLS(q, N)                                                                          \
+  for (int64_t _u = 0, q = (int64_t)blockIdx.x * (BS * CPT) + threadIdx.x; _u < CPT; ++_u, q += BS) \
+    if (q < (N))
 
 // ---------------------------------------------------------------------------
 // Deterministic reductions: per-block FP64 partials; the last block to finish
 // sums them in a fixed order and runs the finalize action.
 // ---------------------------------------------------------------------------
 struct Red {
-  double* partials;      // NB * 2
+  double* partials;      // 2 per block of the largest reduction launch
   unsigned int* counter; // 1
   double* sc;            // scalars
 };
@@ -357,9 +395,10 @@ __device__ __forceinline__ float divergence_at(int nx, int ny, int nz, const Fac
 __global__ void __launch_bounds__(BS) k_div_sum(int nx, int ny, int nz, Faces3 U, const int8_t* __restrict__ bm0, Red R) {
   const int64_t N = (int64_t)nx * ny * nz;
   double s = 0, c = 0;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+  FOR_CELLS(q, N) {
     if (bm0[q] > 0) continue;
-    int k = (int)(q % nz), j = (int)((q / nz) % ny), i = (int)(q / ((int64_t)ny * nz));
+    int k = (int)((uint32_t)q % (uint32_t)nz), j = (int)(((uint32_t)q / (uint32_t)nz) % (uint32_t)ny),
+        i = (int)((uint32_t)q / ((uint32_t)ny * (uint32_t)nz));
     s += (double)divergence_at(nx, ny, nz, U, i, j, k);
     c += 1.0;
   }
@@ -376,14 +415,15 @@ __global__ void __launch_bounds__(BS) k_init(int nx, int ny, int nz, Faces3 U, c
   const double* sc = R.sc;
   float corr = (float)sc[S_CORR];
   double s = 0;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+  FOR_CELLS(q, N) {
     float v;
     if (b) {
       v = b[q];
     } else if (bm0 && bm0[q] > 0) {
       v = 0.f;
     } else {
-      int k = (int)(q % nz), j = (int)((q / nz) % ny), i = (int)(q / ((int64_t)ny * nz));
+      int k = (int)((uint32_t)q % (uint32_t)nz), j = (int)(((uint32_t)q / (uint32_t)nz) % (uint32_t)ny),
+          i = (int)((uint32_t)q / ((uint32_t)ny * (uint32_t)nz));
       float d = divergence_at(nx, ny, nz, U, i, j, k);
       if (bm0) d -= corr;
       v = d * -1.0f;
@@ -401,7 +441,7 @@ __global__ void __launch_bounds__(BS) k_recenter(Lv L, float* __restrict__ r, Re
   const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
   float mean = (float)R.sc[S_MEAN];
   double s = 0;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+  FOR_CELLS(q, N) {
     float v = r[q] - mean;
     r[q] = v;
     if (!(L.bm && L.bm[q] > 0)) s += (double)v * (double)v;
@@ -414,7 +454,7 @@ __global__ void __launch_bounds__(BS) k_residual_x0(Lv L, int bt, const float* _
                                                    float* __restrict__ w, Red R) {
   const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
   double s = 0;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+  FOR_CELLS(q, N) {
     int i, j, k;
     decode(L, q, i, j, k);
     if (L.bm && L.bm[q] > 0) {
@@ -474,7 +514,7 @@ __global__ void __launch_bounds__(BS) k_p_Ap(Lv L, int bt, const S* __restrict__
   const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
   const float beta = (float)R.sc[S_BETA];
   double s = 0;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+  FOR_CELLS(q, N) {
     int i, j, k;
     decode(L, q, i, j, k);
     if (L.bm && L.bm[q] > 0) continue;  // solid p stays (0), Ap untouched
@@ -502,7 +542,7 @@ __global__ void __launch_bounds__(BS) k_update_xr(Lv L, const S* __restrict__ z,
   const float alpha = (float)R.sc[S_ALPHA];
   const float beta = (float)R.sc[S_BETA];
   double s2 = 0, s1 = 0;
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
+  FOR_CELLS(q, N) {
     float rv = r[q];
     if (!(L.bm && L.bm[q] > 0)) {
       float pn = ld(z, q) + beta * p[q];
@@ -566,11 +606,11 @@ __global__ void __launch_bounds__(BS) k_smooth_last_dot(Lv L, int bt, const S* _
   if (!ignore_done && R.sc[S_DONE] != 0.0) return;
   const int64_t pairs = ((int64_t)L.n0 * L.n1 * L.n2) >> 1;
   double s = 0;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < pairs; t += (int64_t)gridDim.x * blockDim.x) {
-    int h = L.n2 >> 1;
-    int kk = (int)(t % h);
-    int64_t row = t / h;
-    int j = (int)(row % L.n1), i = (int)(row / L.n1);
+  FOR_CELLS(t, pairs) {
+    uint32_t row = fdiv((uint32_t)t, L.fh);
+    int kk = (int)((uint32_t)t - row * (uint32_t)(L.n2 >> 1));
+    uint32_t ii = fdiv(row, L.f1);
+    int j = (int)(row - ii * (uint32_t)L.n1), i = (int)ii;
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       int k = 2 * kk + e;
@@ -644,36 +684,57 @@ __global__ void __launch_bounds__(BS) k_prolong_black(Lv F, Lv C, int bt, const 
 
 // Bottom of the plain V-cycle: the 20 red/black pairs of sim/mgpcg.py:187-189
 // (the first red half-sweep was fused into the restriction) in one workgroup,
-// the level held in LDS.
+// the level held in LDS; each thread decodes its (at most BOT_CPT) cells and
+// their neighbour indices once.
+constexpr int BOT_THREADS = 1024;
+constexpr int BOT_CPT = 4;
 template <class S>
-__global__ void __launch_bounds__(1024) k_bottom_rbgs(Lv L, int bt, const S* __restrict__ r, S* __restrict__ z,
-                                                     const double* __restrict__ sc, int ignore_done, int pairs) {
+__global__ void __launch_bounds__(BOT_THREADS) k_bottom_rbgs(Lv L, int bt, const S* __restrict__ r, S* __restrict__ z,
+                                                            const double* __restrict__ sc, int ignore_done,
+                                                            int pairs) {
   if (!ignore_done && sc[S_DONE] != 0.0) return;
   extern __shared__ float lds[];
   const int N = L.n0 * L.n1 * L.n2;
   float* zs = lds;
-  float* rs = lds + N;
-  float* dg = lds + 2 * N;
-  for (int q = threadIdx.x; q < N; q += blockDim.x) {
+  int q_[BOT_CPT], nb[BOT_CPT][6], col[BOT_CPT];
+  float rv[BOT_CPT], dg[BOT_CPT];
+#pragma unroll
+  for (int c = 0; c < BOT_CPT; ++c) {
+    int q = threadIdx.x + c * BOT_THREADS;
+    q_[c] = q;
+    col[c] = -1;
+    if (q >= N) continue;
     zs[q] = ld(z, q);
-    rs[q] = ld(r, q);
     int i, j, k;
     decode(L, q, i, j, k);
-    dg[q] = (L.bm && L.bm[q] > 0) ? 0.f : num_fluid(L, bt, i, j, k);
+    bool sol = L.bm && L.bm[q] > 0;
+    col[c] = sol ? -1 : ((i + j + k) & 1);
+    rv[c] = ld(r, q);
+    dg[c] = sol ? 1.f : num_fluid(L, bt, i, j, k);
+    int cand[6][3] = {{i + 1, j, k}, {i - 1, j, k}, {i, j + 1, k}, {i, j - 1, k}, {i, j, k + 1}, {i, j, k - 1}};
+    bool ok[6] = {i < L.n0 - 1, i > 0, j < L.n1 - 1, j > 0, k < L.n2 - 1, k > 0};
+#pragma unroll
+    for (int e = 0; e < 6; ++e)
+      nb[c][e] = (ok[e] && !solid_at(L, cand[e][0], cand[e][1], cand[e][2])) ? (int)lid(L, cand[e][0], cand[e][1], cand[e][2])
+                                                                             : -1;
   }
   __syncthreads();
-  for (int s = 0; s < 2 * pairs - 1; ++s) {
-    int phase = (s & 1) == 0 ? 1 : 0;  // black, red, black, ...
-    for (int q = threadIdx.x; q < N; q += blockDim.x) {
-      int i, j, k;
-      decode(L, q, i, j, k);
-      if (((i + j + k) & 1) != phase || dg[q] == 0.f) continue;
-      float ns = neighbor_sum(L, i, j, k, [&](int a, int b, int c) { return zs[lid(L, a, b, c)]; });
-      zs[q] = (rs[q] + ns) / dg[q];
+  for (int s2 = 0; s2 < 2 * pairs - 1; ++s2) {
+    int phase = (s2 & 1) == 0 ? 1 : 0;  // black, red, black, ...
+#pragma unroll
+    for (int c = 0; c < BOT_CPT; ++c) {
+      if (col[c] != phase) continue;
+      float ns = 0.f;
+#pragma unroll
+      for (int e = 0; e < 6; ++e)
+        if (nb[c][e] >= 0) ns += zs[nb[c][e]];
+      zs[q_[c]] = (rv[c] + ns) / dg[c];
     }
     __syncthreads();
   }
-  for (int q = threadIdx.x; q < N; q += blockDim.x) st(z, q, zs[q]);
+#pragma unroll
+  for (int c = 0; c < BOT_CPT; ++c)
+    if (q_[c] < N) st(z, q_[c], zs[q_[c]]);
 }
 
 // Bottom of the mixed V-cycle: z = B^-1 r over the fluid cells of the coarsest
@@ -769,6 +830,11 @@ struct Solver {
   bool dense_ok = false;
   std::vector<int8_t> dense_mask;  // coarse mask the inverse was built for
   size_t elem;
+  // CG iteration batches captured as a HIP graph (replayed on the caller's stream)
+  int use_graph = 0;
+  hipStream_t cap = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int gbatch = 0;
 };
 
 template <class S>
@@ -776,7 +842,7 @@ struct Ops {
   static void vcycle(Solver& s, hipStream_t st, int ignore_done);
 };
 
-int bottom_lds_bytes(const Solver& s) { return (int)(s.nl[s.L - 1] * 3 * sizeof(float)); }
+int bottom_lds_bytes(const Solver& s) { return (int)(s.nl[s.L - 1] * sizeof(float)); }
 
 template <class S>
 void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
@@ -802,8 +868,8 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
     if (s.mode == 1 && s.dense_ok) {
       hipLaunchKernelGGL(k_bottom_dense<S>, dim3(s.nf), dim3(256), 0, st, s.nf, s.fidx, s.Binv, (const S*)s.rl[l],
                          (S*)s.zl[l], s.nl[l], sc, ig);
-    } else if (s.nl[l] <= 4096) {
-      hipLaunchKernelGGL(k_bottom_rbgs<S>, dim3(1), dim3(1024), bottom_lds_bytes(s), st, s.lv[l], s.bt,
+    } else if (s.nl[l] <= BOT_THREADS * BOT_CPT) {
+      hipLaunchKernelGGL(k_bottom_rbgs<S>, dim3(1), dim3(BOT_THREADS), bottom_lds_bytes(s), st, s.lv[l], s.bt,
                          (const S*)s.rl[l], (S*)s.zl[l], sc, ig, 20);
     } else {
       int64_t half = s.nl[l] / 2;
@@ -923,6 +989,8 @@ void free_solver(Solver* s) {
                   (void*)s->sc, (void*)s->Binv, (void*)s->fidx})
     if (p) (void)hipFree(p);
   if (s->sc_host) (void)hipHostFree(s->sc_host);
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->cap) (void)hipStreamDestroy(s->cap);
   delete s;
 }
 
@@ -960,28 +1028,74 @@ int prepare_masks(Solver& s, hipStream_t st) {
   return EDGE_OK;
 }
 
+// One CG iteration after the first preconditioning (sim/mgpcg.py:234-278).
+template <class S>
+void cg_iteration(Solver& s, hipStream_t st) {
+  Red R = red(s);
+  const Lv& L0 = s.lv[0];
+  hipLaunchKernelGGL(k_p_Ap<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
+  hipLaunchKernelGGL(k_update_xr<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, (const S*)s.zl[0], s.x, s.r, s.p, s.Ap, R);
+  Ops<S>::vcycle(s, st, 0);
+  hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r, (S*)s.zl[0],
+                     R, (int)F_ZR, 0);
+}
+
+template <class S>
+int capture_batch(Solver& s, int batch) {
+  if (s.gexec && s.gbatch == batch) return EDGE_OK;
+  if (s.gexec) {
+    (void)hipGraphExecDestroy(s.gexec);
+    s.gexec = nullptr;
+  }
+  if (!s.cap) EDGE_CHECK_HIP(hipStreamCreateWithFlags(&s.cap, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  EDGE_CHECK_HIP(hipStreamBeginCapture(s.cap, hipStreamCaptureModeThreadLocal));
+  for (int b = 0; b < batch; ++b) cg_iteration<S>(s, s.cap);
+  hipError_t le = hipGetLastError();
+  hipError_t e = hipStreamEndCapture(s.cap, &g);
+  if (le != hipSuccess || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    set_error("edge_poisson: graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : e));
+    return EDGE_EHIP;
+  }
+  e = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    s.gexec = nullptr;
+    set_error("edge_poisson: graph instantiate failed: %s", hipGetErrorString(e));
+    return EDGE_EHIP;
+  }
+  s.gbatch = batch;
+  return EDGE_OK;
+}
+
 template <class S>
 int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* stats) {
   Red R = red(s);
   const Lv& L0 = s.lv[0];
   // initial preconditioner application and p, z.r (sim/mgpcg.py:216-228)
   Ops<S>::vcycle(s, st, 1);
-  hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(NB), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r, (S*)s.zl[0], R,
+  hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r, (S*)s.zl[0], R,
                      (int)F_ZR_INIT, 1);
   EDGE_CHECK_LAUNCH();
   int it = 0;
   int done = 0;
   const int maxit = max_iters < 0 ? (1 << 30) : max_iters;
+  if (s.use_graph) {
+    int rc = capture_batch<S>(s, check_every);
+    if (rc) return rc;
+  }
   while (!done) {
-    for (int b = 0; b < check_every; ++b) {
-      hipLaunchKernelGGL(k_p_Ap<S>, dim3(NB), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
-      hipLaunchKernelGGL(k_update_xr<S>, dim3(NB), dim3(BS), 0, st, L0, (const S*)s.zl[0], s.x, s.r, s.p, s.Ap, R);
-      Ops<S>::vcycle(s, st, 0);
-      hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(NB), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r,
-                         (S*)s.zl[0], R, (int)F_ZR, 0);
-      EDGE_CHECK_LAUNCH();
-      ++it;
-      if (it >= maxit) break;
+    if (s.use_graph) {
+      EDGE_CHECK_HIP(hipGraphLaunch(s.gexec, st));
+      it += check_every;
+    } else {
+      for (int b = 0; b < check_every; ++b) {
+        cg_iteration<S>(s, st);
+        EDGE_CHECK_LAUNCH();
+        ++it;
+        if (it >= maxit) break;
+      }
     }
     EDGE_CHECK_HIP(hipMemcpyAsync(s.sc_host, s.sc, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st));
     EDGE_CHECK_HIP(hipStreamSynchronize(st));
@@ -1013,10 +1127,10 @@ int start_solve(Solver& s, hipStream_t st, const float* x0) {
   const Lv& L0 = s.lv[0];
   bool neu = (s.bt == 63);
   if (!neu) hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, st, s.sc, (int)S_MEAN, 0.0);
-  hipLaunchKernelGGL(k_recenter, dim3(NB), dim3(BS), 0, st, L0, s.r, R, (int)F_B2);
+  hipLaunchKernelGGL(k_recenter, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.r, R, (int)F_B2);
   EDGE_CHECK_LAUNCH();
   if (x0) {
-    hipLaunchKernelGGL(k_residual_x0, dim3(NB), dim3(BS), 0, st, L0, s.bt, s.r, s.x, s.Ap, R);
+    hipLaunchKernelGGL(k_residual_x0, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, s.r, s.x, s.Ap, R);
     hipLaunchKernelGGL(k_apply_x0, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N, s.r, s.Ap, s.x, s.sc);
     EDGE_CHECK_LAUNCH();
     // with walls all round, 1'A = 0, so r = b - A x0 stays mean-free
@@ -1070,7 +1184,7 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
     return EDGE_ENOMEM;
   };
   for (int l = 0; l < levels; ++l) {
-    s->lv[l] = Lv{nx >> l, ny >> l, nz >> l, nullptr};
+    s->lv[l] = make_lv(nx >> l, ny >> l, nz >> l);
     s->nl[l] = (int64_t)(nx >> l) * (ny >> l) * (nz >> l);
   }
   if (hipMalloc(&s->r, s->N * 4) != hipSuccess) return fail("hipMalloc r");
@@ -1085,7 +1199,8 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
     }
     if (hipMalloc(&s->zl[l], s->nl[l] * s->elem) != hipSuccess) return fail("hipMalloc level z");
   }
-  if (hipMalloc(&s->partials, NB * 2 * sizeof(double)) != hipSuccess) return fail("hipMalloc partials");
+  if (hipMalloc(&s->partials, (size_t)(red_blocks(s->N) + 1) * 2 * sizeof(double)) != hipSuccess)
+    return fail("hipMalloc partials");
   if (hipMalloc(&s->counter, sizeof(unsigned int)) != hipSuccess) return fail("hipMalloc counter");
   if (hipMalloc(&s->sc, 64 * sizeof(double)) != hipSuccess) return fail("hipMalloc scalars");
   if (hipHostMalloc(&s->sc_host, 128 * sizeof(double)) != hipSuccess) return fail("hipHostMalloc");
@@ -1163,8 +1278,8 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, s.nx, s.ny, s.nz, s.bt, U,
                      s.solid ? s.user_bm : nullptr, s.solid ? s.user_bvel : nullptr);
   if (s.solid)
-    hipLaunchKernelGGL(k_div_sum, dim3(NB), dim3(BS), 0, st, s.nx, s.ny, s.nz, U, (const int8_t*)s.bm[0], red(s));
-  hipLaunchKernelGGL(k_init, dim3(NB), dim3(BS), 0, st, s.nx, s.ny, s.nz, U, nullptr,
+    hipLaunchKernelGGL(k_div_sum, dim3(red_blocks(s.N)), dim3(BS), 0, st, s.nx, s.ny, s.nz, U, (const int8_t*)s.bm[0], red(s));
+  hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, s.nx, s.ny, s.nz, U, nullptr,
                      s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s));
   EDGE_CHECK_LAUNCH();
   rc = start_solve(s, st, x0);
@@ -1194,7 +1309,7 @@ int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int sto
   if (rc) return rc;
   rc = prepare_masks(s, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_init, dim3(NB), dim3(BS), 0, st, s.nx, s.ny, s.nz, Faces3{}, b,
+  hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, s.nx, s.ny, s.nz, Faces3{}, b,
                      s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s));
   EDGE_CHECK_LAUNCH();
   rc = start_solve(s, st, x0);
@@ -1219,13 +1334,13 @@ int edge_mg_vcycle(void* handle, const float* r, float* z, void* stream) {
   EDGE_CHECK_HIP(hipMemcpyAsync(s.r, r, s.N * 4, hipMemcpyDeviceToDevice, st));
   if (s.mode == 1) {
     Ops<__hip_bfloat16>::vcycle(s, st, 1);
-    hipLaunchKernelGGL(k_smooth_last_dot<__hip_bfloat16>, dim3(NB), dim3(BS), 0, st, s.lv[0], s.bt,
+    hipLaunchKernelGGL(k_smooth_last_dot<__hip_bfloat16>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, s.lv[0], s.bt,
                        (const __hip_bfloat16*)s.rl[0], s.r, (__hip_bfloat16*)s.zl[0], red(s), (int)F_NONE, 1);
     hipLaunchKernelGGL(k_to_float<__hip_bfloat16>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N,
                        (const __hip_bfloat16*)s.zl[0], z);
   } else {
     Ops<float>::vcycle(s, st, 1);
-    hipLaunchKernelGGL(k_smooth_last_dot<float>, dim3(NB), dim3(BS), 0, st, s.lv[0], s.bt, (const float*)s.rl[0], s.r,
+    hipLaunchKernelGGL(k_smooth_last_dot<float>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, s.lv[0], s.bt, (const float*)s.rl[0], s.r,
                        (float*)s.zl[0], red(s), (int)F_NONE, 1);
     EDGE_CHECK_HIP(hipMemcpyAsync(z, s.zl[0], s.N * 4, hipMemcpyDeviceToDevice, st));
   }
@@ -1243,6 +1358,14 @@ int edge_mg_apply_A(void* handle, const float* p, float* Ap, void* stream) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_apply_A, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.lv[0], s.bt, p, Ap);
   EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+// Replay the CG iterations as a captured HIP graph (1) or launch them one by one (0).
+int edge_mg_set_graph(void* handle, int enable) {
+  clear_error();
+  EDGE_REQUIRE(handle, "edge_mg_set_graph: null handle");
+  ((Solver*)handle)->use_graph = enable ? 1 : 0;
   return EDGE_OK;
 }
 

@@ -42,6 +42,8 @@ SIGNATURES = {
     "edge_map_smoke": "ippipppiiifp",
     "edge_map_smoke_psi": "ippipiiifp",
     "edge_clamp_u": "pppppppppiiip",
+    "edge_cell_psi": "ppiiifp",
+    "edge_smoke_at": "ippipiiifp",
     "edge_advect_u_sl": "ppppppiiiffp",
     "edge_max_speed": "ppppiiip",
     "edge_axpy": "pppflp",
@@ -59,6 +61,7 @@ SIGNATURES = {
     "edge_mg_vcycle": "pppp",
     "edge_mg_apply_A": "pppp",
     "edge_mg_dense_bottom": "p",
+    "edge_mg_set_graph": "pi",
 }
 
 _RESTYPE = {"edge_last_error": ctypes.c_char_p}

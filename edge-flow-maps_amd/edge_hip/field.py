@@ -12,16 +12,21 @@ Every write through this module or a kernel wrapper bumps a per-storage
 version, which is how derived data (the Hermite tables) knows when to rebuild.
 Code that writes `Field.data` directly must call `Field.touch()` afterwards.
 """
+import itertools
+
 import numpy as np
 import torch
 
 from . import _lib
 
+# storage address -> version; versions come from one global counter, so a new
+# allocation at a recycled address never matches a version seen before
 _VERSIONS = {}
+_COUNTER = itertools.count(1)
 
 
 def touch_ptr(ptr):
-    _VERSIONS[ptr] = _VERSIONS.get(ptr, 0) + 1
+    _VERSIONS[ptr] = next(_COUNTER)
 
 
 def version_of(ptr):
@@ -61,6 +66,7 @@ class Field:
         self.dtype = _torch_dtype(dtype)
         tshape = ((n,) if n else ()) + (self.shape if self.shape else (1,))
         self.data = torch.zeros(tshape, dtype=self.dtype, device=device)
+        self.touch()
 
     # -- identity / versions -------------------------------------------------
     @property

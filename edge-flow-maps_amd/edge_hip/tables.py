@@ -28,6 +28,12 @@ def _alloc(psi):
     return torch.empty(3 * n * 8, dtype=torch.float32, device=psi.data.device)
 
 
+def table_version(kind, psi, T, dT=None, d2T=None, dx=None):
+    """(key, version) identifying the map's current contents for derived caches."""
+    fields = [psi, *T] + (list(dT) if dT is not None else []) + (list(d2T) if d2T is not None else [])
+    return _key_and_version(kind, fields, dx)
+
+
 def hermite_table(kind, psi, T, dT=None, d2T=None, dx=None):
     """Return the (possibly cached) table of the map psi with rows T=(T_x,T_y,T_z)."""
     fields = [psi, *T] + (list(dT) if dT is not None else []) + (list(d2T) if d2T is not None else [])

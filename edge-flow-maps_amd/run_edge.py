@@ -109,6 +109,7 @@ class Simulation:
         self.i = -1
         self.sub_t = 0.0
         self.timing = None   # dict name -> list of (start, end) torch.cuda.Event pairs, when enabled
+        self.rhs_snapshots = None  # list: velocities handed to each projection, when enabled
 
     def _mark(self, name):
         """Record a HIP event on the current stream at a phase boundary (timing only)."""
@@ -145,6 +146,8 @@ class Simulation:
                 self.dT_z_garm)
 
     def poisson(self):
+        if self.rhs_snapshots is not None:
+            self.rhs_snapshots.append([self.u_x.data.clone(), self.u_y.data.clone(), self.u_z.data.clone()])
         self.solver.Poisson(self.u_x, self.u_y, self.u_z, max_iters=self.poisson_max_iters)
         self.poisson_stats.append(dict(self.solver.stats, time=self.solver.last_time))
 

@@ -9,7 +9,7 @@ import torch
 
 from edge_hip import _lib
 from edge_hip.field import as_ptr, stream_handle, touch
-from edge_hip.tables import hermite_table
+from edge_hip.tables import hermite_table, table_version
 
 
 def _cells_of_mac(u_x):
@@ -61,12 +61,71 @@ def _garm_table(m, dx):
     return hermite_table("garm", psi, (T_x, T_y, T_z), (dT_x, dT_y, dT_z), None, dx)
 
 
+# Data derived from one map state and shared by the step's pull-backs: the
+# face map (psi and the column d psi / d x_a at every face, i.e.
+# split_psi_hermite of the garm map) for map_u_psi, and the cell positions for
+# map_smoke_psi / clamp_smoke_psi.  Keyed by the map's storage and version.
+_derived = {"key": None, "face": None, "psic": None, "bufs": {}}
+
+
+def _derived_for(m, dx):
+    psi, T_x, T_y, T_z, dT_x, dT_y, dT_z = m
+    key = table_version("garm", psi, (T_x, T_y, T_z), (dT_x, dT_y, dT_z), None, dx)
+    if _derived["key"] != key:
+        _derived["key"] = key
+        _derived["face"] = None
+        _derived["psic"] = None
+    return _derived
+
+
+def _bufs(shape, device):
+    b = _derived["bufs"].get((shape, device))
+    if b is None:
+        nx, ny, nz = shape
+        fs = [(3, nx + 1, ny, nz), (3, nx, ny + 1, nz), (3, nx, ny, nz + 1)]
+        b = dict(phi=[torch.empty(f, device=device) for f in fs], F=[torch.empty(f, device=device) for f in fs],
+                 psic=torch.empty((3, nx, ny, nz), device=device))
+        _derived["bufs"][(shape, device)] = b
+    return b
+
+
+def face_map_of(m, dx):
+    """(phi[3], F[3]) tensors: the map and its d/dx_a column at the faces (cached)."""
+    d = _derived_for(m, dx)
+    if d["face"] is None:
+        psi = m[0]
+        nx, ny, nz = psi.shape
+        b = _bufs(psi.shape, psi.data.device)
+        tab = _garm_table(m, dx)
+        _lib.call("edge_split_psi_hermite", tab.data_ptr(), *[t.data_ptr() for t in b["phi"]],
+                  *[t.data_ptr() for t in b["F"]], nx, ny, nz, float(dx), stream_handle())
+        d["face"] = (b["phi"], b["F"])
+    return d["face"]
+
+
+def cell_psi_of(m, dx):
+    """(3, nx, ny, nz) tensor: the smoke pull-back positions of each cell (cached)."""
+    d = _derived_for(m, dx)
+    if d["psic"] is None:
+        psi = m[0]
+        nx, ny, nz = psi.shape
+        b = _bufs(psi.shape, psi.data.device)
+        tab = _garm_table(m, dx)
+        _lib.call("edge_cell_psi", tab.data_ptr(), b["psic"].data_ptr(), nx, ny, nz, float(dx), stream_handle())
+        d["psic"] = b["psic"]
+    return d["psic"]
+
+
 def map_u_psi(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, dx, garm_map):
-    """run_edge.py:376-417; garm_map = (psi_garm, T_x/y/z_garm, dT_x/y/z_garm)."""
+    """run_edge.py:376-417; garm_map = (psi_garm, T_x/y/z_garm, dT_x/y/z_garm).
+
+    u1_a(X_a) = sum_c d psi_c / d x_a (X_a) u0_c(psi(X_a)): the map's value and
+    column at the faces are evaluated once per map state and shared by the
+    step's two calls; the pull-back itself is map_u's kernel."""
     nx, ny, nz = _cells_of_mac(u_x0)
-    tab = _garm_table(garm_map, dx)
-    _lib.call("edge_map_u_psi", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
-              tab.data_ptr(), nx, ny, nz, float(dx), stream_handle())
+    phi, F = face_map_of(garm_map, dx)
+    _lib.call("edge_map_u", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
+              *[t.data_ptr() for t in F], *[t.data_ptr() for t in phi], nx, ny, nz, float(dx), stream_handle())
     touch(u_x1, u_y1, u_z1)
 
 
@@ -89,14 +148,14 @@ def clamp_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx):
 
 def _smoke_psi(mode, smoke0, smoke1, dx, garm_map):
     nx, ny, nz = smoke0.shape
-    tab = _garm_table(garm_map, dx)
-    _lib.call("edge_map_smoke_psi", mode, as_ptr(smoke0), as_ptr(smoke1), smoke0.n or 1, tab.data_ptr(), nx, ny, nz,
+    psic = cell_psi_of(garm_map, dx)
+    _lib.call("edge_smoke_at", mode, as_ptr(smoke0), as_ptr(smoke1), smoke0.n or 1, psic.data_ptr(), nx, ny, nz,
               float(dx), stream_handle())
     touch(smoke1)
 
 
 def map_smoke_psi(smoke0, smoke1, dx, garm_map):
-    """run_edge.py:443-466."""
+    """run_edge.py:443-466 (cell positions shared by the step's three smoke pull-backs)."""
     _smoke_psi(0, smoke0, smoke1, dx, garm_map)
 
 

@@ -9,6 +9,8 @@ keyword-only with defaults that reproduce the reference:
               or "relative" (||r|| <= rel_tol ||b||, FP64 norms)
   check_every CG iterations between host convergence checks (the reference
               reads a scalar every iteration: 1)
+  graph       replay each batch of check_every CG iterations as one captured
+              HIP graph (default: on for "mixed", off for "plain")
   x0          warm start for Poisson(), used only if ||b - A x0|| <= ||b||
 After a solve, `self.p` holds the pressure (the reference's get_result(self.p),
 sim/mgpcg.py:438) and `self.stats` the solve statistics.
@@ -36,7 +38,7 @@ def _bt_array(boundary_types):
 
 class _MGBase:
     def __init__(self, boundary_types, N, base_level=3, levels=None, mode="plain", stop="reference",
-                 check_every=None, abs_tol=1e-12, rel_tol=1e-12):
+                 check_every=None, abs_tol=1e-12, rel_tol=1e-12, graph=None):
         self.N = [int(n) for n in N]
         self.dim = 3
         self.n_mg_levels = int(levels) if levels is not None else int(math.log2(min(self.N))) - base_level + 1
@@ -50,6 +52,8 @@ class _MGBase:
         h = ctypes.c_void_p()
         _lib.call("edge_mg_create", *self.N, self.n_mg_levels, 0 if mode == "plain" else 1, self._bt, ctypes.byref(h))
         self._h = h
+        self.graph = (mode == "mixed") if graph is None else bool(graph)
+        _lib.call("edge_mg_set_graph", self._h, int(self.graph))
         self.p = Field(None, float, self.N)
         self.stats = {}
         self.last_time = 0.0

@@ -122,6 +122,13 @@ int edge_map_smoke(int mode, const float* s0, float* s1, int nc, const float* ph
 /* mode 0 map_smoke_psi (run_edge.py:443-466), 1 clamp_smoke_psi (480-505). */
 int edge_map_smoke_psi(int mode, const float* s0, float* s1, int nc, const float* tab, int nx, int ny, int nz,
                        float dx, void* stream);
+/* The smoke pull-back's cell positions (split_to_mac + the 1/6 average of
+ * map_smoke_psi, run_edge.py:430-466), computed once per map: psic 3 planes. */
+int edge_cell_psi(const float* tab, float* psic, int nx, int ny, int nz, float dx, void* stream);
+/* map_smoke_psi (mode 0) / clamp_smoke_psi (mode 1) at precomputed positions
+ * psic (run_edge.py:443-466, 480-505). */
+int edge_smoke_at(int mode, const float* s0, float* s1, int nc, const float* psic, int nx, int ny, int nz, float dx,
+                  void* stream);
 /* clamp_u (run_edge.py:508-519) on the three face grids. */
 int edge_clamp_u(const float* ux, const float* uy, const float* uz, const float* u1x, const float* u1y,
                  const float* u1z, float* u2x, float* u2y, float* u2z, int nx, int ny, int nz, void* stream);
@@ -179,6 +186,10 @@ int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int sto
  * and Ap = A p (get_Ap, sim/mgpcg.py:119-124). */
 int edge_mg_vcycle(void* handle, const float* r, float* z, void* stream);
 int edge_mg_apply_A(void* handle, const float* p, float* Ap, void* stream);
+/* Replay each batch of check_every CG iterations as one captured HIP graph
+ * (enable=1) instead of launching its ~40 kernels one by one.  No
+ * counterpart in the reference (its CG loop is host-driven, sim/mgpcg.py:234). */
+int edge_mg_set_graph(void* handle, int enable);
 /* 1 if the mixed solver uses the dense coarsest solve (0: red-black bottom). */
 int edge_mg_dense_bottom(void* handle);
 
