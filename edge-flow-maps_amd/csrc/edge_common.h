@@ -31,6 +31,13 @@ inline Grid make_grid(int nx, int ny, int nz, float dx) {
   g.ny = ny;
   g.nz = nz;
   g.dx = dx;
+  g.cz = make_fdiv((uint32_t)nz);
+  g.cy = make_fdiv((uint32_t)ny);
+  for (int a = 0; a < 3; ++a) {
+    g.fz[a] = make_fdiv((uint32_t)(nz + (a == 2)));
+    g.fy[a] = make_fdiv((uint32_t)(ny + (a == 1)));
+    g.nf[a] = (uint32_t)((int64_t)(nx + (a == 0)) * (ny + (a == 1)) * (nz + (a == 2)));
+  }
   int n[3] = {nx, ny, nz};
   for (int a = 0; a < 3; ++a) {
     g.hi_c[a] = (float)((double)n[a] - 2.0 - 1e-6);
@@ -42,7 +49,9 @@ inline Grid make_grid(int nx, int ny, int nz, float dx) {
 
 inline bool dims_ok(int nx, int ny, int nz) {
   // the 3-tap B-spline and the Hermite corner pair need at least 4 cells per axis
-  return nx >= 4 && ny >= 4 && nz >= 4 && (int64_t)nx * ny * nz < (int64_t)1 << 31;
+  // index decodes use 32-bit arithmetic: every face grid together below 2^31
+  const int64_t nfaces = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
+  return nx >= 4 && ny >= 4 && nz >= 4 && nfaces < (int64_t)1 << 31;
 }
 
 inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }

@@ -21,11 +21,29 @@
 namespace edge {
 
 // ---------------------------------------------------------------------------
+// q / d for 0 <= q < 2^31 by a multiply-high (round-up magic number):
+// q / d = (umulhi(q, m) + q) >> l with l = ceil(log2 d), m = 2^32 (2^l - d) / d + 1.
+// ---------------------------------------------------------------------------
+struct FDiv {
+  uint32_t d, m, l;
+};
+inline FDiv make_fdiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+  return FDiv{d, (uint32_t)m, l};
+}
+EDGE_DEV uint32_t fdiv(uint32_t q, const FDiv& f) { return (__umulhi(q, f.m) + q) >> f.l; }
+
+// ---------------------------------------------------------------------------
 // Grid parameters passed by value to every kernel.
 // ---------------------------------------------------------------------------
 struct Grid {
   int nx, ny, nz;   // cells
   float dx;
+  FDiv cz, cy;          // division by nz, ny (cell index decode)
+  FDiv fz[3], fy[3];    // division by the k and j extents of face grid a
+  uint32_t nf[3];       // face counts
   // Clamp upper bounds dim-2-1e-6 rounded to f32 (util/taichi_utils.py:262-264):
   // hi_c[a] for a cell-sized axis, hi_f[a] for the staggered (+1) axis.
   float hi_c[3], hi_f[3];
@@ -43,6 +61,39 @@ EDGE_DEV void face_dims(const Grid& g, int a, int d[3]) {
   d[0] = g.nx + (a == 0);
   d[1] = g.ny + (a == 1);
   d[2] = g.nz + (a == 2);
+}
+
+// (i, j, k) of cell q (q < 2^31).
+EDGE_DEV void cell_ijk(const Grid& g, int64_t q64, int& i, int& j, int& k) {
+  uint32_t q = (uint32_t)q64;
+  uint32_t row = fdiv(q, g.cz);
+  k = (int)(q - row * (uint32_t)g.nz);
+  uint32_t ii = fdiv(row, g.cy);
+  j = (int)(row - ii * (uint32_t)g.ny);
+  i = (int)ii;
+}
+
+// (i, j, k) of element t of face grid a.
+EDGE_DEV void face_ijk(const Grid& g, int a, uint32_t t, int& i, int& j, int& k) {
+  uint32_t row = fdiv(t, g.fz[a]);
+  k = (int)(t - row * g.fz[a].d);
+  uint32_t ii = fdiv(row, g.fy[a]);
+  j = (int)(row - ii * g.fy[a].d);
+  i = (int)ii;
+}
+
+// Thread t over the concatenated face grids x, y, z: axis a, index l in that
+// grid, its size nf and (i, j, k).  False past the end.
+EDGE_DEV bool face_split(const Grid& g, int64_t t64, int& a, int64_t& l, int64_t& nf, int& i, int& j, int& k) {
+  uint32_t t = (uint32_t)t64;
+  if (t64 >= (int64_t)g.nf[0] + g.nf[1] + g.nf[2]) return false;
+  if (t < g.nf[0]) { a = 0; }
+  else if (t < g.nf[0] + g.nf[1]) { a = 1; t -= g.nf[0]; }
+  else { a = 2; t -= g.nf[0] + g.nf[1]; }
+  l = t;
+  nf = g.nf[a];
+  face_ijk(g, a, t, i, j, k);
+  return true;
 }
 
 // ---------------------------------------------------------------------------

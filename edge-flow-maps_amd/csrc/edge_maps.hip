@@ -19,20 +19,7 @@ struct CFaces {
 // Decode a flat index over the concatenated face grids x, y, z.
 __device__ __forceinline__ bool face_of(const Grid& g, int64_t t, int& a, int64_t& local, int64_t& nf, int& i, int& j,
                                         int& k) {
-  const int64_t n0 = (int64_t)(g.nx + 1) * g.ny * g.nz;
-  const int64_t n1 = (int64_t)g.nx * (g.ny + 1) * g.nz;
-  const int64_t n2 = (int64_t)g.nx * g.ny * (g.nz + 1);
-  if (t < n0) { a = 0; nf = n0; }
-  else if (t < n0 + n1) { a = 1; t -= n0; nf = n1; }
-  else if (t < n0 + n1 + n2) { a = 2; t -= n0 + n1; nf = n2; }
-  else return false;
-  local = t;
-  int d[3];
-  face_dims(g, a, d);
-  k = (int)(t % d[2]);
-  j = (int)((t / d[2]) % d[1]);
-  i = (int)(t / ((int64_t)d[1] * d[2]));
-  return true;
+  return face_split(g, t, a, local, nf, i, j, k);
 }
 
 __device__ __forceinline__ int64_t nfaces(const Grid& g) {
@@ -138,7 +125,8 @@ __global__ void __launch_bounds__(256) k_map_smoke(Grid g, int mode, const float
   const int64_t N = ncell(g);
   const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (q >= N) return;
-  int k = (int)(q % g.nz), j = (int)((q / g.nz) % g.ny), i = (int)(q / ((int64_t)g.ny * g.nz));
+  int i, j, k;
+  cell_ijk(g, q, i, j, k);
   const int64_t n0 = (int64_t)(g.nx + 1) * g.ny * g.nz, n1 = (int64_t)g.nx * (g.ny + 1) * g.nz,
                 n2 = (int64_t)g.nx * g.ny * (g.nz + 1);
   int64_t fx0 = ((int64_t)i * g.ny + j) * g.nz + k, fx1 = fx0 + (int64_t)g.ny * g.nz;
@@ -168,7 +156,8 @@ __global__ void __launch_bounds__(256) k_map_smoke_psi(Grid g, int mode, const f
   const int64_t N = ncell(g);
   const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (q >= N) return;
-  int k = (int)(q % g.nz), j = (int)((q / g.nz) % g.ny), i = (int)(q / ((int64_t)g.ny * g.nz));
+  int i, j, k;
+  cell_ijk(g, q, i, j, k);
   float acc[3] = {0.f, 0.f, 0.f};
 #pragma unroll 1
   for (int f = 0; f < 6; ++f) {
@@ -196,7 +185,8 @@ __global__ void __launch_bounds__(256) k_cell_psi(Grid g, const float4* __restri
   const int64_t N = ncell(g);
   const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (q >= N) return;
-  int k = (int)(q % g.nz), j = (int)((q / g.nz) % g.ny), i = (int)(q / ((int64_t)g.ny * g.nz));
+  int i, j, k;
+  cell_ijk(g, q, i, j, k);
   float acc[3] = {0.f, 0.f, 0.f};
 #pragma unroll 1
   for (int f = 0; f < 6; ++f) {
@@ -284,7 +274,8 @@ __global__ void __launch_bounds__(256) k_max_speed(Grid g, Mac U, unsigned int* 
   const int64_t N = ncell(g);
   float m = 1.e-3f;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (int64_t)gridDim.x * blockDim.x) {
-    int k = (int)(q % g.nz), j = (int)((q / g.nz) % g.ny), i = (int)(q / ((int64_t)g.ny * g.nz));
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     float u = 0.5f * (U.ux[q] + U.ux[q + (int64_t)g.ny * g.nz]);
     int64_t qy = ((int64_t)i * (g.ny + 1) + j) * g.nz + k;
     float v = 0.5f * (U.uy[qy] + U.uy[qy + g.nz]);
@@ -292,10 +283,16 @@ __global__ void __launch_bounds__(256) k_max_speed(Grid g, Mac U, unsigned int* 
     float w = 0.5f * (U.uz[qz] + U.uz[qz + 1]);
     m = fmaxf(m, sqrtf(u * u + v * v + w * w));
   }
-  // wave64 reduction, then one atomic per wave
+  // wave64 reduction, block maximum in LDS, one atomic per block
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+  __shared__ float wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+    atomicMax(out, __float_as_uint(b));
+  }
 }
 
 __global__ void k_set_u32(unsigned int* p, unsigned int v) { *p = v; }
@@ -543,7 +540,7 @@ int edge_max_speed(const float* ux, const float* uy, const float* uz, float* out
   Grid g = make_grid(nx, ny, nz, 1.0f);
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, (hipStream_t)stream, (unsigned int*)out_dev, 0u);
   EDGE_CHECK_LAUNCH();
-  int nb = std::min(blocks_for(ncell(g), 256), 4096);
+  int nb = std::min(blocks_for(ncell(g), 256), 1024);
   hipLaunchKernelGGL(k_max_speed, dim3(nb), dim3(256), 0, (hipStream_t)stream, g, Mac{ux, uy, uz},
                      (unsigned int*)out_dev);
   EDGE_CHECK_LAUNCH();

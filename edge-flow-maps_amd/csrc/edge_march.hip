@@ -14,12 +14,6 @@ struct MapOut {
   float* d2T[3];
 };
 
-__device__ __forceinline__ void cell_ijk(const Grid& g, int64_t q, int& i, int& j, int& k) {
-  k = (int)(q % g.nz);
-  j = (int)((q / g.nz) % g.ny);
-  i = (int)(q / ((int64_t)g.ny * g.nz));
-}
-
 __device__ __forceinline__ void write_map_row(const MapOut& o, int64_t N, int64_t q, const float psi[3],
                                               const float T[3][3]) {
 #pragma unroll
@@ -425,9 +419,8 @@ __global__ void __launch_bounds__(256) k_reset_faces(Grid g, FaceMap fm) {
   else return;
   int d[3];
   face_dims(g, a, d);
-  int k = (int)(t % d[2]);
-  int j = (int)((t / d[2]) % d[1]);
-  int i = (int)(t / ((int64_t)d[1] * d[2]));
+  int i, j, k;
+  face_ijk(g, a, (uint32_t)t, i, j, k);
   float Xf[3];
   face_coord(a, i, j, k, g.dx, Xf);
 #pragma unroll

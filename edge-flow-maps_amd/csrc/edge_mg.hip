@@ -87,19 +87,6 @@ __device__ __forceinline__ void st<__hip_bfloat16>(__hip_bfloat16* p, int64_t i,
   p[i] = __float2bfloat16(v);
 }
 
-// q / d for 0 <= q < 2^31 by a multiply-high (round-up magic number):
-// q / d = (umulhi(q, m) + q) >> l with l = ceil(log2 d), m = 2^32 (2^l - d) / d + 1.
-struct FDiv {
-  uint32_t d, m, l;
-};
-inline FDiv make_fdiv(uint32_t d) {
-  uint32_t l = 0;
-  while ((1ull << l) < d) ++l;
-  uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
-  return FDiv{d, (uint32_t)m, l};
-}
-__device__ __forceinline__ uint32_t fdiv(uint32_t q, const FDiv& f) { return (__umulhi(q, f.m) + q) >> f.l; }
-
 struct Lv {
   int n0, n1, n2;
   const int8_t* bm;   // 1 = solid (null when no solids)
@@ -191,12 +178,11 @@ inline int red_blocks(int64_t n) { return (int)((n + (int64_t)BS * CPT - 1) / ((
     if (q < (N))
 
 // ---------------------------------------------------------------------------
-// Deterministic reductions: per-block FP64 partials; the last block to finish
-// sums them in a fixed order and runs the finalize action.
+// Deterministic reductions: per-block FP64 partials, then one small kernel sums
+// them in a fixed order and runs the finalize action.
 // ---------------------------------------------------------------------------
 struct Red {
   double* partials;      // 2 per block of the largest reduction launch
-  unsigned int* counter; // 1
   double* sc;            // scalars
 };
 
@@ -269,9 +255,11 @@ __device__ void finalize(double* sc, int fin, double s0, double s1) {
   }
 }
 
-__device__ void block_reduce_finish(const Red& R, int fin, double s0, double s1) {
+// Per-block partial sums (fixed order within the block).  The launch that
+// follows, k_finish, adds the partials in a fixed order and runs the finalize
+// action; the kernel boundary orders the two, so no fence or counter is needed.
+__device__ void block_partial(const Red& R, double s0, double s1) {
   __shared__ double sh[2][BS / 64];
-  __shared__ bool last;
   s0 = wave_sum(s0);
   s1 = wave_sum(s1);
   int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
@@ -282,27 +270,29 @@ __device__ void block_reduce_finish(const Red& R, int fin, double s0, double s1)
   __syncthreads();
   if (threadIdx.x == 0) {
     double a = 0, b = 0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+#pragma unroll
+    for (int q = 0; q < BS / 64; ++q) {
       a += sh[0][q];
       b += sh[1][q];
     }
     R.partials[2 * blockIdx.x] = a;
     R.partials[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    unsigned t = atomicAdd(R.counter, 1u);
-    last = (t == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const volatile double* P = R.partials;
+}
+
+constexpr int FIN_THREADS = 1024;
+// guard: skip when the solve has already stopped (the producing kernel skipped too).
+__global__ void __launch_bounds__(FIN_THREADS) k_finish(Red R, int nb, int fin, int guard) {
+  if (guard && R.sc[S_DONE] != 0.0) return;
+  __shared__ double sh[2][FIN_THREADS / 64];
   double a = 0, b = 0;
-  for (int q = threadIdx.x; q < (int)gridDim.x; q += blockDim.x) {
-    a += P[2 * q];
-    b += P[2 * q + 1];
+  for (int q = threadIdx.x; q < nb; q += FIN_THREADS) {
+    a += R.partials[2 * q];
+    b += R.partials[2 * q + 1];
   }
   a = wave_sum(a);
   b = wave_sum(b);
+  int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
   if (ln == 0) {
     sh[0][w] = a;
     sh[1][w] = b;
@@ -310,13 +300,11 @@ __device__ void block_reduce_finish(const Red& R, int fin, double s0, double s1)
   __syncthreads();
   if (threadIdx.x == 0) {
     double ta = 0, tb = 0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+    for (int q = 0; q < FIN_THREADS / 64; ++q) {
       ta += sh[0][q];
       tb += sh[1][q];
     }
     finalize(R.sc, fin, ta, tb);
-    *R.counter = 0u;
-    __threadfence();
   }
 }
 
@@ -331,31 +319,16 @@ struct Faces3 {
   float* f[3];
 };
 
-__device__ __forceinline__ bool face_decode(int nx, int ny, int nz, int64_t t, int& a, int64_t& l, int& i, int& j,
-                                            int& k) {
-  const int64_t n0 = (int64_t)(nx + 1) * ny * nz, n1 = (int64_t)nx * (ny + 1) * nz,
-                n2 = (int64_t)nx * ny * (nz + 1);
-  if (t < n0) a = 0;
-  else if (t < n0 + n1) { a = 1; t -= n0; }
-  else if (t < n0 + n1 + n2) { a = 2; t -= n0 + n1; }
-  else return false;
-  l = t;
-  int d1 = ny + (a == 1), d2 = nz + (a == 2);
-  k = (int)(t % d2);
-  j = (int)((t / d2) % d1);
-  i = (int)(t / ((int64_t)d1 * d2));
-  return true;
-}
-
 // apply_bc: box walls zeroed, then the faces of solid cells set to boundary_vel;
 // where two solid cells share a face the higher-index cell's value stands (the
 // result of a sequential loop over cells).
-__global__ void __launch_bounds__(BS) k_apply_bc(int nx, int ny, int nz, int bt, Faces3 U, const int* __restrict__ bm,
+__global__ void __launch_bounds__(BS) k_apply_bc(Grid g, int bt, Faces3 U, const int* __restrict__ bm,
                                                 const float* __restrict__ bvel) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int a, i, j, k;
-  int64_t l;
-  if (!face_decode(nx, ny, nz, t, a, l, i, j, k)) return;
+  int64_t l, nfa;
+  if (!face_split(g, t, a, l, nfa, i, j, k)) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   int I[3] = {i, j, k};
   int n[3] = {nx, ny, nz};
   float v = U.f[a][l];
@@ -392,25 +365,27 @@ __device__ __forceinline__ float divergence_at(int nx, int ny, int nz, const Fac
 }
 
 // solid: sum and count of the fluid divergence
-__global__ void __launch_bounds__(BS) k_div_sum(int nx, int ny, int nz, Faces3 U, const int8_t* __restrict__ bm0, Red R) {
+__global__ void __launch_bounds__(BS) k_div_sum(Grid g, Faces3 U, const int8_t* __restrict__ bm0, Red R) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int64_t N = (int64_t)nx * ny * nz;
   double s = 0, c = 0;
   FOR_CELLS(q, N) {
     if (bm0[q] > 0) continue;
-    int k = (int)((uint32_t)q % (uint32_t)nz), j = (int)(((uint32_t)q / (uint32_t)nz) % (uint32_t)ny),
-        i = (int)((uint32_t)q / ((uint32_t)ny * (uint32_t)nz));
+    int i, j, k;
+    cell_ijk(g, q, i, j, k);
     s += (double)divergence_at(nx, ny, nz, U, i, j, k);
     c += 1.0;
   }
-  block_reduce_finish(R, F_DIVSUM, s, c);
+  block_partial(R, s, c);
 }
 
 // init(u_div, -1) (sim/mgpcg.py:67-75): r = -(div - corr) on fluid cells (0 on
 // solids), x = x0 or 0, p = 0; partial sum of r for the recenter.
 // When b is given instead (solve API), r = b.
-__global__ void __launch_bounds__(BS) k_init(int nx, int ny, int nz, Faces3 U, const float* __restrict__ b,
+__global__ void __launch_bounds__(BS) k_init(Grid g, Faces3 U, const float* __restrict__ b,
                                             const int8_t* __restrict__ bm0, float* __restrict__ r, float* __restrict__ x,
                                             float* __restrict__ p, const float* __restrict__ x0, Red R) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int64_t N = (int64_t)nx * ny * nz;
   const double* sc = R.sc;
   float corr = (float)sc[S_CORR];
@@ -422,8 +397,8 @@ __global__ void __launch_bounds__(BS) k_init(int nx, int ny, int nz, Faces3 U, c
     } else if (bm0 && bm0[q] > 0) {
       v = 0.f;
     } else {
-      int k = (int)((uint32_t)q % (uint32_t)nz), j = (int)(((uint32_t)q / (uint32_t)nz) % (uint32_t)ny),
-          i = (int)((uint32_t)q / ((uint32_t)ny * (uint32_t)nz));
+      int i, j, k;
+      cell_ijk(g, q, i, j, k);
       float d = divergence_at(nx, ny, nz, U, i, j, k);
       if (bm0) d -= corr;
       v = d * -1.0f;
@@ -433,11 +408,11 @@ __global__ void __launch_bounds__(BS) k_init(int nx, int ny, int nz, Faces3 U, c
     p[q] = 0.f;
     s += (double)v;
   }
-  block_reduce_finish(R, F_MEAN, s, 0.0);
+  block_partial(R, s, 0.0);
 }
 
 // r -= mean over all cells (sim/mgpcg.py:169-175) and ||r||^2 over fluid cells.
-__global__ void __launch_bounds__(BS) k_recenter(Lv L, float* __restrict__ r, Red R, int fin) {
+__global__ void __launch_bounds__(BS) k_recenter(Lv L, float* __restrict__ r, Red R) {
   const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
   float mean = (float)R.sc[S_MEAN];
   double s = 0;
@@ -446,7 +421,7 @@ __global__ void __launch_bounds__(BS) k_recenter(Lv L, float* __restrict__ r, Re
     r[q] = v;
     if (!(L.bm && L.bm[q] > 0)) s += (double)v * (double)v;
   }
-  block_reduce_finish(R, fin, s, 0.0);
+  block_partial(R, s, 0.0);
 }
 
 // warm start: w = r - A x0 and ||w||^2; decide whether to keep x0.
@@ -467,7 +442,7 @@ __global__ void __launch_bounds__(BS) k_residual_x0(Lv L, int bt, const float* _
     w[q] = v;
     s += (double)v * (double)v;
   }
-  block_reduce_finish(R, F_X0, s, 0.0);
+  block_partial(R, s, 0.0);
 }
 
 __global__ void __launch_bounds__(BS) k_apply_x0(int64_t N, float* __restrict__ r, const float* __restrict__ w,
@@ -480,12 +455,12 @@ __global__ void __launch_bounds__(BS) k_apply_x0(int64_t N, float* __restrict__ 
 
 // subtract_grad_p (sim/mgpcg.py:406-432) with p = x, + apply_bc, and the
 // optional force buffer -(grad p) (sim/mgpcg.py:447-479).
-__global__ void __launch_bounds__(BS) k_subtract_grad(int nx, int ny, int nz, Faces3 U, Faces3 F,
-                                                     const float* __restrict__ P) {
+__global__ void __launch_bounds__(BS) k_subtract_grad(Grid g, Faces3 U, Faces3 F, const float* __restrict__ P) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int a, i, j, k;
-  int64_t l;
-  if (!face_decode(nx, ny, nz, t, a, l, i, j, k)) return;
+  int64_t l, nfa;
+  if (!face_split(g, t, a, l, nfa, i, j, k)) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   int I[3] = {i, j, k};
   int n[3] = {nx, ny, nz};
   auto pat = [&](int ii, int jj, int kk) {
@@ -528,7 +503,7 @@ __global__ void __launch_bounds__(BS) k_p_Ap(Lv L, int bt, const S* __restrict__
     s += (double)pq * (double)ap;
   }
   // p itself is written by k_update_xr: neighbours above read the old p
-  block_reduce_finish(R, F_ALPHA, s, 0.0);
+  block_partial(R, s, 0.0);
 }
 
 // p = z + beta p, x += alpha p, r -= alpha Ap (fluid cells); r.r over fluid
@@ -554,7 +529,7 @@ __global__ void __launch_bounds__(BS) k_update_xr(Lv L, const S* __restrict__ z,
     }
     s1 += (double)rv;
   }
-  block_reduce_finish(R, F_XR, s2, s1);
+  block_partial(R, s2, s1);
 }
 
 // ---------------------------------------------------------------------------
@@ -601,7 +576,7 @@ __global__ void __launch_bounds__(BS) k_smooth(Lv L, int bt, const S* __restrict
 // thread per (k even, k+1) pair, updates the red cell, reads the black one.
 template <class S>
 __global__ void __launch_bounds__(BS) k_smooth_last_dot(Lv L, int bt, const S* __restrict__ rl,
-                                                       const float* __restrict__ r, S* __restrict__ z, Red R, int fin,
+                                                       const float* __restrict__ r, S* __restrict__ z, Red R,
                                                        int ignore_done) {
   if (!ignore_done && R.sc[S_DONE] != 0.0) return;
   const int64_t pairs = ((int64_t)L.n0 * L.n1 * L.n2) >> 1;
@@ -628,7 +603,7 @@ __global__ void __launch_bounds__(BS) k_smooth_last_dot(Lv L, int bt, const S* _
       s += (double)zv * (double)r[q];
     }
   }
-  block_reduce_finish(R, fin, s, 0.0);
+  block_partial(R, s, 0.0);
 }
 
 // Restriction (sim/mgpcg.py:147-153) in gather form: each coarse cell sums
@@ -817,7 +792,6 @@ struct Solver {
   int8_t* bm[MAXL];  // null when no solids
   float *r, *x, *p, *Ap;
   double* partials;
-  unsigned int* counter;
   double* sc;
   double* sc_host;   // pinned
   bool solid = false;
@@ -836,6 +810,13 @@ struct Solver {
   hipGraphExec_t gexec = nullptr;
   int gbatch = 0;
 };
+
+Red red(Solver& s) { return Red{s.partials, s.sc}; }
+
+// the finishing launch of a reduction over `nb` blocks
+void finish(Solver& s, hipStream_t st, int nb, Fin fin, int guard) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(FIN_THREADS), 0, st, red(s), nb, (int)fin, guard);
+}
 
 template <class S>
 struct Ops {
@@ -892,7 +873,6 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
   }
 }
 
-Red red(Solver& s) { return Red{s.partials, s.counter, s.sc}; }
 
 // Build B^-1 for the coarsest level on the host (FP64 Cholesky).  Returns
 // false when B is not positive definite (e.g. a disconnected fluid region).
@@ -985,7 +965,7 @@ void free_solver(Solver* s) {
     if (s->zl[l]) (void)hipFree(s->zl[l]);
     if (s->bm[l]) (void)hipFree(s->bm[l]);
   }
-  for (void* p : {(void*)s->r, (void*)s->x, (void*)s->p, (void*)s->Ap, (void*)s->partials, (void*)s->counter,
+  for (void* p : {(void*)s->r, (void*)s->x, (void*)s->p, (void*)s->Ap, (void*)s->partials,
                   (void*)s->sc, (void*)s->Binv, (void*)s->fidx})
     if (p) (void)hipFree(p);
   if (s->sc_host) (void)hipHostFree(s->sc_host);
@@ -1034,10 +1014,13 @@ void cg_iteration(Solver& s, hipStream_t st) {
   Red R = red(s);
   const Lv& L0 = s.lv[0];
   hipLaunchKernelGGL(k_p_Ap<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
+  finish(s, st, red_blocks(s.N), F_ALPHA, 1);
   hipLaunchKernelGGL(k_update_xr<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, (const S*)s.zl[0], s.x, s.r, s.p, s.Ap, R);
+  finish(s, st, red_blocks(s.N), F_XR, 1);
   Ops<S>::vcycle(s, st, 0);
   hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r, (S*)s.zl[0],
-                     R, (int)F_ZR, 0);
+                     R, 0);
+  finish(s, st, red_blocks(s.N / 2), F_ZR, 1);
 }
 
 template <class S>
@@ -1076,7 +1059,8 @@ int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* st
   // initial preconditioner application and p, z.r (sim/mgpcg.py:216-228)
   Ops<S>::vcycle(s, st, 1);
   hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r, (S*)s.zl[0], R,
-                     (int)F_ZR_INIT, 1);
+                     1);
+  finish(s, st, red_blocks(s.N / 2), F_ZR_INIT, 0);
   EDGE_CHECK_LAUNCH();
   int it = 0;
   int done = 0;
@@ -1127,10 +1111,12 @@ int start_solve(Solver& s, hipStream_t st, const float* x0) {
   const Lv& L0 = s.lv[0];
   bool neu = (s.bt == 63);
   if (!neu) hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, st, s.sc, (int)S_MEAN, 0.0);
-  hipLaunchKernelGGL(k_recenter, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.r, R, (int)F_B2);
+  hipLaunchKernelGGL(k_recenter, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.r, R);
+  finish(s, st, red_blocks(s.N), F_B2, 0);
   EDGE_CHECK_LAUNCH();
   if (x0) {
     hipLaunchKernelGGL(k_residual_x0, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, s.r, s.x, s.Ap, R);
+    finish(s, st, red_blocks(s.N), F_X0, 0);
     hipLaunchKernelGGL(k_apply_x0, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N, s.r, s.Ap, s.x, s.sc);
     EDGE_CHECK_LAUNCH();
     // with walls all round, 1'A = 0, so r = b - A x0 stays mean-free
@@ -1175,7 +1161,6 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
   std::memset(s->bm, 0, sizeof(s->bm));
   s->r = s->x = s->p = s->Ap = nullptr;
   s->partials = nullptr;
-  s->counter = nullptr;
   s->sc = nullptr;
   s->sc_host = nullptr;
   auto fail = [&](const char* what) {
@@ -1201,10 +1186,8 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
   }
   if (hipMalloc(&s->partials, (size_t)(red_blocks(s->N) + 1) * 2 * sizeof(double)) != hipSuccess)
     return fail("hipMalloc partials");
-  if (hipMalloc(&s->counter, sizeof(unsigned int)) != hipSuccess) return fail("hipMalloc counter");
   if (hipMalloc(&s->sc, 64 * sizeof(double)) != hipSuccess) return fail("hipMalloc scalars");
   if (hipHostMalloc(&s->sc_host, 128 * sizeof(double)) != hipSuccess) return fail("hipHostMalloc");
-  if (hipMemset(s->counter, 0, sizeof(unsigned int)) != hipSuccess) return fail("hipMemset");
   if (hipMemset(s->p, 0, s->N * 4) != hipSuccess) return fail("hipMemset");
   if (s->nl[levels - 1] * 3 * 4 > 160 * 1024 && mode == 0) {
     // bottom falls back to per-sweep launches; fine
@@ -1275,21 +1258,24 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   if (rc) return rc;
   rc = prepare_masks(s, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, s.nx, s.ny, s.nz, s.bt, U,
+  hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), s.bt, U,
                      s.solid ? s.user_bm : nullptr, s.solid ? s.user_bvel : nullptr);
-  if (s.solid)
-    hipLaunchKernelGGL(k_div_sum, dim3(red_blocks(s.N)), dim3(BS), 0, st, s.nx, s.ny, s.nz, U, (const int8_t*)s.bm[0], red(s));
-  hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, s.nx, s.ny, s.nz, U, nullptr,
+  if (s.solid) {
+    hipLaunchKernelGGL(k_div_sum, dim3(red_blocks(s.N)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), U, (const int8_t*)s.bm[0], red(s));
+    finish(s, st, red_blocks(s.N), F_DIVSUM, 0);
+  }
+  hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), U, nullptr,
                      s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s));
+  finish(s, st, red_blocks(s.N), F_MEAN, 0);
   EDGE_CHECK_LAUNCH();
   rc = start_solve(s, st, x0);
   if (rc) return rc;
   int cg = run_cg_any(s, st, max_iters, check_every, stats);
   if (cg != EDGE_OK && cg != EDGE_ENOCONV) return cg;
   // get_result + subtract_grad_p + apply_bc
-  hipLaunchKernelGGL(k_subtract_grad, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, s.nx, s.ny, s.nz, U,
+  hipLaunchKernelGGL(k_subtract_grad, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), U,
                      Faces3{{fx, fy, fz}}, (const float*)s.x);
-  hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, s.nx, s.ny, s.nz, s.bt, U,
+  hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), s.bt, U,
                      s.solid ? s.user_bm : nullptr, s.solid ? s.user_bvel : nullptr);
   EDGE_CHECK_LAUNCH();
   if (p_out) EDGE_CHECK_HIP(hipMemcpyAsync(p_out, s.x, s.N * 4, hipMemcpyDeviceToDevice, st));
@@ -1309,8 +1295,9 @@ int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int sto
   if (rc) return rc;
   rc = prepare_masks(s, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, s.nx, s.ny, s.nz, Faces3{}, b,
+  hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), Faces3{}, b,
                      s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s));
+  finish(s, st, red_blocks(s.N), F_MEAN, 0);
   EDGE_CHECK_LAUNCH();
   rc = start_solve(s, st, x0);
   if (rc) return rc;
@@ -1335,13 +1322,13 @@ int edge_mg_vcycle(void* handle, const float* r, float* z, void* stream) {
   if (s.mode == 1) {
     Ops<__hip_bfloat16>::vcycle(s, st, 1);
     hipLaunchKernelGGL(k_smooth_last_dot<__hip_bfloat16>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, s.lv[0], s.bt,
-                       (const __hip_bfloat16*)s.rl[0], s.r, (__hip_bfloat16*)s.zl[0], red(s), (int)F_NONE, 1);
+                       (const __hip_bfloat16*)s.rl[0], s.r, (__hip_bfloat16*)s.zl[0], red(s), 1);
     hipLaunchKernelGGL(k_to_float<__hip_bfloat16>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N,
                        (const __hip_bfloat16*)s.zl[0], z);
   } else {
     Ops<float>::vcycle(s, st, 1);
     hipLaunchKernelGGL(k_smooth_last_dot<float>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, s.lv[0], s.bt, (const float*)s.rl[0], s.r,
-                       (float*)s.zl[0], red(s), (int)F_NONE, 1);
+                       (float*)s.zl[0], red(s), 1);
     EDGE_CHECK_HIP(hipMemcpyAsync(z, s.zl[0], s.N * 4, hipMemcpyDeviceToDevice, st));
   }
   EDGE_CHECK_LAUNCH();

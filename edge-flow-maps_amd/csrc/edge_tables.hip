@@ -44,9 +44,8 @@ __global__ void __launch_bounds__(256) k_pack_table(Grid g, MapIn m, float4* __r
   int64_t N = ncell(g);
   int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (q >= N) return;
-  int k = (int)(q % g.nz);
-  int j = (int)((q / g.nz) % g.ny);
-  int i = (int)(q / ((int64_t)g.ny * g.nz));
+  int i, j, k;
+  cell_ijk(g, q, i, j, k);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float4 lo, hi;
@@ -70,9 +69,8 @@ __global__ void __launch_bounds__(256) k_fd_dT(Grid g, MapIn m, float* __restric
   int64_t N = ncell(g);
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= N) return;
-  int k = (int)(q % g.nz);
-  int j = (int)((q / g.nz) % g.ny);
-  int i = (int)(q / ((int64_t)g.ny * g.nz));
+  int i, j, k;
+  cell_ijk(g, q, i, j, k);
   float* out[3] = {dTx, dTy, dTz};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -91,9 +89,8 @@ __global__ void __launch_bounds__(256) k_fd_d2T(Grid g, MapIn m, float* __restri
   int64_t N = ncell(g);
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= N) return;
-  int k = (int)(q % g.nz);
-  int j = (int)((q / g.nz) % g.ny);
-  int i = (int)(q / ((int64_t)g.ny * g.nz));
+  int i, j, k;
+  cell_ijk(g, q, i, j, k);
   float* out[3] = {d2Tx, d2Ty, d2Tz};
 #pragma unroll
   for (int c = 0; c < 3; ++c) out[c][q] = fd_xyz(m.dT[c], g, i, j, k);
