@@ -137,7 +137,7 @@ def main():
     import torch
     from run_edge import Simulation
 
-    sim = Simulation("cfg2", solver_mode="mixed", stop="relative", rel_tol=1e-6, res=tuple(a.res), check_every=4)
+    sim = Simulation("cfg2", solver_mode="mixed", stop="relative", rel_tol=1e-6, res=tuple(a.res), check_every=0)
     sim.initialize()
     for _ in range(a.warmup):
         sim.substep()

@@ -17,6 +17,29 @@
 #include <stdint.h>
 
 #define EDGE_DEV __device__ __forceinline__
+// Keeps the instruction scheduler from interleaving the work on either side
+// (bounds the gathers in flight, and so the registers they hold).
+#ifndef EDGE_SCHED_FENCE
+#define EDGE_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#endif
+// Tuning knobs (compile-time): minimum waves per SIMD requested for the RK4
+// march kernels (0 = the compiler's choice).
+#ifndef EDGE_WPE_MARCH
+#define EDGE_WPE_MARCH 0
+#endif
+#ifndef EDGE_WPE_FACES
+#define EDGE_WPE_FACES 0
+#endif
+#if EDGE_WPE_MARCH
+#define EDGE_MARCH_ATTR __attribute__((amdgpu_waves_per_eu(EDGE_WPE_MARCH, 8)))
+#else
+#define EDGE_MARCH_ATTR
+#endif
+#if EDGE_WPE_FACES
+#define EDGE_FACES_ATTR __attribute__((amdgpu_waves_per_eu(EDGE_WPE_FACES, 8)))
+#else
+#define EDGE_FACES_ATTR
+#endif
 
 namespace edge {
 
@@ -128,23 +151,39 @@ EDGE_DEV Axis3 bspline_axis(float u, float hi) {
   return A;
 }
 
-// Value and gradient (in index units) of the quadratic B-spline of one field.
-// f: the field; dims (d0,d1,d2); A/B/C: axis weights.
+// Buffer-resource gathers: the field base and size live in scalar registers,
+// each lane supplies one 32-bit byte offset, the wave-uniform part of an
+// address goes in the scalar offset and small constants in the immediate
+// field.  Reads past `bytes` return 0 instead of faulting.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+EDGE_DEV Rsrc make_rsrc(const void* p, uint32_t bytes) {
+  // dword 3 = 0x00020000: the raw-buffer format word of the gfx9 family (gfx950 included)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+EDGE_DEV float bload(Rsrc r, uint32_t voff, uint32_t soff) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+EDGE_DEV float4 bload4(Rsrc r, uint32_t voff, uint32_t soff) {
+  auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
+// Value and gradient (in index units) of the quadratic B-spline of one field
+// with strides s0 (axis 0) and s1 (axis 1), in elements; A/B/C: axis weights.
+// Every tap read is inside the grid: the position clamp to [1, dim-2-eps]
+// puts the first tap in [0, dim-3] (bspline_axis), so the reference's sample()
+// clamp (util/taichi_utils.py:61-67) never binds on a tap of non-zero weight.
 template <bool GRAD>
-EDGE_DEV float bspline_eval(const float* __restrict__ f, int d1, int d2, int m0, int m1, int m2,
-                            const Axis3& A, const Axis3& B, const Axis3& C, float g[3]) {
-  int kk[3];
-#pragma unroll
-  for (int t = 0; t < 3; ++t) kk[t] = min(C.i0 + t, m2);
+EDGE_DEV float bspline_eval(Rsrc f, uint32_t s0, uint32_t s1, const Axis3& A, const Axis3& B, const Axis3& C,
+                            float g[3]) {
+  const uint32_t vb = 4u * ((uint32_t)A.i0 * s0 + (uint32_t)B.i0 * s1 + (uint32_t)C.i0);
   float val = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    int ii = min(A.i0 + a, m0);
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      int jj = min(B.i0 + b, m1);
-      const float* row = f + ((int64_t)ii * d1 + jj) * d2;
-      float v0 = row[kk[0]], v1 = row[kk[1]], v2 = row[kk[2]];
+      const uint32_t so = 4u * (a * s0 + b * s1);
+      float v0 = bload(f, vb, so), v1 = bload(f, vb + 4u, so), v2 = bload(f, vb + 8u, so);
       float s = v0 * C.w[0] + v1 * C.w[1] + v2 * C.w[2];
       float wab = A.w[a] * B.w[b];
       val += s * wab;
@@ -181,11 +220,12 @@ EDGE_DEV void interp_u_mac(const Grid& g, const Mac& U, const float p[3], float 
   Axis3 ow1 = bspline_axis(q1, g.hi_f[1]), ot1 = bspline_axis(q1 - 0.5f, g.hi_c[1]);
   Axis3 ow2 = bspline_axis(q2, g.hi_f[2]), ot2 = bspline_axis(q2 - 0.5f, g.hi_c[2]);
   float gr[3];
-  out[0] = bspline_eval<GRAD>(U.ux, g.ny, g.nz, g.nx, g.ny - 1, g.nz - 1, ow0, ot1, ot2, gr);
+  const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
+  out[0] = bspline_eval<GRAD>(make_rsrc(U.ux, 4u * (nx + 1) * ny * nz), ny * nz, nz, ow0, ot1, ot2, gr);
   if (GRAD) { float inv = 1.0f / g.dx; J[0][0] = gr[0] * inv; J[0][1] = gr[1] * inv; J[0][2] = gr[2] * inv; }
-  out[1] = bspline_eval<GRAD>(U.uy, g.ny + 1, g.nz, g.nx - 1, g.ny, g.nz - 1, ot0, ow1, ot2, gr);
+  out[1] = bspline_eval<GRAD>(make_rsrc(U.uy, 4u * nx * (ny + 1) * nz), (ny + 1) * nz, nz, ot0, ow1, ot2, gr);
   if (GRAD) { float inv = 1.0f / g.dx; J[1][0] = gr[0] * inv; J[1][1] = gr[1] * inv; J[1][2] = gr[2] * inv; }
-  out[2] = bspline_eval<GRAD>(U.uz, g.ny, g.nz + 1, g.nx - 1, g.ny - 1, g.nz, ot0, ot1, ow2, gr);
+  out[2] = bspline_eval<GRAD>(make_rsrc(U.uz, 4u * nx * ny * (nz + 1)), ny * (nz + 1), nz + 1, ot0, ot1, ow2, gr);
   if (GRAD) { float inv = 1.0f / g.dx; J[2][0] = gr[0] * inv; J[2][1] = gr[1] * inv; J[2][2] = gr[2] * inv; }
 }
 
@@ -238,16 +278,18 @@ EDGE_DEV HPoint hermite_point(const Grid& g, const float p[3]) {
 // component from its table.  Corner indices are always inside the grid: the
 // clamp keeps the base in [1, dim-2].
 template <bool GRAD>
-EDGE_DEV float hermite_eval(const Grid& g, const float4* __restrict__ tab, const HPoint& P, float gr[3]) {
+EDGE_DEV float hermite_eval(const Grid& g, Rsrc tab, const HPoint& P, float gr[3]) {
+  const uint32_t s1 = (uint32_t)g.nz, s0 = (uint32_t)g.ny * s1;
+  const uint32_t vb = 32u * ((uint32_t)P.X.i0 * s0 + (uint32_t)P.Y.i0 * s1 + (uint32_t)P.Z.i0);
   float val = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
+      const uint32_t so = 32u * (i * s0 + j * s1);
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        int64_t q = cell_id(g, P.X.i0 + i, P.Y.i0 + j, P.Z.i0 + k);
-        float4 lo = tab[2 * q], hi = tab[2 * q + 1];
+        float4 lo = bload4(tab, vb + 32u * k, so), hi = bload4(tab, vb + 32u * k + 16u, so);
         // coefficient tensor c[a][b][cz] (a,b,cz in {f=0, g=1})
         float cfff = lo.x, cgff = lo.y, cfgf = lo.z, cffg = lo.w;
         float cfgg = hi.x, cgfg = hi.y, cggf = hi.z, cggg = hi.w;
@@ -289,7 +331,8 @@ EDGE_DEV float hermite_eval(const Grid& g, const float4* __restrict__ tab, const
 }
 
 // All three components of the map at p: value v and rows G (grad psi_c, 1/dx units).
-template <bool GRAD>
+// FENCE: evaluate the components one after another (fewer gathers in flight).
+template <bool GRAD, bool FENCE = false>
 EDGE_DEV void map_eval(const Grid& g, const float4* __restrict__ tab, const float p[3], float v[3], float G[3][3]) {
   HPoint P = hermite_point(g, p);
   int64_t N = ncell(g);
@@ -297,12 +340,13 @@ EDGE_DEV void map_eval(const Grid& g, const float4* __restrict__ tab, const floa
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float gr[3];
-    v[c] = hermite_eval<GRAD>(g, tab + 2 * N * c, P, gr);
+    v[c] = hermite_eval<GRAD>(g, make_rsrc(tab + 2 * N * c, (uint32_t)(32 * N)), P, gr);
     if (GRAD) {
       G[c][0] = gr[0] * inv;
       G[c][1] = gr[1] * inv;
       G[c][2] = gr[2] * inv;
     }
+    if (FENCE) EDGE_SCHED_FENCE();
   }
 }
 
@@ -326,7 +370,7 @@ EDGE_DEV void rk4_psi_T(const Grid& g, const Mac& U, float psi[3], float T[3][3]
     for (int a = 0; a < 3; ++a) Cs[b][a] = C[b][a];
   float su[3] = {0.f, 0.f, 0.f}, sC[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
   const float half = 0.5f * dt;
-#pragma unroll
+#pragma unroll 1
   for (int st = 0; st < 4; ++st) {
     float u[3], J[3][3];
     interp_u_mac<true>(g, U, ps, u, J);
@@ -366,7 +410,7 @@ EDGE_DEV void rk4_psi(const Grid& g, const Mac& U, float psi[3], float dt) {
   float ps[3] = {psi[0], psi[1], psi[2]};
   float su[3] = {0.f, 0.f, 0.f};
   const float half = 0.5f * dt;
-#pragma unroll
+#pragma unroll 1
   for (int st = 0; st < 4; ++st) {
     float u[3];
     interp_u_mac<false>(g, U, ps, u, nullptr);
@@ -389,7 +433,7 @@ EDGE_DEV void rk4_phi_F(const Grid& g, const Mac& U, float phi[3], float F[3], f
   float ps[3] = {phi[0], phi[1], phi[2]}, Fs[3] = {F[0], F[1], F[2]};
   float su[3] = {0.f, 0.f, 0.f}, sF[3] = {0.f, 0.f, 0.f};
   const float half = 0.5f * dt;
-#pragma unroll
+#pragma unroll 1
   for (int st = 0; st < 4; ++st) {
     float u[3], J[3][3];
     interp_u_mac<true>(g, U, ps, u, J);

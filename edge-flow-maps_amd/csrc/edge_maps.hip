@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) k_split_psi(Grid g, const float4* __restr
   if (!face_of(g, t, a, l, nf, i, j, k)) return;
   float X[3], psi[3], G[3][3];
   face_coord(a, i, j, k, g.dx, X);
-  map_eval<true>(g, tab, X, psi, G);
+  map_eval<true, true>(g, tab, X, psi, G);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     PHI.f[a][c * nf + l] = psi[c];
@@ -179,46 +179,6 @@ __global__ void __launch_bounds__(256) k_map_smoke_psi(Grid g, int mode, const f
   }
 }
 
-// split_to_mac + the 1/6 average of map_smoke_psi (run_edge.py:430-466),
-// stored per cell so the three smoke pull-backs of a step share it.
-__global__ void __launch_bounds__(256) k_cell_psi(Grid g, const float4* __restrict__ tab, float* __restrict__ out) {
-  const int64_t N = ncell(g);
-  const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (q >= N) return;
-  int i, j, k;
-  cell_ijk(g, q, i, j, k);
-  float acc[3] = {0.f, 0.f, 0.f};
-#pragma unroll 1
-  for (int f = 0; f < 6; ++f) {
-    int a = f >> 1, up = f & 1;
-    float X[3], v[3];
-    face_coord(a, i + (a == 0 ? up : 0), j + (a == 1 ? up : 0), k + (a == 2 ? up : 0), g.dx, X);
-    map_eval<false>(g, tab, X, v, nullptr);
-    acc[0] += v[0];
-    acc[1] += v[1];
-    acc[2] += v[2];
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) out[c * N + q] = (1.0f / 6.0f) * acc[c];
-}
-
-// map_smoke_psi / clamp_smoke_psi (run_edge.py:443-466, 480-505) at given
-// cell positions psic (3 planes).
-template <int NC>
-__global__ void __launch_bounds__(256) k_smoke_at(Grid g, int mode, const float* __restrict__ s0, float* __restrict__ s1,
-                                                  const float* __restrict__ psic) {
-  const int64_t N = ncell(g);
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= N) return;
-  float p[3] = {psic[q], psic[N + q], psic[2 * N + q]};
-  float out[NC], mn[NC], mx[NC];
-  linear_taps<NC>(g, s0, p, mode, out, mn, mx);
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    if (mode == 0) s1[c * N + q] = out[c];
-    else s1[c * N + q] = fminf(fmaxf(s1[c * N + q], mn[c]), mx[c]);
-  }
-}
 
 // run_edge.py:508-519 clamp_u over all three face grids: u2 = clamp(u, min, max
 // of the six neighbours in u1).
@@ -470,36 +430,6 @@ int edge_map_smoke_psi(int mode, const float* s0, float* s1, int nc, const float
   else
     hipLaunchKernelGGL(k_map_smoke_psi<1>, grid, dim3(256), 0, (hipStream_t)stream, g, mode, s0, s1,
                        (const float4*)tab);
-  EDGE_CHECK_LAUNCH();
-  return EDGE_OK;
-}
-
-// Cell positions of the map for the smoke pull-back (run_edge.py:443-466):
-// psic (3 planes) = mean of the map's values at the cell's six face centres.
-int edge_cell_psi(const float* tab, float* psic, int nx, int ny, int nz, float dx, void* stream) {
-  clear_error();
-  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_cell_psi: bad dims");
-  EDGE_REQUIRE(tab && psic, "edge_cell_psi: null pointer");
-  Grid g = make_grid(nx, ny, nz, dx);
-  hipLaunchKernelGGL(k_cell_psi, dim3(blocks_for(ncell(g), 256)), dim3(256), 0, (hipStream_t)stream, g,
-                     (const float4*)tab, psic);
-  EDGE_CHECK_LAUNCH();
-  return EDGE_OK;
-}
-
-// mode 0: s1 = interp_1(s0, psic); mode 1: s1 clamped to the 8-tap min/max of s0 at psic.
-int edge_smoke_at(int mode, const float* s0, float* s1, int nc, const float* psic, int nx, int ny, int nz, float dx,
-                  void* stream) {
-  clear_error();
-  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_smoke_at: bad dims");
-  EDGE_REQUIRE(s0 && s1 && psic && (mode == 0 || mode == 1), "edge_smoke_at: bad arguments");
-  EDGE_REQUIRE(nc == 4 || nc == 1, "edge_smoke_at: nc must be 1 or 4 (got %d)", nc);
-  Grid g = make_grid(nx, ny, nz, dx);
-  dim3 grid(blocks_for(ncell(g), 256));
-  if (nc == 4)
-    hipLaunchKernelGGL(k_smoke_at<4>, grid, dim3(256), 0, (hipStream_t)stream, g, mode, s0, s1, psic);
-  else
-    hipLaunchKernelGGL(k_smoke_at<1>, grid, dim3(256), 0, (hipStream_t)stream, g, mode, s0, s1, psic);
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }

@@ -29,7 +29,7 @@ __device__ __forceinline__ void write_map_row(const MapOut& o, int64_t N, int64_
 // (-,-,-), (+,+,-), (-,+,+), (+,-,+) of sim/hermite.py:1446-1449.  Each point is
 // RK4-marched with its Jacobian (-dt), composed with the old map, and the four
 // eps-points give dT by calc_ed4_garm_func (sim/hermite.py:1049-1075).
-__global__ void __launch_bounds__(256) k_march_edge(Grid g, Mac U, const float4* __restrict__ tab,
+__global__ void __launch_bounds__(256) EDGE_MARCH_ATTR k_march_edge(Grid g, Mac U, const float4* __restrict__ tab,
                                                     const float* __restrict__ X, MapOut o, float dt, float eps) {
   const int64_t N = ncell(g);
   const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(256) k_march_ed8(Grid g, Mac U, const float4* 
 // sim/hermite.py:1398-1421 advect_ps_garm up to the copies: reset_to_identity_1,
 // advance_map_1_transpose (-dt), interp_psi_hermite_garm (full Hermite).
 // dT / d2T follow by calc_fd_3_from_grad (edge_calc_fd3).
-__global__ void __launch_bounds__(256) k_march_ge(Grid g, Mac U, const float4* __restrict__ tab,
+__global__ void __launch_bounds__(256) EDGE_MARCH_ATTR k_march_ge(Grid g, Mac U, const float4* __restrict__ tab,
                                                   const float* __restrict__ X, MapOut o, float dt) {
   const int64_t N = ncell(g);
   const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -177,7 +177,7 @@ struct FaceMap {
   float* F[3];
 };
 
-__global__ void __launch_bounds__(256) k_advance_faces(Grid g, Mac U, FaceMap fm, float dt) {
+__global__ void __launch_bounds__(256) EDGE_FACES_ATTR k_advance_faces(Grid g, Mac U, FaceMap fm, float dt) {
   const int64_t nf0 = (int64_t)(g.nx + 1) * g.ny * g.nz;
   const int64_t nf1 = (int64_t)g.nx * (g.ny + 1) * g.nz;
   const int64_t nf2 = (int64_t)g.nx * g.ny * (g.nz + 1);
@@ -350,13 +350,13 @@ __global__ void __launch_bounds__(256) k_eval_points(Grid g, int what, const flo
     Axis3 C = bspline_axis(p[2] / g.dx - bl2, (float)((double)d2 - 2.0 - 1e-6));
     float gr[3];
     if (grad) {
-      val[t] = bspline_eval<true>(f, d1, d2, d0 - 1, d1 - 1, d2 - 1, A, B, C, gr);
+      val[t] = bspline_eval<true>(make_rsrc(f, 4u * d0 * d1 * d2), (uint32_t)d1 * d2, d2, A, B, C, gr);
       float inv = 1.0f / g.dx;
       grad[3 * t] = gr[0] * inv;
       grad[3 * t + 1] = gr[1] * inv;
       grad[3 * t + 2] = gr[2] * inv;
     } else {
-      val[t] = bspline_eval<false>(f, d1, d2, d0 - 1, d1 - 1, d2 - 1, A, B, C, gr);
+      val[t] = bspline_eval<false>(make_rsrc(f, 4u * d0 * d1 * d2), (uint32_t)d1 * d2, d2, A, B, C, gr);
     }
   } else if (what == 1) {
     float u[3], J[3][3];

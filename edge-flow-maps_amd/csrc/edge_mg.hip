@@ -36,6 +36,7 @@ using namespace edge;
 namespace {
 
 constexpr int MAXL = 12;
+constexpr int MAXB = 8;        // largest CG batch between host convergence checks
 constexpr int NB = 1024;       // blocks of every reduction kernel
 constexpr int BS = 256;
 
@@ -712,6 +713,150 @@ __global__ void __launch_bounds__(BOT_THREADS) k_bottom_rbgs(Lv L, int bt, const
     if (q_[c] < N) st(z, q_[c], zs[q_[c]]);
 }
 
+// The coarse end of the V-cycle in one workgroup: levels lt..L-1 (each at most
+// TAIL_MAX_CELLS cells) held in LDS, so their ~10 half-sweeps and transfers
+// cost barriers instead of kernel launches.  Same smoother, transfers,
+// storage type S and order of operations as the per-level kernels above.
+//   part 0: everything (bottom = the 20 red-black pairs in LDS);
+//   part 1: the way down to level L-1 (r[L-1] and every tail level written back);
+//   part 2: the way up from a z[L-1] given in global memory (dense bottom).
+// On entry level lt's r and z (first red half-sweep done) are in global
+// memory; on exit level lt's z is.
+constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_MAX_CELLS = 8192;
+struct Tail {
+  int lt, L;
+  Lv lv[MAXL];
+  void* r[MAXL];   // global level storage (S)
+  void* z[MAXL];
+  int off[MAXL];   // LDS offset (elements of S) of level l's r; z follows
+};
+
+template <class S>
+__device__ __forceinline__ void tail_smooth(const Lv& L, int bt, const S* r, S* z, int phase) {
+  const int half = (L.n0 * L.n1 * L.n2) >> 1;
+  for (int t = threadIdx.x; t < half; t += TAIL_THREADS) {
+    int i, j, k;
+    colour_cell(L, t, phase, i, j, k);
+    int q = (int)lid(L, i, j, k);
+    if (L.bm && L.bm[q] > 0) continue;
+    float ns = neighbor_sum(L, i, j, k, [&](int a, int b, int c) { return ld(z, lid(L, a, b, c)); });
+    st(z, q, (ld(r, q) + ns) / num_fluid(L, bt, i, j, k));
+  }
+  __syncthreads();
+}
+
+template <class S>
+__global__ void __launch_bounds__(TAIL_THREADS) k_vtail(Tail T, int bt, int part, const double* __restrict__ sc,
+                                                      int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  extern __shared__ __align__(16) unsigned char tail_lds[];
+  S* lds = (S*)tail_lds;
+  auto R = [&](int l) { return lds + T.off[l]; };
+  auto Z = [&](int l) { const Lv& L = T.lv[l]; return lds + T.off[l] + L.n0 * L.n1 * L.n2; };
+  auto ncells = [&](int l) { const Lv& L = T.lv[l]; return L.n0 * L.n1 * L.n2; };
+  const int last = T.L - 1;
+  if (part != 2) {
+    {
+      const int n = ncells(T.lt);
+      const S* gr = (const S*)T.r[T.lt];
+      const S* gz = (const S*)T.z[T.lt];
+      for (int q = threadIdx.x; q < n; q += TAIL_THREADS) {
+        R(T.lt)[q] = gr[q];
+        Z(T.lt)[q] = gz[q];
+      }
+      __syncthreads();
+    }
+    for (int l = T.lt; l < last; ++l) {
+      const Lv& F = T.lv[l];
+      const Lv& C = T.lv[l + 1];
+      S* rf = R(l);
+      S* zf = Z(l);
+      tail_smooth(F, bt, rf, zf, 1);
+      tail_smooth(F, bt, rf, zf, 0);
+      tail_smooth(F, bt, rf, zf, 1);
+      // restriction + fill(0) + first red half-sweep of level l+1 (as k_restrict)
+      S* rc = R(l + 1);
+      S* zc = Z(l + 1);
+      const int nc = ncells(l + 1);
+      for (int q = threadIdx.x; q < nc; q += TAIL_THREADS) {
+        int I, J, K;
+        decode(C, q, I, J, K);
+        float acc = 0.f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          int i = 2 * I + (c >> 2), j = 2 * J + ((c >> 1) & 1), k = 2 * K + (c & 1);
+          int f = (int)lid(F, i, j, k);
+          if (F.bm && F.bm[f] > 0) continue;
+          float ns = neighbor_sum(F, i, j, k, [&](int a, int b, int cc) { return ld(zf, lid(F, a, b, cc)); });
+          float res = ld(rf, f) - (num_fluid(F, bt, i, j, k) * ld(zf, f) - ns);
+          acc += res * 1.0f / 2.0f;
+        }
+        st(rc, q, acc);
+        float zv = 0.f;
+        if (((I + J + K) & 1) == 0 && !(C.bm && C.bm[q] > 0)) zv = ld(rc, q) / num_fluid(C, bt, I, J, K);
+        st(zc, q, zv);
+      }
+      __syncthreads();
+    }
+    if (part == 0) {
+      // bottom: the remaining 39 half-sweeps of sim/mgpcg.py:187-189 (black first)
+      for (int s2 = 0; s2 < 39; ++s2) tail_smooth(T.lv[last], bt, R(last), Z(last), (s2 & 1) == 0 ? 1 : 0);
+    } else {
+      // hand the tail levels to the dense bottom and the up part
+      for (int l = T.lt; l <= last; ++l) {
+        const int n = ncells(l);
+        S* gr = (S*)T.r[l];
+        S* gz = (S*)T.z[l];
+        for (int q = threadIdx.x; q < n; q += TAIL_THREADS) {
+          gr[q] = R(l)[q];
+          gz[q] = Z(l)[q];
+        }
+      }
+      return;
+    }
+  } else {
+    for (int l = T.lt; l <= last; ++l) {
+      const int n = ncells(l);
+      const S* gr = (const S*)T.r[l];
+      const S* gz = (const S*)T.z[l];
+      for (int q = threadIdx.x; q < n; q += TAIL_THREADS) {
+        R(l)[q] = gr[q];
+        Z(l)[q] = gz[q];
+      }
+    }
+    __syncthreads();
+  }
+  for (int l = last - 1; l >= T.lt; --l) {
+    const Lv& F = T.lv[l];
+    const Lv& C = T.lv[l + 1];
+    S* rf = R(l);
+    S* zf = Z(l);
+    const S* zc = Z(l + 1);
+    // prolongation + first black half-sweep (as k_prolong_black)
+    const int half = ncells(l) >> 1;
+    for (int t = threadIdx.x; t < half; t += TAIL_THREADS) {
+      int i, j, k;
+      colour_cell(F, t, 1, i, j, k);
+      int q = (int)lid(F, i, j, k);
+      if (F.bm && F.bm[q] > 0) continue;
+      float ns = neighbor_sum(F, i, j, k, [&](int a, int b, int c) {
+        return ld(zf, lid(F, a, b, c)) + ld(zc, lid(C, a >> 1, b >> 1, c >> 1));
+      });
+      st(zf, q, (ld(rf, q) + ns) / num_fluid(F, bt, i, j, k));
+    }
+    __syncthreads();
+    tail_smooth(F, bt, rf, zf, 0);
+    tail_smooth(F, bt, rf, zf, 1);
+    tail_smooth(F, bt, rf, zf, 0);  // l > 0 in the tail
+  }
+  {
+    const int n = ncells(T.lt);
+    S* gz = (S*)T.z[T.lt];
+    for (int q = threadIdx.x; q < n; q += TAIL_THREADS) gz[q] = Z(T.lt)[q];
+  }
+}
+
 // Bottom of the mixed V-cycle: z = B^-1 r over the fluid cells of the coarsest
 // level, B = A + c 1 1^T (SPD; equal to the pseudo-inverse on mean-free r), FP64.
 template <class S>
@@ -807,11 +952,54 @@ struct Solver {
   // CG iteration batches captured as a HIP graph (replayed on the caller's stream)
   int use_graph = 0;
   hipStream_t cap = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  int gbatch = 0;
+  hipGraphExec_t gexec[MAXB + 1] = {};  // one captured batch per size 1..MAXB
+  int hist[2] = {0, 0};                  // iterations of the last two solves (adaptive batches)
+  // coarse levels from tail_from on go to one workgroup (0: never)
+  int tail_from = 0;
 };
 
+// First level of the one-workgroup tail for the current setup (0 = no tail):
+// the first level >= 1 small enough for LDS, used around a dense bottom (the
+// tail replaces ~8 short launches by 2; with the red-black bottom the
+// dedicated bottom kernel is faster than the tail's generic sweeps).
+int tail_level(const Solver& s) {
+  if (!s.tail_from) return 0;
+  if (!(s.mode == 1 && s.dense_ok) || s.tail_from == s.L - 1) return 0;
+  return s.tail_from;
+}
+
+void drop_graph(Solver& s) {
+  for (auto& g : s.gexec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
 Red red(Solver& s) { return Red{s.partials, s.sc}; }
+
+// the tail plan of the current level set (masks included)
+Tail tail_plan(const Solver& s, size_t elem, int* lds_bytes) {
+  Tail T;
+  T.lt = tail_level(s);
+  T.L = s.L;
+  int off = 0;
+  for (int l = 0; l < MAXL; ++l) {
+    T.r[l] = nullptr;
+    T.z[l] = nullptr;
+    T.off[l] = 0;
+  }
+  for (int l = 0; l < s.L; ++l) {
+    T.lv[l] = s.lv[l];
+    T.r[l] = s.rl[l];
+    T.z[l] = s.zl[l];
+    if (l >= T.lt) {
+      T.off[l] = off;
+      off += 2 * (int)s.nl[l];
+    }
+  }
+  if (lds_bytes) *lds_bytes = off * (int)elem;
+  return T;
+}
 
 // the finishing launch of a reduction over `nb` blocks
 void finish(Solver& s, hipStream_t st, int nb, Fin fin, int guard) {
@@ -832,7 +1020,8 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
   S* r0 = (S*)s.rl[0];
   S* z0 = (S*)s.zl[0];
   hipLaunchKernelGGL(k_first0<S>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig);
-  for (int l = 0; l < L - 1; ++l) {
+  const int lt = tail_level(s);
+  for (int l = 0; l < (lt ? lt : L - 1); ++l) {
     S* rl = (S*)s.rl[l];
     S* zl = (S*)s.zl[l];
     int64_t half = s.nl[l] / 2;
@@ -843,10 +1032,22 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
     hipLaunchKernelGGL(k_restrict<S>, dim3(blocks_for(s.nl[l + 1], BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt,
                        rl, zl, (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
   }
-  // bottom
-  {
+  const bool dense = s.mode == 1 && s.dense_ok;
+  if (lt) {
+    int lds = 0;
+    Tail T = tail_plan(s, sizeof(S), &lds);
+    if (!dense) {
+      hipLaunchKernelGGL(k_vtail<S>, dim3(1), dim3(TAIL_THREADS), lds, st, T, s.bt, 0, sc, ig);
+    } else {
+      int l = L - 1;
+      hipLaunchKernelGGL(k_vtail<S>, dim3(1), dim3(TAIL_THREADS), lds, st, T, s.bt, 1, sc, ig);
+      hipLaunchKernelGGL(k_bottom_dense<S>, dim3(s.nf), dim3(256), 0, st, s.nf, s.fidx, s.Binv, (const S*)s.rl[l],
+                         (S*)s.zl[l], s.nl[l], sc, ig);
+      hipLaunchKernelGGL(k_vtail<S>, dim3(1), dim3(TAIL_THREADS), lds, st, T, s.bt, 2, sc, ig);
+    }
+  } else {
     int l = L - 1;
-    if (s.mode == 1 && s.dense_ok) {
+    if (dense) {
       hipLaunchKernelGGL(k_bottom_dense<S>, dim3(s.nf), dim3(256), 0, st, s.nf, s.fidx, s.Binv, (const S*)s.rl[l],
                          (S*)s.zl[l], s.nl[l], sc, ig);
     } else if (s.nl[l] <= BOT_THREADS * BOT_CPT) {
@@ -859,7 +1060,7 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
                            (const S*)s.rl[l], (S*)s.zl[l], (it & 1) == 0 ? 1 : 0, sc, ig);
     }
   }
-  for (int l = L - 2; l >= 0; --l) {
+  for (int l = (lt ? lt : L - 1) - 1; l >= 0; --l) {
     S* rl = (S*)s.rl[l];
     S* zl = (S*)s.zl[l];
     int64_t half = s.nl[l] / 2;
@@ -969,7 +1170,7 @@ void free_solver(Solver* s) {
                   (void*)s->sc, (void*)s->Binv, (void*)s->fidx})
     if (p) (void)hipFree(p);
   if (s->sc_host) (void)hipHostFree(s->sc_host);
-  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  drop_graph(*s);
   if (s->cap) (void)hipStreamDestroy(s->cap);
   delete s;
 }
@@ -1003,7 +1204,10 @@ int prepare_masks(Solver& s, hipStream_t st) {
     std::vector<int8_t> cm(s.nl[s.L - 1]);
     EDGE_CHECK_HIP(hipMemcpyAsync(cm.data(), s.bm[s.L - 1], cm.size(), hipMemcpyDeviceToHost, st));
     EDGE_CHECK_HIP(hipStreamSynchronize(st));
-    if (!s.dense_ok || cm != s.dense_mask) s.dense_ok = build_dense(s, cm);
+    if (!s.dense_ok || cm != s.dense_mask) {
+      s.dense_ok = build_dense(s, cm);
+      drop_graph(s);  // the bottom's matrix, size and the tail split may have changed
+    }
   }
   return EDGE_OK;
 }
@@ -1025,11 +1229,7 @@ void cg_iteration(Solver& s, hipStream_t st) {
 
 template <class S>
 int capture_batch(Solver& s, int batch) {
-  if (s.gexec && s.gbatch == batch) return EDGE_OK;
-  if (s.gexec) {
-    (void)hipGraphExecDestroy(s.gexec);
-    s.gexec = nullptr;
-  }
+  if (s.gexec[batch]) return EDGE_OK;
   if (!s.cap) EDGE_CHECK_HIP(hipStreamCreateWithFlags(&s.cap, hipStreamNonBlocking));
   hipGraph_t g = nullptr;
   EDGE_CHECK_HIP(hipStreamBeginCapture(s.cap, hipStreamCaptureModeThreadLocal));
@@ -1041,15 +1241,26 @@ int capture_batch(Solver& s, int batch) {
     set_error("edge_poisson: graph capture failed: %s", hipGetErrorString(le != hipSuccess ? le : e));
     return EDGE_EHIP;
   }
-  e = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&s.gexec[batch], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
-    s.gexec = nullptr;
+    s.gexec[batch] = nullptr;
     set_error("edge_poisson: graph instantiate failed: %s", hipGetErrorString(e));
     return EDGE_EHIP;
   }
-  s.gbatch = batch;
   return EDGE_OK;
+}
+
+// CG iterations to run before the next host check.  check_every >= 1: fixed
+// batches (at most MAXB).  0, adaptive: the first batch stops one short of the
+// fewest iterations of the last two solves and later batches are single
+// iterations, so few launches are spent after convergence (the kernels of
+// such an iteration return at once, but each launch still costs its slot).
+int next_batch(const Solver& s, int check_every, int done_iters) {
+  if (check_every >= 1) return std::min(check_every, MAXB);
+  if (done_iters > 0 || s.hist[0] <= 0) return 1;
+  int lo = s.hist[1] > 0 ? std::min(s.hist[0], s.hist[1]) : s.hist[0];
+  return std::max(1, std::min(lo - 1, MAXB));
 }
 
 template <class S>
@@ -1065,26 +1276,29 @@ int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* st
   int it = 0;
   int done = 0;
   const int maxit = max_iters < 0 ? (1 << 30) : max_iters;
-  if (s.use_graph) {
-    int rc = capture_batch<S>(s, check_every);
-    if (rc) return rc;
-  }
   while (!done) {
+    int batch = std::min(next_batch(s, check_every, it), std::max(1, maxit - it));
+    // a batch of b iterations: the captured graphs of sizes in b's binary expansion
     if (s.use_graph) {
-      EDGE_CHECK_HIP(hipGraphLaunch(s.gexec, st));
-      it += check_every;
+      for (int b = MAXB; b >= 1; b >>= 1)
+        if (batch & b) {
+          int rc = capture_batch<S>(s, b);
+          if (rc) return rc;
+          EDGE_CHECK_HIP(hipGraphLaunch(s.gexec[b], st));
+        }
     } else {
-      for (int b = 0; b < check_every; ++b) {
+      for (int b = 0; b < batch; ++b) {
         cg_iteration<S>(s, st);
         EDGE_CHECK_LAUNCH();
-        ++it;
-        if (it >= maxit) break;
       }
     }
+    it += batch;
     EDGE_CHECK_HIP(hipMemcpyAsync(s.sc_host, s.sc, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st));
     EDGE_CHECK_HIP(hipStreamSynchronize(st));
     done = s.sc_host[S_DONE] != 0.0 || it >= maxit;
   }
+  s.hist[1] = s.hist[0];
+  s.hist[0] = (int)s.sc_host[S_ITERS];
   if (stats) {
     stats[0] = s.sc_host[S_ITERS];
     stats[1] = s.sc_host[S_RTR];
@@ -1196,6 +1410,20 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
     std::vector<int8_t> cm(s->nl[levels - 1], 0);
     s->dense_ok = build_dense(*s, cm);
   }
+  for (int l = 1; l < levels; ++l)
+    if (s->nl[l] <= TAIL_MAX_CELLS) {
+      s->tail_from = l;
+      break;
+    }
+  if (s->tail_from) {
+    int bytes = 0;
+    for (int l = s->tail_from; l < levels; ++l) bytes += 2 * (int)s->nl[l] * (int)s->elem;
+    hipError_t e = mode == 1 ? hipFuncSetAttribute((const void*)k_vtail<__hip_bfloat16>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
+                             : hipFuncSetAttribute((const void*)k_vtail<float>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) s->tail_from = 0;  // per-level kernels instead
+  }
   if (hipDeviceSynchronize() != hipSuccess) return fail("hipDeviceSynchronize");
   *handle = s;
   return EDGE_OK;
@@ -1217,6 +1445,7 @@ int edge_mg_set_solid(void* handle, const int* bm, const float* bvel) {
   clear_error();
   EDGE_REQUIRE(handle, "edge_mg_set_solid: null handle");
   Solver& s = *(Solver*)handle;
+  drop_graph(s);  // captured launches hold the level masks
   if (!bm) {
     s.solid = false;
     for (int l = 0; l < s.L; ++l) s.lv[l].bm = nullptr;
@@ -1240,7 +1469,7 @@ int edge_mg_set_solid(void* handle, const int* bm, const float* bvel) {
 // pressure; f* (optional) get -(grad p) (Poisson_with_buffer).
 // stop_mode 0: the reference criterion; 1: ||r|| <= rel_tol ||b||.
 // x0 (optional): warm start, used only when ||b - A x0|| <= ||b||.
-// check_every: CG iterations between host convergence checks.
+// check_every: CG iterations between host convergence checks (0 = adaptive).
 // stats (host double[9]): iters, rTr, ||b||^2, fluid div sum, corrected,
 // converged, z0.r0, ||b - A x0||^2, x0 used.
 int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, float* fx, float* fy, float* fz,
@@ -1248,7 +1477,7 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
                  double* stats, void* stream) {
   clear_error();
   EDGE_REQUIRE(handle && ux && uy && uz, "edge_poisson: null argument");
-  EDGE_REQUIRE(check_every >= 1, "edge_poisson: check_every must be >= 1");
+  EDGE_REQUIRE(check_every >= 0, "edge_poisson: check_every must be >= 0 (0 = adaptive)");
   EDGE_REQUIRE(stop_mode == 0 || stop_mode == 1, "edge_poisson: bad stop_mode");
   Solver& s = *(Solver*)handle;
   hipStream_t st = (hipStream_t)stream;
@@ -1288,7 +1517,7 @@ int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int sto
                   double rel_tol, const float* x0, int check_every, double* stats, void* stream) {
   clear_error();
   EDGE_REQUIRE(handle && b && x, "edge_mg_solve: null argument");
-  EDGE_REQUIRE(check_every >= 1, "edge_mg_solve: check_every must be >= 1");
+  EDGE_REQUIRE(check_every >= 0, "edge_mg_solve: check_every must be >= 0 (0 = adaptive)");
   Solver& s = *(Solver*)handle;
   hipStream_t st = (hipStream_t)stream;
   int rc = set_scalars(s, st, max_iters, stop_mode, abs_tol, rel_tol);

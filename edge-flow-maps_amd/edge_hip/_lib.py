@@ -7,7 +7,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libedge_hip.so")
+# EDGE_HIP_LIBRARY: path of an alternative build of libedge_hip.so (e.g. one
+# compiled with other tuning knobs); default: the in-tree build.
+LIB_PATH = os.environ.get("EDGE_HIP_LIBRARY") or os.path.join(_HERE, "libedge_hip.so")
 
 _P, _I, _L, _F, _D = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_float, ctypes.c_double
 _CODES = {"p": _P, "i": _I, "l": _L, "f": _F, "d": _D}
@@ -42,8 +44,6 @@ SIGNATURES = {
     "edge_map_smoke": "ippipppiiifp",
     "edge_map_smoke_psi": "ippipiiifp",
     "edge_clamp_u": "pppppppppiiip",
-    "edge_cell_psi": "ppiiifp",
-    "edge_smoke_at": "ippipiiifp",
     "edge_advect_u_sl": "ppppppiiiffp",
     "edge_max_speed": "ppppiiip",
     "edge_axpy": "pppflp",

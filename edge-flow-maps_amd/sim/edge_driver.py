@@ -63,9 +63,9 @@ def _garm_table(m, dx):
 
 # Data derived from one map state and shared by the step's pull-backs: the
 # face map (psi and the column d psi / d x_a at every face, i.e.
-# split_psi_hermite of the garm map) for map_u_psi, and the cell positions for
-# map_smoke_psi / clamp_smoke_psi.  Keyed by the map's storage and version.
-_derived = {"key": None, "face": None, "psic": None, "bufs": {}}
+# split_psi_hermite of the garm map), read by map_u_psi, map_smoke_psi and
+# clamp_smoke_psi.  Keyed by the map's storage and version.
+_derived = {"key": None, "face": None, "bufs": {}}
 
 
 def _derived_for(m, dx):
@@ -74,7 +74,6 @@ def _derived_for(m, dx):
     if _derived["key"] != key:
         _derived["key"] = key
         _derived["face"] = None
-        _derived["psic"] = None
     return _derived
 
 
@@ -83,8 +82,7 @@ def _bufs(shape, device):
     if b is None:
         nx, ny, nz = shape
         fs = [(3, nx + 1, ny, nz), (3, nx, ny + 1, nz), (3, nx, ny, nz + 1)]
-        b = dict(phi=[torch.empty(f, device=device) for f in fs], F=[torch.empty(f, device=device) for f in fs],
-                 psic=torch.empty((3, nx, ny, nz), device=device))
+        b = dict(phi=[torch.empty(f, device=device) for f in fs], F=[torch.empty(f, device=device) for f in fs])
         _derived["bufs"][(shape, device)] = b
     return b
 
@@ -101,19 +99,6 @@ def face_map_of(m, dx):
                   *[t.data_ptr() for t in b["F"]], nx, ny, nz, float(dx), stream_handle())
         d["face"] = (b["phi"], b["F"])
     return d["face"]
-
-
-def cell_psi_of(m, dx):
-    """(3, nx, ny, nz) tensor: the smoke pull-back positions of each cell (cached)."""
-    d = _derived_for(m, dx)
-    if d["psic"] is None:
-        psi = m[0]
-        nx, ny, nz = psi.shape
-        b = _bufs(psi.shape, psi.data.device)
-        tab = _garm_table(m, dx)
-        _lib.call("edge_cell_psi", tab.data_ptr(), b["psic"].data_ptr(), nx, ny, nz, float(dx), stream_handle())
-        d["psic"] = b["psic"]
-    return d["psic"]
 
 
 def map_u_psi(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, dx, garm_map):
@@ -147,15 +132,17 @@ def clamp_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx):
 
 
 def _smoke_psi(mode, smoke0, smoke1, dx, garm_map):
+    # psi_c = 1/6 sum of psi at the cell's six faces (run_edge.py:448-465) is
+    # map_smoke's average of a face map, here the cached psi at the faces
     nx, ny, nz = smoke0.shape
-    psic = cell_psi_of(garm_map, dx)
-    _lib.call("edge_smoke_at", mode, as_ptr(smoke0), as_ptr(smoke1), smoke0.n or 1, psic.data_ptr(), nx, ny, nz,
-              float(dx), stream_handle())
+    phi, _ = face_map_of(garm_map, dx)
+    _lib.call("edge_map_smoke", mode, as_ptr(smoke0), as_ptr(smoke1), smoke0.n or 1, *[t.data_ptr() for t in phi],
+              nx, ny, nz, float(dx), stream_handle())
     touch(smoke1)
 
 
 def map_smoke_psi(smoke0, smoke1, dx, garm_map):
-    """run_edge.py:443-466 (cell positions shared by the step's three smoke pull-backs)."""
+    """run_edge.py:443-466 (face positions shared with map_u_psi)."""
     _smoke_psi(0, smoke0, smoke1, dx, garm_map)
 
 

@@ -8,9 +8,10 @@ keyword-only with defaults that reproduce the reference:
   stop        "reference" (rTr < max(abs_tol, rel_tol * z0'r0), sim/mgpcg.py:228)
               or "relative" (||r|| <= rel_tol ||b||, FP64 norms)
   check_every CG iterations between host convergence checks (the reference
-              reads a scalar every iteration: 1)
-  graph       replay each batch of check_every CG iterations as one captured
-              HIP graph (default: on for "mixed", off for "plain")
+              reads a scalar every iteration: 1; at most 8), or 0 for batches
+              sized from the last two solves (default for "mixed")
+  graph       replay each batch of CG iterations as captured HIP graphs
+              (default: on for "mixed", off for "plain")
   x0          warm start for Poisson(), used only if ||b - A x0|| <= ||b||
 After a solve, `self.p` holds the pressure (the reference's get_result(self.p),
 sim/mgpcg.py:438) and `self.stats` the solve statistics.
@@ -46,7 +47,7 @@ class _MGBase:
         self.stop = stop
         self.abs_tol = abs_tol
         self.rel_tol = rel_tol
-        self.check_every = check_every if check_every is not None else (1 if mode == "plain" else 4)
+        self.check_every = check_every if check_every is not None else (1 if mode == "plain" else 0)
         self.boundary_types = boundary_types
         self._bt = _bt_array(boundary_types)
         h = ctypes.c_void_p()

@@ -122,13 +122,6 @@ int edge_map_smoke(int mode, const float* s0, float* s1, int nc, const float* ph
 /* mode 0 map_smoke_psi (run_edge.py:443-466), 1 clamp_smoke_psi (480-505). */
 int edge_map_smoke_psi(int mode, const float* s0, float* s1, int nc, const float* tab, int nx, int ny, int nz,
                        float dx, void* stream);
-/* The smoke pull-back's cell positions (split_to_mac + the 1/6 average of
- * map_smoke_psi, run_edge.py:430-466), computed once per map: psic 3 planes. */
-int edge_cell_psi(const float* tab, float* psic, int nx, int ny, int nz, float dx, void* stream);
-/* map_smoke_psi (mode 0) / clamp_smoke_psi (mode 1) at precomputed positions
- * psic (run_edge.py:443-466, 480-505). */
-int edge_smoke_at(int mode, const float* s0, float* s1, int nc, const float* psic, int nx, int ny, int nz, float dx,
-                  void* stream);
 /* clamp_u (run_edge.py:508-519) on the three face grids. */
 int edge_clamp_u(const float* ux, const float* uy, const float* uz, const float* u1x, const float* u1y,
                  const float* u1z, float* u2x, float* u2y, float* u2z, int nx, int ny, int nz, void* stream);
@@ -172,7 +165,8 @@ int edge_mg_set_solid(void* handle, const int* bm, const float* bvel);
  * max(abs_tol, rel_tol z0'r0), sim/mgpcg.py:228,254; solid rTr < abs_tol,
  * sim/mgpcg_solid.py:281), 1 = ||r|| <= rel_tol ||b|| (FP64 norms).  x0
  * optional warm start (kept only if ||b - A x0|| <= ||b||).  check_every:
- * iterations between host convergence checks.  stats: host double[9]
+ * iterations between host convergence checks (at most 8), or 0 for batches
+ * sized from the solver's last two solves.  stats: host double[9]
  * {iters, rTr, ||b||^2, fluid div sum, corrected, converged, z0'r0,
  * ||b - A x0||^2, x0 used}.  Returns -4 (results written) when max_iters ran
  * out. */
@@ -186,8 +180,8 @@ int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int sto
  * and Ap = A p (get_Ap, sim/mgpcg.py:119-124). */
 int edge_mg_vcycle(void* handle, const float* r, float* z, void* stream);
 int edge_mg_apply_A(void* handle, const float* p, float* Ap, void* stream);
-/* Replay each batch of check_every CG iterations as one captured HIP graph
- * (enable=1) instead of launching its ~40 kernels one by one.  No
+/* Replay each batch of CG iterations as captured HIP graphs (enable=1)
+ * instead of launching their kernels one by one.  No
  * counterpart in the reference (its CG loop is host-driven, sim/mgpcg.py:234). */
 int edge_mg_set_graph(void* handle, int enable);
 /* 1 if the mixed solver uses the dense coarsest solve (0: red-black bottom). */

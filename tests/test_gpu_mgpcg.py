@@ -225,3 +225,38 @@ def test_drop_in_classes(orc32, dev):
     assert abs(solver.stats["iters"] - rst[0]) <= 1
     for a in range(3):
         assert scale_err(fields[a].to_numpy(), ru[a]) < 1e-4
+
+
+def test_vcycle_parity_deeper_hierarchy(orc32, dev):
+    # 64x64x32, 4 levels: every per-level kernel on levels 1-2 and the red-black bottom
+    n = (64, 64, 32)
+    rng = np.random.default_rng(11)
+    r = rng.normal(size=n)
+    r -= r.mean()
+    S = Solver(n, 4, 0)
+    assert scale_err(N_(S.vcycle(T_(r, dev))), orc32.vcycle(r, 4)) < 1e-5
+    bm = np.zeros(n, np.int32)
+    bm[20:30, 10:40, 8:20] = 1
+    S.set_solid(torch.from_numpy(bm).to(dev), T_(np.zeros((3,) + n, np.float32), dev))
+    fl = bm == 0
+    rr = r * fl
+    assert scale_err(N_(S.vcycle(T_(rr, dev)))[fl], orc32.vcycle(rr, 4, bm=bm)[fl]) < 1e-5
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_adaptive_batches_match_single_steps(dev, graph):
+    # mixed at 32x16x16, 3 levels: level 1 and the dense bottom run as the one-workgroup tail
+    from edge_hip import _lib
+    u = smooth_velocity(*N3, DX, seed=8)
+    ref = Solver(N3, 3, 1)
+    p1, st1, code = ref.poisson([T_(c, dev) for c in u], stop_mode=1, rel_tol=1e-6, check_every=1)
+    assert code == 0
+    S = Solver(N3, 3, 1)
+    _lib.call("edge_mg_set_graph", S.h, graph)
+    for rep in range(3):  # the batch plan uses the history of earlier solves
+        p, st, code = S.poisson([T_(c, dev) for c in u], stop_mode=1, rel_tol=1e-6, check_every=0)
+        assert code == 0 and st[0] == st1[0]
+        assert scale_err(N_(p), N_(p1)) < 1e-5
+    # a fixed batch larger than the iterations needed stops at the same iteration
+    p, st, code = S.poisson([T_(c, dev) for c in u], stop_mode=1, rel_tol=1e-6, check_every=8)
+    assert code == 0 and st[0] == st1[0]
