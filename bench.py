@@ -31,7 +31,48 @@ FP32_PEAK_TFLOPS = 157.3
 # compulsory FP32 reads + writes)
 MARCH_FLOP_PER_CELL = 33600.0      # fused EDGE march: 5 points x (RK4 with Jacobian + Hermite compose)
 MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0}  # one CG iteration
+MARCH_BYTES_PER_CELL = 180.0       # u 12 + old map 84 + new map 84 (compulsory FP32)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
+
+def workload_name(res, case):
+    if case == "cfg5":
+        return f"EDGE step {res[0]}x{res[1]}x{res[2]} one ring + laden particles, mixed projection"
+    return f"EDGE step {res[0]}x{res[1]}x{res[2]} four rings, mixed projection"
+
+
+def read_pmc(d, counter):
+    """{kernel short name: [per-dispatch value]} from a rocprofv3 --pmc csv directory."""
+    import csv
+    import glob
+    out = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            if r.get("Counter_Name") != counter:
+                continue
+            name = r["Kernel_Name"].replace("(anonymous namespace)::", "").replace("void ", "").split("(")[0]
+            out.setdefault(name.split("<")[0], []).append(float(r["Counter_Value"]))
+    return out
+
+
+def summarize_pmc(fetch_dir, write_dir, probe_mib):
+    """HBM bytes per launch of the march kernel from FETCH_SIZE / WRITE_SIZE (KiB per
+    dispatch), each scaled by what the counter reads for the copy probe of known size."""
+    import numpy as np
+    f, w = read_pmc(fetch_dir, "FETCH_SIZE"), read_pmc(write_dir, "WRITE_SIZE")
+    known_kib = probe_mib * 1024.0
+    # the probe is the last k_scale dispatch (the step's own k_scale launches come first)
+    f_probe, w_probe = f["k_scale"][-1], w["k_scale"][-1]
+    cf, cw = known_kib / f_probe, known_kib / w_probe
+    fm, wm = float(np.median(f["k_march_edge"])), float(np.median(w["k_march_edge"]))
+    rec = {"k_march_edge": {
+        "fetch_kib_per_launch": round(fm, 1), "write_kib_per_launch": round(wm, 1),
+        "copy_probe_mib": probe_mib, "copy_fetch_kib": round(f_probe, 1), "copy_write_kib": round(w_probe, 1),
+        "scale_fetch": round(cf, 4), "scale_write": round(cw, 4),
+        "hbm_bytes_per_launch": round((fm * cf + wm * cw) * 1024.0),
+        "launches": len(f["k_march_edge"]),
+    }}
+    return rec
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -41,7 +82,16 @@ def parse():
     ap.add_argument("--res", type=int, nargs=3, default=[256, 128, 128])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
-    ap.add_argument("--poisson-reps", type=int, default=3)
+    ap.add_argument("--poisson-reps", type=int, default=3, help="0 skips the same-RHS solver comparison")
+    ap.add_argument("--case", default="cfg2", choices=["cfg2", "cfg5"],
+                    help="cfg2: the headline EDGE step; cfg5: 512x256x256 with 16M laden particles")
+    ap.add_argument("--particles", type=int, default=None, help="particle count for --case cfg5")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, one plain copy kernel (k_scale, alpha 1) over MIB MiB: the known-size "
+                         "launch that scales rocprofv3's FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="turn two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script with "
+                         "--copy-probe into profiles/traffic.json and exit")
     return ap.parse_args()
 
 
@@ -130,6 +180,13 @@ def cpu_baseline(torch):
 
 def main():
     a = parse()
+    if a.summarize_pmc:
+        rec = summarize_pmc(*a.summarize_pmc, probe_mib=a.copy_probe or 1024)
+        rec["k_march_edge"]["workload"] = workload_name(a.res, a.case)
+        with open(TRAFFIC_FILE, "w") as fh:
+            json.dump(rec, fh, indent=1)
+        print(json.dumps(rec))
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
                                    "(multi-GPU slab decomposition is a design note only)"}))
@@ -137,15 +194,22 @@ def main():
     import torch
     from run_edge import Simulation
 
-    sim = Simulation("cfg2", solver_mode="mixed", stop="relative", rel_tol=1e-6, res=tuple(a.res), check_every=0)
+    res = tuple(a.res) if (a.case == "cfg2" or a.res != [256, 128, 128]) else (512, 256, 256)
+    sim = Simulation(a.case, solver_mode="mixed", stop="relative", rel_tol=1e-6, res=res, check_every=0,
+                     num_particles=a.particles)
     sim.initialize()
     for _ in range(a.warmup):
         sim.substep()
+    from edge_hip import timing
+    timing.reset()
+    timing.enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     walls, phases = run_steps(sim, a.steps, torch)
     torch.cuda.synchronize()
     total_ms = (time.perf_counter() - t0) * 1e3
+    timing.enable(False)
+    march_launch_ms = timing.durations("k_march_edge")
     ms_step = total_ms / a.steps
     ph = {k: statistics.median(p.get(k, 0.0) for p in phases) for k in phases[0]}
     pstats = sim.poisson_stats[-2 * a.steps:]
@@ -165,20 +229,44 @@ def main():
                      poisson_iters=[s["iters"] for s in sim.poisson_stats[-2 * a.steps:]])
         sim.solver = mixed_solver
 
-    pcmp = poisson_compare(sim, a.poisson_reps, torch)
+    pcmp = poisson_compare(sim, a.poisson_reps, torch) if a.poisson_reps > 0 else None
 
     ncell = sim.res_x * sim.res_y * sim.res_z
-    march_ms = ph.get("march", float("nan"))
+    wl = workload_name(res, a.case)
+    # the dominant kernel: the fused EDGE march, one launch per step, timed per launch with
+    # HIP events on its stream; FP32 vector-ALU bound (no matrix op on this path)
+    march_ms = statistics.mean(march_launch_ms)
     march_tflops = MARCH_FLOP_PER_CELL * ncell / (march_ms * 1e-3) / 1e12
-    # the projection's CG iteration, HBM-bound: bytes per iteration / ms per iteration
-    it_ms = pcmp["mixed"]["ms"] / max(pcmp["mixed"]["iters"], 1)   # includes per-solve setup
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        rec = json.load(open(TRAFFIC_FILE)).get("k_march_edge", {})
+        if rec.get("workload") == wl:
+            traffic = rec.get("hbm_bytes_per_launch")
+    roof_march = dict(bound="mfma", pipe="FP32 vector ALU; 157.3 TFLOP/s is MI355X's FP32 peak",
+                      achieved=round(march_tflops, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                      frac=round(march_tflops / FP32_PEAK_TFLOPS, 4), traffic=traffic, kernel="k_march_edge",
+                      flop_per_launch=MARCH_FLOP_PER_CELL * ncell, algorithmic_bytes_per_launch=MARCH_BYTES_PER_CELL * ncell,
+                      avg_launch_ms=round(march_ms, 4), launches=len(march_launch_ms))
+    # the projection's CG iteration (many kernels), HBM-bound: bytes per iteration / ms per iteration
+    if pcmp is not None:
+        it_ms = pcmp["mixed"]["ms"] / max(pcmp["mixed"]["iters"], 1)   # includes per-solve setup
+    else:
+        it_ms = (ph.get("poisson1", 0.0) + ph.get("poisson2", 0.0)) / max(sum(mixed_iters[-2:]), 1)
     mg_gbs = MG_BYTES_PER_CELL["mixed"] * ncell / (it_ms * 1e-3) / 1e9
-    roof_march = dict(bound="valu", achieved=round(march_tflops, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                      frac=round(march_tflops / FP32_PEAK_TFLOPS, 4), traffic=None, kernel="k_march_edge",
-                      time_ms=round(march_ms, 3))
     roof_mg = dict(bound="hbm", achieved=round(mg_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                   frac=round(mg_gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="mixed CG iteration",
+                   frac=round(mg_gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="mixed CG iteration (~30 launches)",
                    time_ms=round(it_ms, 4))
+
+    if a.copy_probe:
+        # one plain copy of known size for the counter runs (outside every timed region)
+        from edge_hip import _lib
+        from edge_hip.field import stream_handle
+        nfl = a.copy_probe * 1024 * 1024 // 4
+        src = torch.ones(nfl, device="cuda")
+        dst = torch.empty_like(src)
+        _lib.call("edge_scale", src.data_ptr(), 1.0, dst.data_ptr(), nfl, stream_handle())
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None if a.no_cpu else cpu_baseline(torch)
 
@@ -194,16 +282,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: four seeded vortex rings (Biot-Savart init on the GPU)",
-        "config": {"workload": f"EDGE step {a.res[0]}x{a.res[1]}x{a.res[2]} four rings, mixed projection",
-                   "cells": ncell, "reinit_every": sim.reinit_every, "poisson_stop": "||r|| <= 1e-6 ||b||",
-                   "solver": "mixed (BF16 multigrid, FP64 dots, dense FP64 coarsest)"},
+        "data": ("synthetic: four seeded vortex rings (Biot-Savart init on the GPU)" if a.case == "cfg2" else
+                 "synthetic: one vortex ring, particles uniform in its torus (numpy default_rng(0))"),
+        "config": {"workload": wl, "cells": ncell, "reinit_every": sim.reinit_every,
+                   "poisson_stop": "||r|| <= 1e-6 ||b||",
+                   "solver": "mixed (BF16 multigrid, FP64 dots, dense FP64 coarsest)",
+                   **({"particles": sim.laden_particle_num} if sim.lp is not None else {})},
         "phases_ms": {k: round(v, 3) for k, v in ph.items()},
         "poisson_iters_mixed": mixed_iters,
         "plain": plain,
         "poisson_same_rhs": pcmp,
-        "roofline": roof_march if march_ms >= it_ms * pcmp["mixed"]["iters"] else roof_mg,
-        "roofline_march": roof_march,
+        "roofline": roof_march,
         "roofline_cg_iteration": roof_mg,
         "cpu_baseline": cpu,
     }

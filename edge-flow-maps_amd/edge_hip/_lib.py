@@ -62,6 +62,14 @@ SIGNATURES = {
     "edge_mg_apply_A": "pppp",
     "edge_mg_dense_bottom": "p",
     "edge_mg_set_graph": "pi",
+    # particles
+    "edge_lp_acc": "pppiiipppllffffp",
+    "edge_lp_vel": "pppllfffffp",
+    "edge_lp_vel_diff": "pppiiipppllffffp",
+    "edge_lp_p2g": "ppppppllfffffiiip",
+    "edge_lp_pos": "ppllffffp",
+    "edge_lp_viscous": "ppppppiiiffp",
+    "edge_lp_max_speed": "plpp",
 }
 
 _RESTYPE = {"edge_last_error": ctypes.c_char_p}

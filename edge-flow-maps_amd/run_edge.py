@@ -8,7 +8,9 @@ Every kernel is a libedge_hip.so kernel; this file only sequences them.
 
 The case table keeps the reference's cases 0 and 1 (run_edge.py:25-60) and
 adds the benchmark configurations ("cfg1": 64x32x32 leapfrogging rings,
-"cfg2": 256x128x128 four seeded rings).  Where the reference copies a field
+"cfg2": 256x128x128 four seeded rings, "cfg5": 512x256x256 with 16M laden
+particles, whose steps add the particle forces of run_edge.py:627-641,661-667
+and the projection with a force buffer, run_edge.py:704-708).  Where the reference copies a field
 whose source is dead afterwards (u <- tmp_u), the storages are exchanged
 instead (Field.swap_storage); the values are identical.
 
@@ -27,8 +29,9 @@ if _HERE not in sys.path:
     sys.path.insert(0, _HERE)
 
 from edge_hip.field import Field, ti  # noqa: E402
-from sim.edge_driver import (calc_max_speed, clamp_smoke_psi, clamp_u3, map_smoke, map_smoke_psi, map_u,  # noqa: E402
-                             map_u_psi, march_phi_grid, reset_to_identity)
+from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smoke_psi, clamp_u3,  # noqa: E402
+                             map_smoke, map_smoke_psi, map_u, map_u_psi, march_phi_grid, reset_to_identity)
+from sim import lp as lp_kernels  # noqa: E402
 from sim.hermite import advect_psi_garm_ed4_fusion_unroll, advect_u_semi_lagrangian, reset_to_identity_2  # noqa
 from sim.init_conditions import init_four_vorts, init_rings, init_vorts_leapfrog, seeded_rings  # noqa: E402
 from sim.mgpcg import MGPCG_3  # noqa: E402
@@ -48,14 +51,36 @@ CASES = {
     # 256x128x128, dx = 1/128, four seeded rings R=0.15, core 0.025, strength 0.02, reinit 12
     "cfg2": dict(init="rings4", res=(256, 128, 128), visualize_dt=1e9, reinit_every=12, CFL=0.5, BFECC_clamp=True,
                  eps_frac=0.025, base_level=3),
+    # 512x256x256, dx = 1/256, one ring (R 0.25, core 0.08) carrying 16,777,216 laden particles seeded
+    # uniformly in the torus of tube radius 0.08 (numpy default_rng(seed)); the particle constants are
+    # those of the reference's ink cases (run_edge.py:80-140) with Re and St set for this unit domain
+    "cfg5": dict(init="torus", res=(512, 256, 256), visualize_dt=1e9, reinit_every=12, CFL=0.25, BFECC_clamp=True,
+                 eps_frac=0.025, base_level=3,
+                 lp=dict(max_num=16777216, Re=1.0e4, St=5.0e-3, gravity=1.0, density_ratio=0.6,
+                         epiral_radius_ratio=35.0, laden_radius=0.01, ring_R=0.25, tube=0.08)),
 }
+
+
+def torus_particles(n, centre, R, tube, seed=0):
+    """n points uniform in the solid torus (axis x) around `centre`: (3, n) float32."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((3, 0), np.float32)
+    c = np.asarray(centre, np.float64)
+    while out.shape[1] < n:
+        m = int((n - out.shape[1]) * 2.4) + 1024
+        p = np.stack([rng.uniform(-tube, tube, m), rng.uniform(-R - tube, R + tube, m),
+                      rng.uniform(-R - tube, R + tube, m)])
+        rho = np.sqrt(p[1] ** 2 + p[2] ** 2)
+        keep = p[0] ** 2 + (rho - R) ** 2 < tube ** 2
+        out = np.concatenate([out, (p[:, keep] + c[:, None]).astype(np.float32)], axis=1)
+    return np.ascontiguousarray(out[:, :n])
 
 
 class Simulation:
     """The fields of run_edge.py:174-254 and one step of its loop."""
 
     def __init__(self, case="cfg1", solver_mode="plain", stop="relative", rel_tol=1e-6, res=None,
-                 reinit_every=None, check_every=None, num_samples=2000, seed=0):
+                 reinit_every=None, check_every=None, num_samples=2000, seed=0, num_particles=None):
         cfg = dict(CASES[case])
         if res is not None:
             cfg["res"] = tuple(res)
@@ -104,6 +129,21 @@ class Simulation:
         # scratch of the reference's fused march (accepted, unused by the in-place kernel)
         self.psi_tmp = self.T_x_tmp = self.T_y_tmp = self.T_z_tmp = None
         self.dT_x_tmp = self.dT_y_tmp = self.dT_z_tmp = None
+        # laden particles (run_edge.py:235-254), when the case has them
+        self.lp = dict(cfg["lp"]) if "lp" in cfg else None
+        if self.lp is not None:
+            if num_particles is not None:
+                self.lp["max_num"] = int(num_particles)
+            n = self.lp["max_num"]
+            self.domain_range = (rx * dx, ry * dx, rz * dx)
+            self.laden_particles_pos = V(3, float, n)
+            self.laden_particles_vel = V(3, float, n)
+            self.laden_particles_vel_diff = V(3, float, n)
+            self.laden_particles_acc = V(3, float, n)
+            self.laden_particle_ref_radius = F(float, n)
+            self.drag_x, self.drag_y, self.drag_z = F(float, fx), F(float, fy), F(float, fz)
+            self.vis_x, self.vis_y, self.vis_z = F(float, fx), F(float, fy), F(float, fz)
+            self.laden_particle_num = n
         self.num_samples = num_samples
         self.seed = seed
         self.i = -1
@@ -135,6 +175,17 @@ class Simulation:
             dom = (self.res_x * self.dx, self.res_y * self.dx, self.res_z * self.dx)
             init_rings(self.X, self.u, self.smoke, self.tmp_smoke, seeded_rings(4, dom, seed=self.seed),
                        num_samples=self.num_samples)
+        elif init == "torus":
+            dom = np.array([self.res_x, self.res_y, self.res_z]) * self.dx
+            c = dom / 2
+            R, tube = self.lp["ring_R"], self.lp["tube"]
+            ring = dict(w=1.0, rad=R, delta=tube, c=tuple(c), ex=(0.0, 0.0, -1.0), ey=(0.0, 1.0, 0.0))
+            init_rings(self.X, self.u, self.smoke, self.tmp_smoke, [ring], num_samples=self.num_samples)
+            n = self.lp["max_num"]
+            pos = torus_particles(n, c, R, tube, seed=self.seed)
+            self.laden_particles_pos.from_numpy(pos.T)
+            self.laden_particles_vel.fill(0.0)
+            self.laden_particle_ref_radius.fill(self.lp["laden_radius"])
         else:
             raise ValueError(init)
         split_central_vector(self.u, self.u_x, self.u_y, self.u_z)
@@ -145,10 +196,13 @@ class Simulation:
         return (self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm, self.dT_y_garm,
                 self.dT_z_garm)
 
-    def poisson(self):
+    def poisson(self, buffers=None):
         if self.rhs_snapshots is not None:
             self.rhs_snapshots.append([self.u_x.data.clone(), self.u_y.data.clone(), self.u_z.data.clone()])
-        self.solver.Poisson(self.u_x, self.u_y, self.u_z, max_iters=self.poisson_max_iters)
+        if buffers is None:
+            self.solver.Poisson(self.u_x, self.u_y, self.u_z, max_iters=self.poisson_max_iters)
+        else:
+            self.solver.Poisson_with_buffer(self.u_x, self.u_y, self.u_z, *buffers, max_iters=self.poisson_max_iters)
         self.poisson_stats.append(dict(self.solver.stats, time=self.solver.last_time))
 
     # -- one step (run_edge.py:588-711) --------------------------------------------
@@ -157,7 +211,11 @@ class Simulation:
         j = self.i % self.reinit_every
         dx = self.dx
         max_speed = calc_max_speed(self.u_x, self.u_y, self.u_z)
-        curr_dt = self.CFL * dx / max_speed
+        if self.lp is not None:
+            max_speed2 = calc_max_particle_speed(self.laden_particles_vel)
+            curr_dt = min(self.CFL * dx / max_speed, self.CFL * dx / max_speed2)
+        else:
+            curr_dt = self.CFL * dx / max_speed
         if self.sub_t + curr_dt >= self.visualize_dt:
             curr_dt = self.visualize_dt - self.sub_t
             self.sub_t = 0.
@@ -176,6 +234,10 @@ class Simulation:
             self.init_u_y.copy_from(self.u_y)
             self.init_u_z.copy_from(self.u_z)
             self.init_smoke.copy_from(self.smoke)
+
+        if self.lp is not None:
+            self._mark("particles")
+            self.particle_forces(curr_dt)
 
         self._mark("sl")
         advect_u_semi_lagrangian(self.u_x, self.u_y, self.u_z, self.tmp_u_x, self.tmp_u_y, self.tmp_u_z, dx,
@@ -196,6 +258,13 @@ class Simulation:
                        self.u_z, dx, curr_dt)
 
         self._mark("maps")
+        if self.lp is not None:
+            # run_edge.py:661-667: the force pulled back through the face map, into init_u
+            map_u(self.vis_x, self.vis_y, self.vis_z, self.tmp_u_x, self.tmp_u_y, self.tmp_u_z, self.F_x, self.F_y,
+                  self.F_z, self.phi_x, self.phi_y, self.phi_z, dx)
+            add_fields(self.init_u_x, self.tmp_u_x, self.init_u_x, curr_dt)
+            add_fields(self.init_u_y, self.tmp_u_y, self.init_u_y, curr_dt)
+            add_fields(self.init_u_z, self.tmp_u_z, self.init_u_z, curr_dt)
         gm = self.garm_map
         map_u_psi(self.init_u_x, self.init_u_y, self.init_u_z, self.u_x, self.u_y, self.u_z, dx, gm)
         map_smoke_psi(self.init_smoke, self.smoke, dx, gm)
@@ -225,9 +294,41 @@ class Simulation:
             self.u_y.swap_storage(self.err_u_y)
             self.u_z.swap_storage(self.err_u_z)
         self._mark("poisson2")
-        self.poisson()
+        if self.lp is not None:
+            # run_edge.py:704-708
+            self.vis_x.fill(0.0)
+            self.vis_y.fill(0.0)
+            self.vis_z.fill(0.0)
+            self.poisson(buffers=(self.vis_x, self.vis_y, self.vis_z))
+        else:
+            self.poisson()
         self._mark("end")
         return curr_dt
+
+    def particle_forces(self, curr_dt):
+        """run_edge.py:627-641: particle drag and viscous force into vis_*."""
+        p, dx = self.lp, self.dx
+        n = self.laden_particle_num
+        Lx, Ly, Lz = self.domain_range
+        lp_kernels.update_particles_acc(self.u_x, self.u_y, self.u_z, self.laden_particles_vel,
+                                        self.laden_particles_pos, self.laden_particles_acc, dx, curr_dt, n, p["St"],
+                                        p["gravity"])
+        lp_kernels.update_particles_velocity(self.laden_particles_acc, self.laden_particles_vel,
+                                             self.laden_particles_pos, curr_dt, dx, n, Lx, Ly, Lz)
+        lp_kernels.calculate_velocity_diff(self.u_x, self.u_y, self.u_z, self.laden_particles_vel,
+                                           self.laden_particles_vel_diff, self.laden_particles_pos, dx, n, p["St"],
+                                           p["density_ratio"], p["epiral_radius_ratio"])
+        lp_kernels.laden_particles_back_with_computing_particle(
+            self.drag_x, self.drag_y, self.drag_z, self.u_x, self.u_y, self.u_z, self.laden_particles_vel,
+            self.laden_particles_vel_diff, self.laden_particles_pos, self.laden_particle_ref_radius, dx, n,
+            p["laden_radius"], Lx, Ly, Lz, self.res_x, self.res_y, self.res_z)
+        lp_kernels.update_particles_position(self.laden_particles_vel, self.laden_particles_pos, curr_dt, n, Lx, Ly,
+                                             Lz)
+        lp_kernels.calculate_viscous_force(self.u_x, self.u_y, self.u_z, self.vis_x, self.vis_y, self.vis_z, dx,
+                                           p["Re"])
+        add_fields(self.vis_x, self.drag_x, self.vis_x, 1.0)
+        add_fields(self.vis_y, self.drag_y, self.vis_y, 1.0)
+        add_fields(self.vis_z, self.drag_z, self.vis_z, 1.0)
 
     def velocity_numpy(self):
         return [self.u_x.to_numpy(), self.u_y.to_numpy(), self.u_z.to_numpy()]
@@ -238,9 +339,11 @@ def main(argv=None):
     ap.add_argument("-c", "--case", default="cfg1")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--solver", default="plain", choices=["plain", "mixed"])
+    ap.add_argument("--res", type=int, nargs=3, default=None)
+    ap.add_argument("--particles", type=int, default=None, help="particle count of a particle case")
     a = ap.parse_args(argv)
     case = int(a.case) if a.case.isdigit() else a.case
-    sim = Simulation(case, solver_mode=a.solver)
+    sim = Simulation(case, solver_mode=a.solver, res=a.res, num_particles=a.particles)
     sim.initialize()
     ti.sync()
     t0 = time.perf_counter()

@@ -30,6 +30,20 @@ def calc_max_speed(u_x, u_y, u_z):
     return float(buf.item())
 
 
+_pspeed_buf = {}
+
+
+def calc_max_particle_speed(laden_particles_vel):
+    """run_edge.py:258-262; returns max(1e-3, max |v|) over every particle slot (host sync)."""
+    dev = laden_particles_vel.data.device
+    buf = _pspeed_buf.get(dev)
+    if buf is None:
+        buf = _pspeed_buf[dev] = torch.zeros(1, dtype=torch.float32, device=dev)
+    _lib.call("edge_lp_max_speed", as_ptr(laden_particles_vel), laden_particles_vel.shape[0], buf.data_ptr(),
+              stream_handle())
+    return float(buf.item())
+
+
 def reset_to_identity(psi_x, psi_y, psi_z, T_x, T_y, T_z, dx):
     """run_edge.py:306-320: face maps phi_a = X_a, F_a = e_a."""
     nx, ny, nz = _cells_of_mac(psi_x)

@@ -17,7 +17,7 @@ a caller can observe are listed per function:
 """
 import ctypes
 
-from edge_hip import _lib
+from edge_hip import _lib, timing
 from edge_hip.field import Field, as_ptr, stream_handle, touch
 from edge_hip.tables import hermite_table
 
@@ -265,9 +265,10 @@ def advect_psi_garm_ed4_fusion_unroll(psi_tmp, T_x_tmp, T_y_tmp, T_z_tmp, dT_x_t
     nx, ny, nz = _dims(psi)
     _check_mac(u_x, u_y, u_z, nx, ny, nz)
     tab = hermite_table("garm", psi, (T_x, T_y, T_z), (dT_x, dT_y, dT_z), None, dx)
-    _lib.call("edge_march_edge", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X), as_ptr(psi),
-              as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), as_ptr(dT_x), as_ptr(dT_y), as_ptr(dT_z), nx, ny, nz,
-              float(dx), float(dt), float(epsilon), stream_handle())
+    with timing.span("k_march_edge"):
+        _lib.call("edge_march_edge", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X), as_ptr(psi),
+                  as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), as_ptr(dT_x), as_ptr(dT_y), as_ptr(dT_z), nx, ny, nz,
+                  float(dx), float(dt), float(epsilon), stream_handle())
     touch(psi, T_x, T_y, T_z, dT_x, dT_y, dT_z)
 
 

@@ -187,6 +187,34 @@ int edge_mg_set_graph(void* handle, int enable);
 /* 1 if the mixed solver uses the dense coarsest solve (0: red-black bottom). */
 int edge_mg_dense_bottom(void* handle);
 
+/* ---- Laden particles (sim/lp.py) ----
+ * Particle vectors (vel, pos, acc, diff) are 3 planes of n floats (the
+ * capacity, laden_particles_max_num); particles q < num are live.  The MAC
+ * velocity is the usual three face grids of an nx x ny x nz cell grid. */
+/* update_particles_acc (sim/lp.py:6-29); acc = 0 for q >= num. */
+int edge_lp_acc(const float* ux, const float* uy, const float* uz, int nx, int ny, int nz, const float* vel,
+                const float* pos, float* acc, long n, long num, float dx, float dt, float St, float gravity,
+                void* stream);
+/* update_particles_velocity (sim/lp.py:32-70). */
+int edge_lp_vel(const float* acc, float* vel, const float* pos, long n, long num, float dt, float dx, float Lx,
+                float Ly, float Lz, void* stream);
+/* calculate_velocity_diff (sim/lp.py:73-97). */
+int edge_lp_vel_diff(const float* ux, const float* uy, const float* uz, int nx, int ny, int nz, const float* vel,
+                     float* diff, const float* pos, long n, long num, float dx, float St, float density_ratio,
+                     float radius_ratio, void* stream);
+/* laden_particles_back_with_computing_particle (sim/lp.py:100-156): drag
+ * faces zeroed, then the P2G scatter of diff (r_p / radius)^3 (FP32 atomics). */
+int edge_lp_p2g(float* dragx, float* dragy, float* dragz, const float* diff, const float* pos, const float* ref_radius,
+                long n, long num, float dx, float radius, float Lx, float Ly, float Lz, int nx, int ny, int nz,
+                void* stream);
+/* update_particles_position (sim/lp.py:158-181). */
+int edge_lp_pos(const float* vel, float* pos, long n, long num, float dt, float Lx, float Ly, float Lz, void* stream);
+/* calculate_viscous_force (sim/lp.py:183-218) on the three face grids. */
+int edge_lp_viscous(const float* ux, const float* uy, const float* uz, float* visx, float* visy, float* visz, int nx,
+                    int ny, int nz, float dx, float Re, void* stream);
+/* calc_max_particle_speed (run_edge.py:258-262): one device float. */
+int edge_lp_max_speed(const float* vel, long n, float* out_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,27 @@ class OracleStep:
         self.X = np.stack([(i + .5) * dx, (j + .5) * dx, (k + .5) * dx]).astype(dtype)
         self.i = -1
         self.iters = []
+        self.lp = None
+
+    def set_particles(self, pos, vel, ref_radius, num, params):
+        """Laden particles (run_edge.py:235-254): pos/vel (3, n); params as the driver's lp dict."""
+        self.lp = dict(params)
+        self.pos = np.ascontiguousarray(pos, self.dt_)
+        self.vel = np.ascontiguousarray(vel, self.dt_)
+        self.rr = np.ascontiguousarray(ref_radius, self.dt_)
+        self.num = int(num)
+        self.domain = (self.nx * self.dx, self.ny * self.dx, self.nz * self.dx)
+
+    def particle_forces(self, dt):
+        """run_edge.py:627-641 with the oracle's sim/lp.py restatement."""
+        o, p, dx, L = self.o, self.lp, self.dx, self.domain
+        acc = o.lp_acc(self.u, self.vel, self.pos, self.num, dx, dt, p["St"], p["gravity"])
+        self.vel = o.lp_vel(acc, self.vel, self.pos, self.num, dt, dx, L)
+        diff = o.lp_vel_diff(self.u, self.vel, self.pos, self.num, dx, p["St"], p["density_ratio"],
+                             p["epiral_radius_ratio"])
+        drag = o.lp_p2g(diff, self.pos, self.rr, self.num, dx, p["laden_radius"], L, (self.nx, self.ny, self.nz))
+        self.pos = o.lp_pos(self.vel, self.pos, self.num, dt, L)
+        self.vis = [(o.lp_viscous(c, dx, p["Re"]) + 1.0 * d).astype(self.dt_) for c, d in zip(self.u, drag)]
 
     def poisson(self):
         self.u, p, st = self.o.poisson(self.u, self.levels, stop_mode=self.stop_mode, rel_tol=self.rel_tol,
@@ -47,6 +68,9 @@ class OracleStep:
         j = self.i % self.reinit_every
         if dt is None:
             dt = self.CFL * dx / o.max_speed(self.u)
+            if self.lp is not None:
+                vmax = max(1e-3, float(np.sqrt((self.vel.astype(np.float64) ** 2).sum(0)).max()))
+                dt = min(dt, self.CFL * dx / vmax)
         self.curr_dt = dt
         n = (self.nx, self.ny, self.nz)
         if j == 0:
@@ -60,11 +84,17 @@ class OracleStep:
             self.dT = np.zeros((3, 3) + n, self.dt_)
             self.init_u = [c.copy() for c in self.u]
             self.init_smoke = self.smoke.copy()
+        if self.lp is not None:
+            self.particle_forces(dt)
         self.u = o.advect_u_sl(self.u, dx, 0.5 * dt)
         self.poisson()
         self.psi, self.T, self.dT = o.march_edge(self.psi, self.T, self.dT, self.u, self.X, dx, dt, self.eps)
         for a in range(3):
             self.phi[a], self.F[a] = o.advance_map_col(self.phi[a], self.F[a], self.u, dx, dt)
+        if self.lp is not None:
+            # run_edge.py:661-667
+            tmp = o.map_u(self.vis, self.F, self.phi, dx)
+            self.init_u = [(i0 + dt * t).astype(self.dt_) for i0, t in zip(self.init_u, tmp)]
         self.u = o.map_u_psi(self.init_u, self.psi, self.T, self.dT, dx)
         self.smoke = o.map_smoke_psi(0, self.init_smoke, None, self.psi, self.T, self.dT, dx)
         err_u = o.map_u(self.u, self.F, self.phi, dx)

@@ -52,6 +52,33 @@ def test_one_step_fixed_iterations(dev, clamp):
     assert scale_err(np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref.psi) < 1e-5
 
 
+def test_particle_step_fixed_iterations(dev):
+    """Two steps of the particle case (drag, viscous force, P2G, force pull-back,
+    projection with a force buffer) against the oracle's composition."""
+    from run_edge import Simulation
+    from oracle.step import OracleStep
+    sim = Simulation("cfg5", solver_mode="plain", res=(32, 16, 16), num_samples=400, num_particles=3000)
+    sim.initialize()
+    u0 = sim.velocity_numpy()
+    s0 = np.moveaxis(sim.smoke.to_numpy(), -1, 0)
+    pos0 = np.moveaxis(sim.laden_particles_pos.to_numpy(), -1, 0)
+    sim.poisson_max_iters = 6
+    sim.solver.rel_tol = 0.0
+    ref = OracleStep(u0, s0, sim.dx, sim.reinit_every, sim.CFL, sim.ed_eps, levels=sim.solver.n_mg_levels,
+                     BFECC_clamp=sim.BFECC_clamp, stop_mode=1, rel_tol=0.0, max_iters=6)
+    ref.set_particles(pos0, np.zeros_like(pos0), sim.laden_particle_ref_radius.to_numpy(), sim.laden_particle_num,
+                      sim.lp)
+    for _ in range(2):
+        dt = sim.substep()
+        ref.substep(dt=dt)
+    torch.cuda.synchronize()
+    for g, r in zip(sim.velocity_numpy(), ref.u):
+        assert scale_err(g, r) < 1e-4
+    assert scale_err(np.moveaxis(sim.laden_particles_vel.to_numpy(), -1, 0), ref.vel) < 1e-4
+    assert scale_err(np.moveaxis(sim.laden_particles_pos.to_numpy(), -1, 0), ref.pos) < 1e-6
+    assert np.abs(ref.vel).max() > 0
+
+
 def test_ten_step_trajectory(dev):
     sim, ref = make((32, 16, 16))
     for s in range(10):

@@ -1,0 +1,116 @@
+"""Pin the oracle's particle kernels (sim/lp.py) with closed forms and an
+adjoint identity, so that a wrong offset, weight, sign, scale or wall rule in
+the oracle fails one of them.  CPU only."""
+import numpy as np
+
+from helpers import face_coords, linear_velocity
+
+NX, NY, NZ, DX = 16, 12, 10, 1.0 / 12
+L = (NX * DX, NY * DX, NZ * DX)
+
+
+def interior_particles(n, margin, seed):
+    rng = np.random.default_rng(seed)
+    lo = margin * DX
+    return np.stack([rng.uniform(lo, L[a] - lo, n) for a in range(3)])
+
+
+def test_acc_and_vel_diff_linear_flow(orc64):
+    A = np.array([[0.3, -1.2, 0.5], [2.0, 0.1, -0.7], [-0.4, 0.9, 1.1]])
+    b = np.array([0.2, -0.5, 1.0])
+    u = linear_velocity(A, b, NX, NY, NZ, DX)
+    n, num = 300, 250
+    pos = interior_particles(n, 2.5, 1)
+    vel = np.random.default_rng(2).normal(size=(3, n))
+    St, g, dt = 0.05, 1.3, 0.01
+    acc = orc64.lp_acc(u, vel, pos, num, DX, dt, St, g)
+    up = A @ pos + b[:, None]
+    k = 1.0 / St
+    want = (np.array([0.0, g, 0.0])[:, None] - k * (vel - up)) / (1 + k * dt)
+    np.testing.assert_allclose(acc[:, :num], want[:, :num], rtol=1e-10, atol=1e-10)
+    assert np.all(acc[:, num:] == 0)
+    dr, rr = 0.6, 3.0
+    diff = orc64.lp_vel_diff(u, vel, pos, num, DX, St, dr, rr)
+    np.testing.assert_allclose(diff[:, :num], (dr / St / rr ** 3) * (vel - up)[:, :num], rtol=1e-10, atol=1e-12)
+    assert np.all(diff[:, num:] == 0)
+
+
+def test_velocity_wall_rules(orc64):
+    # one particle per rule, each moving into its wall, and controls just outside each band
+    d = DX
+    cases = [  # (axis, position on that axis, velocity sign, zeroed?)
+        (1, 0.5 * d, -1, True), (1, 2.0 * d, -1, False),       # lower y wall uses dx
+        (1, L[1] - 2.0 * d, 1, True), (1, L[1] - 4.0 * d, 1, False),
+        (0, 2.0 * d, -1, True), (0, 4.0 * d, -1, False), (0, L[0] - 2.0 * d, 1, True),
+        (2, 2.0 * d, -1, True), (2, L[2] - 2.0 * d, 1, True), (2, L[2] - 2.0 * d, -1, False),
+    ]
+    n = len(cases)
+    pos = np.tile(np.array(L)[:, None] / 2, (1, n))
+    vel = np.zeros((3, n))
+    for q, (a, x, sgn, _) in enumerate(cases):
+        pos[a, q] = x
+        vel[:, q] = 0.3
+        vel[a, q] = 0.5 * sgn
+    acc = np.full((3, n), 2.0)
+    dt = 0.1
+    out = orc64.lp_vel(acc, vel, pos, n, dt, d, L)
+    for q, (a, x, sgn, zeroed) in enumerate(cases):
+        v = vel[:, q] + acc[:, q] * dt
+        want = v.copy()
+        if zeroed:
+            want[a] = 0.0
+        np.testing.assert_allclose(out[:, q], want, atol=1e-14)
+
+
+def test_p2g_is_the_adjoint_of_the_bspline_gather(orc64):
+    # sum_f drag_a(f) G(f) = sum_p diff_a(p) (r_p/r)^3 interp_2(G)(p) for every face field G:
+    # pins the taps, weights, offsets and scale; particles within 3 dx of a wall contribute nothing
+    n = 400
+    pos = interior_particles(n, 1.0, 3)
+    diff = np.random.default_rng(4).normal(size=(3, n))
+    rr = np.random.default_rng(5).uniform(0.5, 1.5, n)
+    radius = 0.9
+    num = 370
+    drag = orc64.lp_p2g(diff, pos, rr, num, DX, radius, L, (NX, NY, NZ))
+    live = np.arange(n) < num
+    inside = np.all((pos >= 3 * DX) & (pos <= np.array(L)[:, None] - 3 * DX), axis=0) & live
+    assert 50 < inside.sum() < num  # both kinds present
+    scale = (rr / radius) ** 3
+    rng = np.random.default_rng(6)
+    for a, BL in enumerate([(0, .5, .5), (.5, 0, .5), (.5, .5, 0)]):
+        G = rng.normal(size=drag[a].shape)
+        gathered = orc64.eval_bspline(G, BL, pos.T, DX, grad=False)
+        lhs = np.sum(drag[a] * G)
+        rhs = np.sum((diff[a] * scale * gathered)[inside])
+        assert abs(lhs - rhs) < 1e-10 * np.sum(np.abs(diff[a] * scale))
+        # partition of unity
+        assert abs(drag[a].sum() - np.sum((diff[a] * scale)[inside])) < 1e-10 * np.abs(diff[a]).sum()
+
+
+def test_position_update_and_clamp(orc64):
+    n = 6
+    pos = np.tile(np.array(L)[:, None] / 2, (1, n))
+    vel = np.zeros((3, n))
+    vel[0, 0], vel[1, 1], vel[2, 2] = -100.0, 100.0, 0.25
+    out = orc64.lp_pos(vel, pos, 5, 0.1, L)
+    assert out[0, 0] == 0.0 and out[1, 1] == L[1]
+    np.testing.assert_allclose(out[2, 2], L[2] / 2 + 0.025)
+    np.testing.assert_allclose(out[:, 3:5], pos[:, 3:5])
+    np.testing.assert_array_equal(out[:, 5], pos[:, 5])  # not live
+
+
+def test_viscous_force_quadratic_and_edge_clamp(orc64):
+    Re = 4.0
+    for a in range(3):
+        X = face_coords(a, NX, NY, NZ, DX)
+        I = X / DX  # index-space coordinates up to the 0.5 offsets
+        ca, cb, cc = 0.7, -0.3, 1.1
+        u = ca * I[0] ** 2 + cb * I[1] ** 2 + cc * I[2] ** 2
+        vis = orc64.lp_viscous(u, DX, Re)
+        np.testing.assert_allclose(vis[1:-1, 1:-1, 1:-1], 2 * (ca + cb + cc) / DX ** 2 / Re, rtol=1e-10)
+        # linear in i: zero inside; sample()'s edge clamp gives +slope at i = 0, -slope at the last i
+        lin = 2.0 * I[0]
+        v2 = orc64.lp_viscous(lin, DX, Re)
+        np.testing.assert_allclose(v2[1:-1], 0.0, atol=1e-9)
+        np.testing.assert_allclose(v2[0], 2.0 / DX ** 2 / Re, rtol=1e-10)
+        np.testing.assert_allclose(v2[-1], -2.0 / DX ** 2 / Re, rtol=1e-10)
