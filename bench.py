@@ -120,8 +120,9 @@ def run_steps(sim, k, torch):
 
 
 def poisson_compare(sim, reps, torch):
-    """Plain vs mixed projection of the same velocities (the inputs of one
-    step's two projections), to ||r|| <= 1e-6 ||b||."""
+    """Plain vs mixed (vs learned warm start + mixed, when the trained network is
+    present) projection of the same velocities (the inputs of one step's two
+    projections), to ||r|| <= 1e-6 ||b||."""
     from sim.mgpcg import MGPCG_3
     import numpy as np
     sim.rhs_snapshots = []
@@ -131,8 +132,12 @@ def poisson_compare(sim, reps, torch):
     bt = np.full((3, 2), 2, np.int32)
     N = [sim.res_x, sim.res_y, sim.res_z]
     res = {}
-    for mode in ("plain", "mixed"):
-        solver = MGPCG_3(bt, N, base_level=3, mode=mode, stop="relative", rel_tol=1e-6)
+    from sim.learned import load_default
+    net = load_default()
+    modes = ("plain", "mixed") + (("learned",) if net is not None else ())
+    for mode in modes:
+        solver = MGPCG_3(bt, N, base_level=3, mode="mixed" if mode == "learned" else mode, stop="relative",
+                         rel_tol=1e-6, warm_start=net if mode == "learned" else None)
         per = []
         for which, base in enumerate(rhs):
             times, iters, conv = [], [], True
@@ -150,11 +155,15 @@ def poisson_compare(sim, reps, torch):
                     iters.append(solver.stats["iters"])
                     conv = conv and bool(solver.stats["converged"])
             per.append(dict(ms=round(statistics.median(times), 3), iters=int(statistics.median(iters)),
-                            converged=conv))
+                            converged=conv, **({"x0_used": bool(solver.stats["x0_used"])} if mode == "learned"
+                                               else {})))
         res[mode] = dict(per_solve=per, ms=round(sum(p["ms"] for p in per), 3),
                          iters=sum(p["iters"] for p in per))
         del solver
     res["speedup_mixed_vs_plain"] = round(res["plain"]["ms"] / res["mixed"]["ms"], 3)
+    if "learned" in res:
+        res["speedup_learned_vs_plain"] = round(res["plain"]["ms"] / res["learned"]["ms"], 3)
+        res["learned_note"] = "mixed solver started from the U-Net's p0 (inference included in ms)"
     return res
 
 

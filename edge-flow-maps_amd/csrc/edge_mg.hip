@@ -1445,7 +1445,8 @@ int edge_mg_set_solid(void* handle, const int* bm, const float* bvel) {
   clear_error();
   EDGE_REQUIRE(handle, "edge_mg_set_solid: null handle");
   Solver& s = *(Solver*)handle;
-  drop_graph(s);  // captured launches hold the level masks
+  if (bm != s.user_bm || bvel != s.user_bvel || (bm != nullptr) != s.solid)
+    drop_graph(s);  // captured launches hold the mask pointers
   if (!bm) {
     s.solid = false;
     for (int l = 0; l < s.L; ++l) s.lv[l].bm = nullptr;
@@ -1510,6 +1511,37 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   if (p_out) EDGE_CHECK_HIP(hipMemcpyAsync(p_out, s.x, s.N * 4, hipMemcpyDeviceToDevice, st));
   if (cg == EDGE_ENOCONV) set_error("edge_poisson: no convergence within %d iterations", max_iters);
   return cg;
+}
+
+// The right-hand side edge_poisson solves for, without solving: u gets the
+// boundary conditions (in place, as edge_poisson does first) and b the
+// recentred -div u over fluid cells (sim/mgpcg.py:59-75, 169-175, 211-217).
+int edge_mg_rhs(void* handle, float* ux, float* uy, float* uz, float* b, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(handle && ux && uy && uz && b, "edge_mg_rhs: null argument");
+  Solver& s = *(Solver*)handle;
+  hipStream_t st = (hipStream_t)stream;
+  Faces3 U{{ux, uy, uz}};
+  int64_t nf = (int64_t)(s.nx + 1) * s.ny * s.nz + (int64_t)s.nx * (s.ny + 1) * s.nz + (int64_t)s.nx * s.ny * (s.nz + 1);
+  int rc = set_scalars(s, st, 1, 1, 0.0, 0.0);
+  if (rc) return rc;
+  rc = prepare_masks(s, st);
+  if (rc) return rc;
+  const Grid g = make_grid(s.nx, s.ny, s.nz, 1.f);
+  hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, g, s.bt, U, s.solid ? s.user_bm : nullptr,
+                     s.solid ? s.user_bvel : nullptr);
+  if (s.solid) {
+    hipLaunchKernelGGL(k_div_sum, dim3(red_blocks(s.N)), dim3(BS), 0, st, g, U, (const int8_t*)s.bm[0], red(s));
+    finish(s, st, red_blocks(s.N), F_DIVSUM, 0);
+  }
+  hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, g, U, nullptr,
+                     s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, nullptr, red(s));
+  finish(s, st, red_blocks(s.N), F_MEAN, 0);
+  EDGE_CHECK_LAUNCH();
+  rc = start_solve(s, st, nullptr);
+  if (rc) return rc;
+  EDGE_CHECK_HIP(hipMemcpyAsync(b, s.r, s.N * 4, hipMemcpyDeviceToDevice, st));
+  return EDGE_OK;
 }
 
 // Solve A x = b (b recentred first when all walls), same options as edge_poisson.

@@ -62,6 +62,20 @@ SIGNATURES = {
     "edge_mg_apply_A": "pppp",
     "edge_mg_dense_bottom": "p",
     "edge_mg_set_graph": "pi",
+    # learned warm start
+    "edge_mg_rhs": "pppppp",
+    "edge_conv3d_bf16_fwd": "pipipppiiiiip",
+    "edge_conv3d_bf16_bwd_data": "pipiippiiip",
+    "edge_conv3d_bf16_bwd_weight": "pipipipiiip",
+    "edge_unet_pool": "ppiiiip",
+    "edge_unet_pool_bwd": "ppiiiip",
+    "edge_unet_relu_bwd": "ppppiiip",
+    "edge_unet_input": "ppipppp",
+    "edge_unet_head": "ppppiipp",
+    "edge_unet_loss_head_bwd": "pppppiipppppp",
+    "edge_unet_pack_weights": "piippp",
+    "edge_adam": "ppppiffffffp",
+    "edge_zero_f32": "plp",
     # particles
     "edge_lp_acc": "pppiiipppllffffp",
     "edge_lp_vel": "pppllfffffp",

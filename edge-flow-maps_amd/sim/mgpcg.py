@@ -13,6 +13,9 @@ keyword-only with defaults that reproduce the reference:
   graph       replay each batch of CG iterations as captured HIP graphs
               (default: on for "mixed", off for "plain")
   x0          warm start for Poisson(), used only if ||b - A x0|| <= ||b||
+  warm_start  a learned initial guess (sim.learned.UNet, or True for the
+              committed network): Poisson() computes b, runs the network
+              and passes its p0 as x0 (the same acceptance rule applies)
 After a solve, `self.p` holds the pressure (the reference's get_result(self.p),
 sim/mgpcg.py:438) and `self.stats` the solve statistics.
 """
@@ -39,7 +42,7 @@ def _bt_array(boundary_types):
 
 class _MGBase:
     def __init__(self, boundary_types, N, base_level=3, levels=None, mode="plain", stop="reference",
-                 check_every=None, abs_tol=1e-12, rel_tol=1e-12, graph=None):
+                 check_every=None, abs_tol=1e-12, rel_tol=1e-12, graph=None, warm_start=None):
         self.N = [int(n) for n in N]
         self.dim = 3
         self.n_mg_levels = int(levels) if levels is not None else int(math.log2(min(self.N))) - base_level + 1
@@ -58,6 +61,14 @@ class _MGBase:
         self.p = Field(None, float, self.N)
         self.stats = {}
         self.last_time = 0.0
+        if warm_start is True:
+            from sim.learned import load_default
+            warm_start = load_default()
+            if warm_start is None:
+                raise RuntimeError("warm_start=True but the trained network (sim/unet_warmstart.npz) is missing")
+        self.warm_start = warm_start
+        self._b = None
+        self._bm = None   # int32 solid mask for the network input (MGPCG_Solid_3 sets it)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -77,6 +88,12 @@ class _MGBase:
         stats = (ctypes.c_double * 9)()
         sm = {"reference": 0, "relative": 1}[self.stop] if stop_mode is None else stop_mode
         t0 = time.perf_counter()
+        if self.warm_start is not None and x0 is None:
+            if self._b is None:
+                self._b = torch.empty(self.N, dtype=torch.float32, device=self.p.data.device)
+            _lib.call("edge_mg_rhs", self._h, as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), self._b.data_ptr(),
+                      stream_handle())
+            x0 = self.warm_start.forward(self._b, self._bm, n=self.N)
         st = _lib.call("edge_poisson", self._h, as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), as_ptr(self.p),
                        as_ptr(f[0]), as_ptr(f[1]), as_ptr(f[2]), int(max_iters), sm,
                        float(self.abs_tol if abs_tol is None else abs_tol),

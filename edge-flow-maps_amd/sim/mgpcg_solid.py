@@ -24,6 +24,7 @@ class MGPCG_Solid_3(_MGBase):
     def Poisson(self, u_x, u_y, u_z, verbose=False, max_iters=100, x0=None):
         # the mask fields may have been re-allocated (swap_storage); re-bind them
         _lib.call("edge_mg_set_solid", self._h, self.boundary_mask.ptr, self.boundary_vel.ptr)
+        self._bm = self.boundary_mask
         st = self._run_poisson(u_x, u_y, u_z, max_iters=max_iters, x0=x0, verbose=verbose)
         if verbose:
             print("[Poisson] Sum of fluid div: ", self.stats["div_sum"])

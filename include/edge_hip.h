@@ -173,6 +173,10 @@ int edge_mg_set_solid(void* handle, const int* bm, const float* bvel);
 int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, float* fx, float* fy, float* fz,
                  int max_iters, int stop_mode, double abs_tol, double rel_tol, const float* x0, int check_every,
                  double* stats, void* stream);
+/* The right-hand side b (nx*ny*nz) edge_poisson would solve for: u gets the
+ * boundary conditions in place (as edge_poisson does first), b = the
+ * recentred -div u on fluid cells (sim/mgpcg.py:59-75,169-175,211-217). */
+int edge_mg_rhs(void* handle, float* ux, float* uy, float* uz, float* b, void* stream);
 /* A x = b with the same solver (MGPCG.solve, sim/mgpcg.py:197-283, after init). */
 int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int stop_mode, double abs_tol,
                   double rel_tol, const float* x0, int check_every, double* stats, void* stream);
@@ -186,6 +190,41 @@ int edge_mg_apply_A(void* handle, const float* p, float* Ap, void* stream);
 int edge_mg_set_graph(void* handle, int enable);
 /* 1 if the mixed solver uses the dense coarsest solve (0: red-black bottom). */
 int edge_mg_dense_bottom(void* handle);
+
+/* ---- Learned warm start: 3-level 3D U-Net (16/32/64 channels) ----
+ * No counterpart in the reference (a new component).  Activations are
+ * channels-last BF16 [voxel][C]; convolution weights [Cout][27][Cin] with
+ * tap = (di*3 + dj)*3 + dk.  FP32 gradients and master weights. */
+/* 3x3x3 'same' convolution y = relu?(conv(concat(x0, upsample2(x1))) + bias);
+ * x1 (half resolution, nearest upsampling) may be NULL with C1 = 0. */
+int edge_conv3d_bf16_fwd(const void* x0, int C0, const void* x1, int C1, const void* w, const float* bias, void* y,
+                         int Cout, int relu, int n0, int n1, int n2, void* stream);
+/* Data gradient of that convolution: g0 [N][S0] += channels < S0, g1 [N/8][Cin-S0]
+ * += the others summed over each parent's children; wt = flipped, transposed W. */
+int edge_conv3d_bf16_bwd_data(const void* g, int Cg, const void* wt, int Cin, int S0, float* g0, float* g1, int n0,
+                              int n1, int n2, void* stream);
+/* Weight gradient dw [Cout][27][C0+C1] += sum_v g[v] x(v + tap). */
+int edge_conv3d_bf16_bwd_weight(const void* g, int Cout, const void* x0, int C0, const void* x1, int C1, float* dw,
+                                int n0, int n1, int n2, void* stream);
+/* 2x2x2 average pooling and its gradient. */
+int edge_unet_pool(const void* x, void* y, int C, int n0, int n1, int n2, void* stream);
+int edge_unet_pool_bwd(const float* gy, float* gx, int C, int n0, int n1, int n2, void* stream);
+/* Gradient through ReLU into BF16 (+ bias gradient when db). */
+int edge_unet_relu_bwd(const float* gy, const void* y, void* g, float* db, int N, int C, int relu, void* stream);
+/* Input features [N][8] from b (scaled by its RMS, returned in scale) and the solid mask. */
+int edge_unet_input(const float* b, const int* bm, int N, void* x, float* scale, double* work, void* stream);
+/* Output head p = scale (h . w + bias). */
+int edge_unet_head(const void* h, const float* w, const float* bias, const float* scale, int N, int C, float* p,
+                   void* stream);
+/* Mean-free squared error against a target pressure, its gradient and the head's gradients. */
+int edge_unet_loss_head_bwd(const float* p, const float* target, const float* scale, const void* h, const float* w,
+                            int N, int C, float* dp, float* gh, float* dw, float* db, double* work, void* stream);
+/* BF16 forward (and data-gradient) weights of one layer from its FP32 weights. */
+int edge_unet_pack_weights(const float* w, int Cout, int Cin, void* wf, void* wt, void* stream);
+/* One Adam step over a flat FP32 parameter vector. */
+int edge_adam(float* p, const float* g, float* m1, float* m2, int n, float lr, float b1, float b2, float eps, float c1,
+              float c2, void* stream);
+int edge_zero_f32(float* p, long n, void* stream);
 
 /* ---- Laden particles (sim/lp.py) ----
  * Particle vectors (vel, pos, acc, diff) are 3 planes of n floats (the
