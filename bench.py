@@ -36,6 +36,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def workload_name(res, case):
+    if case == "cfg3":
+        return f"EDGE step {res[0]}x{res[1]}x{res[2]} delta-wing plate, oblique inflow, solid mixed projection"
     if case == "cfg5":
         return f"EDGE step {res[0]}x{res[1]}x{res[2]} one ring + laden particles, mixed projection"
     return f"EDGE step {res[0]}x{res[1]}x{res[2]} four rings, mixed projection"
@@ -83,8 +85,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--poisson-reps", type=int, default=3, help="0 skips the same-RHS solver comparison")
-    ap.add_argument("--case", default="cfg2", choices=["cfg2", "cfg5"],
-                    help="cfg2: the headline EDGE step; cfg5: 512x256x256 with 16M laden particles")
+    ap.add_argument("--case", default="cfg2", choices=["cfg2", "cfg3", "cfg5"],
+                    help="cfg2: the headline EDGE step; cfg3: delta-wing plate with the solid projection; "
+                         "cfg5: 512x256x256 with 16M laden particles")
     ap.add_argument("--particles", type=int, default=None, help="particle count for --case cfg5")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, one plain copy kernel (k_scale, alpha 1) over MIB MiB: the known-size "
@@ -119,25 +122,35 @@ def run_steps(sim, k, torch):
     return walls, phases
 
 
+def make_solver(sim, mode, warm_start=None):
+    """A fresh projection solver for the simulation's grid (and its solids)."""
+    import numpy as np
+    from sim.mgpcg import MGPCG_3
+    from sim.mgpcg_solid import MGPCG_Solid_3
+    bt = np.full((3, 2), 2, np.int32)
+    N = [sim.res_x, sim.res_y, sim.res_z]
+    kw = dict(base_level=3, mode="mixed" if mode == "learned" else mode, stop="relative", rel_tol=1e-6,
+              warm_start=warm_start)
+    if sim.solid:
+        return MGPCG_Solid_3(bt, sim.boundary_mask, sim.boundary_vel, N, **kw)
+    return MGPCG_3(bt, N, **kw)
+
+
 def poisson_compare(sim, reps, torch):
     """Plain vs mixed (vs learned warm start + mixed, when the trained network is
     present) projection of the same velocities (the inputs of one step's two
     projections), to ||r|| <= 1e-6 ||b||."""
-    from sim.mgpcg import MGPCG_3
-    import numpy as np
+    import math
     sim.rhs_snapshots = []
     sim.substep()
     rhs = sim.rhs_snapshots
     sim.rhs_snapshots = None
-    bt = np.full((3, 2), 2, np.int32)
-    N = [sim.res_x, sim.res_y, sim.res_z]
     res = {}
     from sim.learned import load_default
     net = load_default()
     modes = ("plain", "mixed") + (("learned",) if net is not None else ())
     for mode in modes:
-        solver = MGPCG_3(bt, N, base_level=3, mode="mixed" if mode == "learned" else mode, stop="relative",
-                         rel_tol=1e-6, warm_start=net if mode == "learned" else None)
+        solver = make_solver(sim, mode, warm_start=net if mode == "learned" else None)
         per = []
         for which, base in enumerate(rhs):
             times, iters, conv = [], [], True
@@ -147,15 +160,17 @@ def poisson_compare(sim, reps, torch):
                 torch.cuda.synchronize()
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
-                solver.Poisson(sim.tmp_u_x, sim.tmp_u_y, sim.tmp_u_z)
+                solver.Poisson(sim.tmp_u_x, sim.tmp_u_y, sim.tmp_u_z, max_iters=sim.poisson_max_iters)
                 e1.record()
                 torch.cuda.synchronize()
                 if r > 0:
                     times.append(e0.elapsed_time(e1))
                     iters.append(solver.stats["iters"])
                     conv = conv and bool(solver.stats["converged"])
+            st = solver.stats
             per.append(dict(ms=round(statistics.median(times), 3), iters=int(statistics.median(iters)),
-                            converged=conv, **({"x0_used": bool(solver.stats["x0_used"])} if mode == "learned"
+                            converged=conv, rel_res=float("%.3g" % math.sqrt(max(st["rTr"], 0.0) / max(st["b2"], 1e-300))),
+                            **({"x0_used": bool(solver.stats["x0_used"])} if mode == "learned"
                                                else {})))
         res[mode] = dict(per_solve=per, ms=round(sum(p["ms"] for p in per), 3),
                          iters=sum(p["iters"] for p in per))
@@ -203,7 +218,7 @@ def main():
     import torch
     from run_edge import Simulation
 
-    res = tuple(a.res) if (a.case == "cfg2" or a.res != [256, 128, 128]) else (512, 256, 256)
+    res = tuple(a.res) if (a.case != "cfg5" or a.res != [256, 128, 128]) else (512, 256, 256)
     sim = Simulation(a.case, solver_mode="mixed", stop="relative", rel_tol=1e-6, res=res, check_every=0,
                      num_particles=a.particles)
     sim.initialize()
@@ -226,11 +241,8 @@ def main():
 
     plain = None
     if not a.no_plain:
-        from sim.mgpcg import MGPCG_3
-        import numpy as np
         mixed_solver = sim.solver
-        sim.solver = MGPCG_3(np.full((3, 2), 2, np.int32), [sim.res_x, sim.res_y, sim.res_z], base_level=3,
-                             mode="plain", stop="relative", rel_tol=1e-6)
+        sim.solver = make_solver(sim, "plain")
         run_steps(sim, 1, torch)  # warm the plain solver
         pw, pph = run_steps(sim, a.steps, torch)
         plain = dict(ms_per_step=round(statistics.mean(pw), 3),
@@ -291,8 +303,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": ("synthetic: four seeded vortex rings (Biot-Savart init on the GPU)" if a.case == "cfg2" else
-                 "synthetic: one vortex ring, particles uniform in its torus (numpy default_rng(0))"),
+        "data": {"cfg2": "synthetic: four seeded vortex rings (Biot-Savart init on the GPU)",
+                 "cfg3": "synthetic: uniform oblique inflow past a voxelised triangular plate",
+                 "cfg5": "synthetic: one vortex ring, particles uniform in its torus (numpy default_rng(0))"}[a.case],
         "config": {"workload": wl, "cells": ncell, "reinit_every": sim.reinit_every,
                    "poisson_stop": "||r|| <= 1e-6 ||b||",
                    "solver": "mixed (BF16 multigrid, FP64 dots, dense FP64 coarsest)",

@@ -8,7 +8,8 @@ Every kernel is a libedge_hip.so kernel; this file only sequences them.
 
 The case table keeps the reference's cases 0 and 1 (run_edge.py:25-60) and
 adds the benchmark configurations ("cfg1": 64x32x32 leapfrogging rings,
-"cfg2": 256x128x128 four seeded rings, "cfg5": 512x256x256 with 16M laden
+"cfg2": 256x128x128 four seeded rings, "cfg3": the reference's delta-wing
+case 2 with the solid projection, "cfg5": 512x256x256 with 16M laden
 particles, whose steps add the particle forces of run_edge.py:627-641,661-667
 and the projection with a force buffer, run_edge.py:704-708).  Where the reference copies a field
 whose source is dead afterwards (u <- tmp_u), the storages are exchanged
@@ -33,8 +34,10 @@ from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smok
                              map_smoke, map_smoke_psi, map_u, map_u_psi, march_phi_grid, reset_to_identity)
 from sim import lp as lp_kernels  # noqa: E402
 from sim.hermite import advect_psi_garm_ed4_fusion_unroll, advect_u_semi_lagrangian, reset_to_identity_2  # noqa
-from sim.init_conditions import init_four_vorts, init_rings, init_vorts_leapfrog, seeded_rings  # noqa: E402
+from sim.init_conditions import (init_delta, init_four_vorts, init_rings, init_vorts_leapfrog,  # noqa: E402
+                                 seeded_rings, set_boundary_mask_delta)
 from sim.mgpcg import MGPCG_3  # noqa: E402
+from sim.mgpcg_solid import MGPCG_Solid_3  # noqa: E402
 from util.taichi_utils import (add_fields, center_coords_func, scale_field, split_central_vector,  # noqa: E402
                                x_coords_func, y_coords_func, z_coords_func)
 
@@ -51,6 +54,9 @@ CASES = {
     # 256x128x128, dx = 1/128, four seeded rings R=0.15, core 0.025, strength 0.02, reinit 12
     "cfg2": dict(init="rings4", res=(256, 128, 128), visualize_dt=1e9, reinit_every=12, CFL=0.5, BFECC_clamp=True,
                  eps_frac=0.025, base_level=3),
+    # reference case 2 (run_edge.py:61-79): the delta-wing plate in an oblique inflow, solid projection
+    "cfg3": dict(init="delta", res=(256, 128, 128), visualize_dt=1e9, reinit_every=10, CFL=0.5, BFECC_clamp=True,
+                 eps_frac=0.02, base_level=3, solid=True),
     # 512x256x256, dx = 1/256, one ring (R 0.25, core 0.08) carrying 16,777,216 laden particles seeded
     # uniformly in the torus of tube radius 0.08 (numpy default_rng(seed)); the particle constants are
     # those of the reference's ink cases (run_edge.py:80-140) with Re and St set for this unit domain
@@ -99,10 +105,21 @@ class Simulation:
         kw = dict(mode=solver_mode, stop=stop, check_every=check_every)
         if "levels" in cfg:
             kw["levels"] = cfg["levels"]
-        self.solver = MGPCG_3(boundary_types=boundary_types, N=[rx, ry, rz], base_level=cfg.get("base_level", 3),
-                              rel_tol=rel_tol if stop == "relative" else 1e-12, **kw)
+        self.solid = bool(cfg.get("solid"))
+        if self.solid:
+            # run_edge.py:168-172
+            self.boundary_mask = ti.field(ti.i32, (rx, ry, rz))
+            self.boundary_vel = ti.Vector.field(3, float, (rx, ry, rz))
+            self.solver = MGPCG_Solid_3(boundary_types=boundary_types, boundary_mask=self.boundary_mask,
+                                        boundary_vel=self.boundary_vel, N=[rx, ry, rz],
+                                        base_level=cfg.get("base_level", 3),
+                                        rel_tol=rel_tol if stop == "relative" else 1e-12, **kw)
+            self.poisson_max_iters = 100   # sim/mgpcg_solid.py:415
+        else:
+            self.solver = MGPCG_3(boundary_types=boundary_types, N=[rx, ry, rz], base_level=cfg.get("base_level", 3),
+                                  rel_tol=rel_tol if stop == "relative" else 1e-12, **kw)
+            self.poisson_max_iters = 400   # sim/mgpcg.py:436
         self.poisson_stats = []
-        self.poisson_max_iters = 400   # sim/mgpcg.py:436
         V = ti.Vector.field
         F = ti.field
         cell, fx, fy, fz = (rx, ry, rz), (rx + 1, ry, rz), (rx, ry + 1, rz), (rx, ry, rz + 1)
@@ -175,6 +192,10 @@ class Simulation:
             dom = (self.res_x * self.dx, self.res_y * self.dx, self.res_z * self.dx)
             init_rings(self.X, self.u, self.smoke, self.tmp_smoke, seeded_rings(4, dom, seed=self.seed),
                        num_samples=self.num_samples)
+        elif init == "delta":
+            # run_edge.py:545-548 (the per-step dye emission of run_edge.py:643-648 is not part of this build)
+            init_delta(self.X, self.u, self.smoke, self.tmp_smoke)
+            set_boundary_mask_delta(self.X, self.boundary_mask, self.boundary_vel, 0, self.dx)
         elif init == "torus":
             dom = np.array([self.res_x, self.res_y, self.res_z]) * self.dx
             c = dom / 2

@@ -99,3 +99,73 @@ def init_rings(X, u, smoke1, smoke2, rings, num_samples=2000):
         add_vortex_ring_and_smoke(r["w"], r["rad"], r["delta"], r["c"], r["ex"], r["ey"], X, u, smoke2,
                                   colours[q % 4], num_samples)
         add_fields(smoke1, smoke2, smoke1, 1.0)
+
+
+# ---------------------------------------------------------------------------
+# Delta-wing plate in an oblique inflow (sim/init_conditions.py:127-175)
+# ---------------------------------------------------------------------------
+DELTA_SPEED = 0.6
+DELTA_ATTACK_DEG = 20.0
+
+
+def voxelize_triangle(vertices, dx, subsamples=4):
+    """Cells (i, j) of a pitch-dx grid covered by a triangle in the z = 0 plane.
+
+    The reference voxelizes its plate with trimesh (`mesh.voxelized(dx)`, not
+    available here); this marks every cell of the triangle's bounding box with
+    at least one of subsamples^2 sample points inside the triangle.  Returns
+    the (nx, ny) bool array and the grid origin (x0, y0) of cell (0, 0)."""
+    v = np.asarray(vertices, np.float64)[:, :2]
+    lo = np.floor(v.min(0) / dx) * dx
+    hi = np.ceil(v.max(0) / dx) * dx
+    n = np.maximum(np.round((hi - lo) / dx).astype(int), 1)
+    s = (np.arange(subsamples) + 0.5) / subsamples
+    gi, gj, si, sj = np.meshgrid(np.arange(n[0]), np.arange(n[1]), s, s, indexing="ij")
+    px = lo[0] + (gi + si) * dx
+    py = lo[1] + (gj + sj) * dx
+
+    def side(a, b):
+        return (b[0] - a[0]) * (py - a[1]) - (b[1] - a[1]) * (px - a[0])
+    d0, d1, d2 = side(v[0], v[1]), side(v[1], v[2]), side(v[2], v[0])
+    inside = ((d0 >= 0) & (d1 >= 0) & (d2 >= 0)) | ((d0 <= 0) & (d1 <= 0) & (d2 <= 0))
+    return inside.any(axis=(2, 3)), lo
+
+
+def set_boundary_mask_delta(X, boundary_mask, boundary_vel, _t, dx):
+    """sim/init_conditions.py:127-167: the triangular plate (half-angle 15
+    degrees, length 1, four cells thick, leading edge at (0.1, 0.5, 0.2)) and
+    the box's outer cell layer as solid; the outer layer moves with the
+    inflow (0.6 cos 20, 0, 0.6 sin 20), the plate is at rest."""
+    shape = boundary_mask.shape
+    ang = np.radians(15.0)
+    vert = np.array([[0.0, 0.0, 0.0], [np.cos(ang), np.sin(ang), 0.0], [np.cos(ang), -np.sin(ang), 0.0]])
+    plate, _ = voxelize_triangle(vert, dx)
+    size = plate.shape
+    # a flat triangle voxelizes one cell thick: z offset 0.2 // dx (- 1 // 2)
+    off = np.array([int(0.1 // dx), int(0.5 // dx) - size[1] // 2, int(0.2 // dx)], np.int64)
+    mask = np.zeros(shape, np.int32)
+    sx = min(size[0], shape[0] - off[0])
+    sy = min(size[1], shape[1] - off[1])
+    for t in range(4):
+        mask[off[0]:off[0] + sx, off[1]:off[1] + sy, off[2] + t] += plate[:sx, :sy]
+    mask[mask > 0] = 1
+    vel = np.zeros(shape + (3,), np.float32)
+    a = np.radians(DELTA_ATTACK_DEG)
+    edge = np.zeros(shape, bool)
+    edge[[0, -1], :, :] = True
+    edge[:, [0, -1], :] = True
+    edge[:, :, [0, -1]] = True
+    mask[edge] = 1
+    vel[edge] = (DELTA_SPEED * np.cos(a), 0.0, DELTA_SPEED * np.sin(a))
+    boundary_mask.from_numpy(mask)
+    boundary_vel.from_numpy(vel)
+
+
+def init_delta(X, u, smoke1, smoke2):
+    """sim/init_conditions.py:170-175: uniform oblique inflow, no dye."""
+    a = np.radians(DELTA_ATTACK_DEG)
+    n = u.shape
+    v = np.empty(n + (3,), np.float32)
+    v[...] = (DELTA_SPEED * np.cos(a), 0.0, DELTA_SPEED * np.sin(a))
+    u.from_numpy(v)
+    smoke1.fill(0.0)
