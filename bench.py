@@ -30,12 +30,17 @@ FP32_PEAK_TFLOPS = 157.3
 # algorithmic work per unit (minimal separable counts; FMA = 2 flops; bytes =
 # compulsory FP32 reads + writes)
 MARCH_FLOP_PER_CELL = 33600.0      # fused EDGE march: 5 points x (RK4 with Jacobian + Hermite compose)
+# ED8: 8 points x (RK4 of the point + full-Hermite value); GE: 1 point (RK4 with Jacobian + Hermite gradient)
+SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 21400.0, "ge": 6800.0}
+SCHEME_BYTES_PER_CELL = {"edge": 180.0, "ed8": 204.0, "ge": 324.0}
 MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0}  # one CG iteration
 MARCH_BYTES_PER_CELL = 180.0       # u 12 + old map 84 + new map 84 (compulsory FP32)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def workload_name(res, case):
+def workload_name(res, case, scheme="edge"):
+    if scheme != "edge":
+        return f"{scheme.upper()} step {res[0]}x{res[1]}x{res[2]} four rings, mixed projection"
     if case == "cfg3":
         return f"EDGE step {res[0]}x{res[1]}x{res[2]} delta-wing plate, oblique inflow, solid mixed projection"
     if case == "cfg5":
@@ -89,6 +94,8 @@ def parse():
                     help="cfg2: the headline EDGE step; cfg3: delta-wing plate with the solid projection; "
                          "cfg5: 512x256x256 with 16M laden particles")
     ap.add_argument("--particles", type=int, default=None, help="particle count for --case cfg5")
+    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge"],
+                    help="backward-map march: EDGE (run_edge.py), ED8 (run_ed8.py) or GE (run_ge.py)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, one plain copy kernel (k_scale, alpha 1) over MIB MiB: the known-size "
                          "launch that scales rocprofv3's FETCH_SIZE / WRITE_SIZE")
@@ -220,7 +227,7 @@ def main():
 
     res = tuple(a.res) if (a.case != "cfg5" or a.res != [256, 128, 128]) else (512, 256, 256)
     sim = Simulation(a.case, solver_mode="mixed", stop="relative", rel_tol=1e-6, res=res, check_every=0,
-                     num_particles=a.particles)
+                     num_particles=a.particles, scheme=a.scheme)
     sim.initialize()
     for _ in range(a.warmup):
         sim.substep()
@@ -233,7 +240,8 @@ def main():
     torch.cuda.synchronize()
     total_ms = (time.perf_counter() - t0) * 1e3
     timing.enable(False)
-    march_launch_ms = timing.durations("k_march_edge")
+    march_kernel = {"edge": "k_march_edge", "ed8": "k_march_ed8", "ge": "k_march_ge"}[a.scheme]
+    march_launch_ms = timing.durations(march_kernel)
     ms_step = total_ms / a.steps
     ph = {k: statistics.median(p.get(k, 0.0) for p in phases) for k in phases[0]}
     pstats = sim.poisson_stats[-2 * a.steps:]
@@ -253,20 +261,21 @@ def main():
     pcmp = poisson_compare(sim, a.poisson_reps, torch) if a.poisson_reps > 0 else None
 
     ncell = sim.res_x * sim.res_y * sim.res_z
-    wl = workload_name(res, a.case)
+    wl = workload_name(res, a.case, a.scheme)
     # the dominant kernel: the fused EDGE march, one launch per step, timed per launch with
     # HIP events on its stream; FP32 vector-ALU bound (no matrix op on this path)
     march_ms = statistics.mean(march_launch_ms)
-    march_tflops = MARCH_FLOP_PER_CELL * ncell / (march_ms * 1e-3) / 1e12
+    flop_cell = SCHEME_FLOP_PER_CELL[a.scheme]
+    march_tflops = flop_cell * ncell / (march_ms * 1e-3) / 1e12
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if os.path.exists(TRAFFIC_FILE) and a.scheme == "edge":
         rec = json.load(open(TRAFFIC_FILE)).get("k_march_edge", {})
         if rec.get("workload") == wl:
             traffic = rec.get("hbm_bytes_per_launch")
     roof_march = dict(bound="mfma", pipe="FP32 vector ALU; 157.3 TFLOP/s is MI355X's FP32 peak",
                       achieved=round(march_tflops, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                      frac=round(march_tflops / FP32_PEAK_TFLOPS, 4), traffic=traffic, kernel="k_march_edge",
-                      flop_per_launch=MARCH_FLOP_PER_CELL * ncell, algorithmic_bytes_per_launch=MARCH_BYTES_PER_CELL * ncell,
+                      frac=round(march_tflops / FP32_PEAK_TFLOPS, 4), traffic=traffic, kernel=march_kernel,
+                      flop_per_launch=flop_cell * ncell, algorithmic_bytes_per_launch=SCHEME_BYTES_PER_CELL[a.scheme] * ncell,
                       avg_launch_ms=round(march_ms, 4), launches=len(march_launch_ms))
     # the projection's CG iteration (many kernels), HBM-bound: bytes per iteration / ms per iteration
     if pcmp is not None:

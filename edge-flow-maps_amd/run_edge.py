@@ -33,7 +33,9 @@ from edge_hip.field import Field, ti  # noqa: E402
 from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smoke_psi, clamp_u3,  # noqa: E402
                              map_smoke, map_smoke_psi, map_u, map_u_psi, march_phi_grid, reset_to_identity)
 from sim import lp as lp_kernels  # noqa: E402
-from sim.hermite import advect_psi_garm_ed4_fusion_unroll, advect_u_semi_lagrangian, reset_to_identity_2  # noqa
+from sim.edge_driver import clamp_smoke, split_psi_hermite  # noqa: E402
+from sim.hermite import (advect_ps_garm, advect_ps_hermite_ed8, advect_psi_garm_ed4_fusion_unroll,  # noqa: E402
+                         advect_u_semi_lagrangian, reset_to_identity_2, reset_to_identity_3)
 from sim.init_conditions import (init_delta, init_four_vorts, init_rings, init_vorts_leapfrog,  # noqa: E402
                                  seeded_rings, set_boundary_mask_delta)
 from sim.mgpcg import MGPCG_3  # noqa: E402
@@ -86,7 +88,12 @@ class Simulation:
     """The fields of run_edge.py:174-254 and one step of its loop."""
 
     def __init__(self, case="cfg1", solver_mode="plain", stop="relative", rel_tol=1e-6, res=None,
-                 reinit_every=None, check_every=None, num_samples=2000, seed=0, num_particles=None):
+                 reinit_every=None, check_every=None, num_samples=2000, seed=0, num_particles=None, scheme="edge"):
+        """scheme: "edge" (run_edge.py), "ed8" (run_ed8.py) or "ge" (run_ge.py): the backward
+        map's march; ED8 and GE keep a full Hermite map and pull back through its face values."""
+        if scheme not in ("edge", "ed8", "ge"):
+            raise ValueError(f"scheme must be edge, ed8 or ge, got {scheme!r}")
+        self.scheme = scheme
         cfg = dict(CASES[case])
         if res is not None:
             cfg["res"] = tuple(res)
@@ -143,6 +150,11 @@ class Simulation:
         self.psi_garm = V(3, float, cell)
         self.T_x_garm, self.T_y_garm, self.T_z_garm = V(3, float, cell), V(3, float, cell), V(3, float, cell)
         self.dT_x_garm, self.dT_y_garm, self.dT_z_garm = V(3, float, cell), V(3, float, cell), V(3, float, cell)
+        if scheme != "edge":
+            # run_ed8.py / run_ge.py: the map also holds d2T; psi and T at the faces (split_psi_hermite)
+            self.d2T_x, self.d2T_y, self.d2T_z = F(float, cell), F(float, cell), F(float, cell)
+            self.psi_x, self.psi_y, self.psi_z = V(3, float, fx), V(3, float, fy), V(3, float, fz)
+            self.T_x, self.T_y, self.T_z = V(3, float, fx), V(3, float, fy), V(3, float, fz)
         # scratch of the reference's fused march (accepted, unused by the in-place kernel)
         self.psi_tmp = self.T_x_tmp = self.T_y_tmp = self.T_z_tmp = None
         self.dT_x_tmp = self.dT_y_tmp = self.dT_z_tmp = None
@@ -249,8 +261,12 @@ class Simulation:
 
         if j == 0:
             reset_to_identity(self.phi_x, self.phi_y, self.phi_z, self.F_x, self.F_y, self.F_z, dx)
-            reset_to_identity_2(self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm,
-                                self.dT_y_garm, self.dT_z_garm, self.X)
+            if self.scheme == "edge":
+                reset_to_identity_2(self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm,
+                                    self.dT_y_garm, self.dT_z_garm, self.X)
+            else:
+                reset_to_identity_3(self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm,
+                                    self.dT_y_garm, self.dT_z_garm, self.d2T_x, self.d2T_y, self.d2T_z, self.X)
             self.init_u_x.copy_from(self.u_x)
             self.init_u_y.copy_from(self.u_y)
             self.init_u_z.copy_from(self.u_z)
@@ -270,10 +286,14 @@ class Simulation:
         self.poisson()
 
         self._mark("march")
-        advect_psi_garm_ed4_fusion_unroll(self.psi_tmp, self.T_x_tmp, self.T_y_tmp, self.T_z_tmp, self.dT_x_tmp,
-                                          self.dT_y_tmp, self.dT_z_tmp, self.psi_garm, self.T_x_garm, self.T_y_garm,
-                                          self.T_z_garm, self.dT_x_garm, self.dT_y_garm, self.dT_z_garm, self.u_x,
-                                          self.u_y, self.u_z, self.X, dx, curr_dt, self.ed_eps)
+        if self.scheme == "edge":
+            advect_psi_garm_ed4_fusion_unroll(self.psi_tmp, self.T_x_tmp, self.T_y_tmp, self.T_z_tmp, self.dT_x_tmp,
+                                              self.dT_y_tmp, self.dT_z_tmp, self.psi_garm, self.T_x_garm,
+                                              self.T_y_garm, self.T_z_garm, self.dT_x_garm, self.dT_y_garm,
+                                              self.dT_z_garm, self.u_x, self.u_y, self.u_z, self.X, dx, curr_dt,
+                                              self.ed_eps)
+        else:
+            self.march_full_map(curr_dt)
         self._mark("phi_march")
         march_phi_grid(self.phi_x, self.phi_y, self.phi_z, self.F_x, self.F_y, self.F_z, self.u_x, self.u_y,
                        self.u_z, dx, curr_dt)
@@ -286,24 +306,37 @@ class Simulation:
             add_fields(self.init_u_x, self.tmp_u_x, self.init_u_x, curr_dt)
             add_fields(self.init_u_y, self.tmp_u_y, self.init_u_y, curr_dt)
             add_fields(self.init_u_z, self.tmp_u_z, self.init_u_z, curr_dt)
-        gm = self.garm_map
-        map_u_psi(self.init_u_x, self.init_u_y, self.init_u_z, self.u_x, self.u_y, self.u_z, dx, gm)
-        map_smoke_psi(self.init_smoke, self.smoke, dx, gm)
-        map_u(self.u_x, self.u_y, self.u_z, self.err_u_x, self.err_u_y, self.err_u_z, self.F_x, self.F_y, self.F_z,
-              self.phi_x, self.phi_y, self.phi_z, dx)
+        if self.scheme == "edge":
+            gm = self.garm_map
+
+            def pull_u(u0, u1):
+                map_u_psi(*u0, *u1, dx, gm)
+
+            def pull_s(s0, s1, clamp=False):
+                (clamp_smoke_psi if clamp else map_smoke_psi)(s0, s1, dx, gm)
+        else:
+            # run_ed8.py:409-433: through the face values of the full map
+            def pull_u(u0, u1):
+                map_u(*u0, *u1, self.T_x, self.T_y, self.T_z, self.psi_x, self.psi_y, self.psi_z, dx)
+
+            def pull_s(s0, s1, clamp=False):
+                (clamp_smoke if clamp else map_smoke)(s0, s1, self.psi_x, self.psi_y, self.psi_z, dx)
+        U, IU = (self.u_x, self.u_y, self.u_z), (self.init_u_x, self.init_u_y, self.init_u_z)
+        EU, TU = (self.err_u_x, self.err_u_y, self.err_u_z), (self.tmp_u_x, self.tmp_u_y, self.tmp_u_z)
+        pull_u(IU, U)
+        pull_s(self.init_smoke, self.smoke)
+        map_u(*U, *EU, self.F_x, self.F_y, self.F_z, self.phi_x, self.phi_y, self.phi_z, dx)
         map_smoke(self.smoke, self.err_smoke, self.phi_x, self.phi_y, self.phi_z, dx)
-        for e, i0 in ((self.err_u_x, self.init_u_x), (self.err_u_y, self.init_u_y), (self.err_u_z, self.init_u_z),
-                      (self.err_smoke, self.init_smoke)):
+        for e, i0 in zip(EU + (self.err_smoke,), IU + (self.init_smoke,)):
             add_fields(e, i0, e, -1.)
-        for e in (self.err_u_x, self.err_u_y, self.err_u_z, self.err_smoke):
+        for e in EU + (self.err_smoke,):
             scale_field(e, 0.5, e)
-        map_u_psi(self.err_u_x, self.err_u_y, self.err_u_z, self.tmp_u_x, self.tmp_u_y, self.tmp_u_z, dx, gm)
-        map_smoke_psi(self.err_smoke, self.tmp_smoke, dx, gm)
-        add_fields(self.u_x, self.tmp_u_x, self.err_u_x, -1.)
-        add_fields(self.u_y, self.tmp_u_y, self.err_u_y, -1.)
-        add_fields(self.u_z, self.tmp_u_z, self.err_u_z, -1.)
+        pull_u(EU, TU)
+        pull_s(self.err_smoke, self.tmp_smoke)
+        for u_, t_, e_ in zip(U, TU, EU):
+            add_fields(u_, t_, e_, -1.)
         add_fields(self.smoke, self.tmp_smoke, self.smoke, -1.)
-        clamp_smoke_psi(self.init_smoke, self.smoke, dx, gm)
+        pull_s(self.init_smoke, self.smoke, clamp=True)
         if self.BFECC_clamp:
             clamp_u3(self.err_u_x, self.err_u_y, self.err_u_z, self.u_x, self.u_y, self.u_z, self.tmp_u_x,
                      self.tmp_u_y, self.tmp_u_z)
@@ -325,6 +358,19 @@ class Simulation:
             self.poisson()
         self._mark("end")
         return curr_dt
+
+    def march_full_map(self, curr_dt):
+        """run_ed8.py:226-230 / run_ge.py:220-231: the ED8 or GE march of the full
+        Hermite map, then psi and T at the faces."""
+        dx = self.dx
+        full = (self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm, self.dT_y_garm,
+                self.dT_z_garm, self.d2T_x, self.d2T_y, self.d2T_z)
+        if self.scheme == "ed8":
+            advect_ps_hermite_ed8(*([None] * 8), *full, self.u_x, self.u_y, self.u_z, self.X, dx, curr_dt,
+                                  self.ed_eps)
+        else:
+            advect_ps_garm(None, None, None, None, *full, self.u_x, self.u_y, self.u_z, self.X, dx, curr_dt)
+        split_psi_hermite(self.psi_x, self.psi_y, self.psi_z, self.T_x, self.T_y, self.T_z, full, dx)
 
     def particle_forces(self, curr_dt):
         """run_edge.py:627-641: particle drag and viscous force into vis_*."""
@@ -362,9 +408,10 @@ def main(argv=None):
     ap.add_argument("--solver", default="plain", choices=["plain", "mixed"])
     ap.add_argument("--res", type=int, nargs=3, default=None)
     ap.add_argument("--particles", type=int, default=None, help="particle count of a particle case")
+    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge"])
     a = ap.parse_args(argv)
     case = int(a.case) if a.case.isdigit() else a.case
-    sim = Simulation(case, solver_mode=a.solver, res=a.res, num_particles=a.particles)
+    sim = Simulation(case, solver_mode=a.solver, res=a.res, num_particles=a.particles, scheme=a.scheme)
     sim.initialize()
     ti.sync()
     t0 = time.perf_counter()

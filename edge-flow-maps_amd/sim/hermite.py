@@ -217,9 +217,10 @@ def advect_ps_hermite_ed8(psi_000, psi_001, psi_010, psi_011, psi_100, psi_101, 
     nx, ny, nz = _dims(psi)
     _check_mac(u_x, u_y, u_z, nx, ny, nz)
     tab = hermite_table("full", psi, (T_x, T_y, T_z), (dT_x, dT_y, dT_z), (d2T_x, d2T_y, d2T_z), dx)
-    _lib.call("edge_march_ed8", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X), as_ptr(psi),
-              as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), as_ptr(dT_x), as_ptr(dT_y), as_ptr(dT_z), as_ptr(d2T_x),
-              as_ptr(d2T_y), as_ptr(d2T_z), nx, ny, nz, float(dx), float(dt), float(epsilon), stream_handle())
+    with timing.span("k_march_ed8"):
+        _lib.call("edge_march_ed8", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X), as_ptr(psi),
+                  as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), as_ptr(dT_x), as_ptr(dT_y), as_ptr(dT_z), as_ptr(d2T_x),
+                  as_ptr(d2T_y), as_ptr(d2T_z), nx, ny, nz, float(dx), float(dt), float(epsilon), stream_handle())
     touch(psi, T_x, T_y, T_z, dT_x, dT_y, dT_z, d2T_x, d2T_y, d2T_z)
 
 
@@ -253,8 +254,9 @@ def advect_ps_garm(psi_tmp, T_x_tmp, T_y_tmp, T_z_tmp, psi, T_x, T_y, T_z, dT_x,
     nx, ny, nz = _dims(psi)
     _check_mac(u_x, u_y, u_z, nx, ny, nz)
     tab = hermite_table("full", psi, (T_x, T_y, T_z), (dT_x, dT_y, dT_z), (d2T_x, d2T_y, d2T_z), dx)
-    _lib.call("edge_march_ge", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X), as_ptr(psi),
-              as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), nx, ny, nz, float(dx), float(dt), stream_handle())
+    with timing.span("k_march_ge"):
+        _lib.call("edge_march_ge", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X), as_ptr(psi),
+                  as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), nx, ny, nz, float(dx), float(dt), stream_handle())
     touch(psi, T_x, T_y, T_z)
     calc_fd_3_from_grad(T_x, T_y, T_z, dT_x, dT_y, dT_z, d2T_x, d2T_y, d2T_z, dx)
 

@@ -6,7 +6,7 @@ every kernel is the oracle's literal restatement (edge_oracle*.cpp).
 """
 import numpy as np
 
-from .oracle import Oracle
+from .oracle import HK_FULL, Oracle
 
 
 def face_coords(a, nx, ny, nz, dx):
@@ -20,7 +20,7 @@ class OracleStep:
     """Fields of run_edge.py:174-254 as numpy arrays; `substep()` = one loop iteration."""
 
     def __init__(self, u, smoke, dx, reinit_every, CFL, eps, levels, BFECC_clamp, dtype=np.float32,
-                 stop_mode=1, rel_tol=1e-6, max_iters=400):
+                 stop_mode=1, rel_tol=1e-6, max_iters=400, scheme="edge"):
         self.o = Oracle(dtype)
         self.dt_ = np.dtype(dtype)
         self.u = [np.ascontiguousarray(c, dtype) for c in u]
@@ -35,6 +35,7 @@ class OracleStep:
         self.i = -1
         self.iters = []
         self.lp = None
+        self.scheme = scheme   # "edge" (run_edge.py), "ed8" (run_ed8.py), "ge" (run_ge.py)
 
     def set_particles(self, pos, vel, ref_radius, num, params):
         """Laden particles (run_edge.py:235-254): pos/vel (3, n); params as the driver's lp dict."""
@@ -44,6 +45,27 @@ class OracleStep:
         self.rr = np.ascontiguousarray(ref_radius, self.dt_)
         self.num = int(num)
         self.domain = (self.nx * self.dx, self.ny * self.dx, self.nz * self.dx)
+
+    def march_full_map(self, dt):
+        """ED8 (sim/hermite.py:1290-1338) or GE (1398-1424), unfused, then
+        split_psi_hermite (run_ed8.py:190-223)."""
+        o, dx, n = self.o, self.dx, (self.nx, self.ny, self.nz)
+        X = self.X.reshape(3, -1)
+        if self.scheme == "ed8":
+            P8 = []
+            for e in range(8):   # bit 0: x sign, bit 1: y, bit 2: z (sim/hermite.py:704-725)
+                sg = np.array([1 if e & 1 else -1, 1 if e & 2 else -1, 1 if e & 4 else -1], float)
+                p = o.advance_map_0((X + self.eps * sg[:, None]).astype(self.dt_), self.u, dx, -dt)
+                p, _ = o.compose(HK_FULL, self.psi, self.T, self.dT, self.d2T, p, None, dx)
+                P8.append(p.reshape((3,) + n))
+            self.psi, self.T, self.dT, self.d2T = o.calc_ed8(np.stack(P8), self.eps)
+        else:
+            Tm = np.broadcast_to(np.eye(3, dtype=self.dt_)[:, :, None], (3, 3, X.shape[1])).copy()
+            p, Tm = o.advance_map_1_transpose(X.copy(), Tm, self.u, dx, -dt)
+            p, Tm = o.compose(HK_FULL, self.psi, self.T, self.dT, self.d2T, p, Tm, dx)
+            self.psi, self.T = p.reshape((3,) + n), Tm.reshape((3, 3) + n)
+            self.dT, self.d2T = o.fd3(self.T, None, dx, want_dT=True)
+        self.psif, self.Tf = o.split_psi_hermite(HK_FULL, self.psi, self.T, self.dT, self.d2T, dx)
 
     def particle_forces(self, dt):
         """run_edge.py:627-641 with the oracle's sim/lp.py restatement."""
@@ -82,30 +104,46 @@ class OracleStep:
             for c in range(3):
                 self.T[c, c] = 1
             self.dT = np.zeros((3, 3) + n, self.dt_)
+            self.d2T = np.zeros((3,) + n, self.dt_)
             self.init_u = [c.copy() for c in self.u]
             self.init_smoke = self.smoke.copy()
         if self.lp is not None:
             self.particle_forces(dt)
         self.u = o.advect_u_sl(self.u, dx, 0.5 * dt)
         self.poisson()
-        self.psi, self.T, self.dT = o.march_edge(self.psi, self.T, self.dT, self.u, self.X, dx, dt, self.eps)
+        if self.scheme == "edge":
+            self.psi, self.T, self.dT = o.march_edge(self.psi, self.T, self.dT, self.u, self.X, dx, dt, self.eps)
+        else:
+            self.march_full_map(dt)
         for a in range(3):
             self.phi[a], self.F[a] = o.advance_map_col(self.phi[a], self.F[a], self.u, dx, dt)
         if self.lp is not None:
             # run_edge.py:661-667
             tmp = o.map_u(self.vis, self.F, self.phi, dx)
             self.init_u = [(i0 + dt * t).astype(self.dt_) for i0, t in zip(self.init_u, tmp)]
-        self.u = o.map_u_psi(self.init_u, self.psi, self.T, self.dT, dx)
-        self.smoke = o.map_smoke_psi(0, self.init_smoke, None, self.psi, self.T, self.dT, dx)
+        if self.scheme == "edge":
+            def pull_u(u0):
+                return o.map_u_psi(u0, self.psi, self.T, self.dT, dx)
+
+            def pull_s(mode, s0, s1):
+                return o.map_smoke_psi(mode, s0, s1, self.psi, self.T, self.dT, dx)
+        else:
+            def pull_u(u0):   # run_ed8.py:409-427: through the face values of the full map
+                return o.map_u(u0, self.Tf, self.psif, dx)
+
+            def pull_s(mode, s0, s1):
+                return o.map_smoke(mode, s0, s1, self.psif, dx)
+        self.u = pull_u(self.init_u)
+        self.smoke = pull_s(0, self.init_smoke, None)
         err_u = o.map_u(self.u, self.F, self.phi, dx)
         err_s = o.map_smoke(0, self.smoke, None, self.phi, dx)
         err_u = [((e + -1. * i0) * 0.5).astype(self.dt_) for e, i0 in zip(err_u, self.init_u)]
         err_s = ((err_s + -1. * self.init_smoke) * 0.5).astype(self.dt_)
-        tmp_u = o.map_u_psi(err_u, self.psi, self.T, self.dT, dx)
-        tmp_s = o.map_smoke_psi(0, err_s, None, self.psi, self.T, self.dT, dx)
+        tmp_u = pull_u(err_u)
+        tmp_s = pull_s(0, err_s, None)
         err_u = [(u + -1. * t).astype(self.dt_) for u, t in zip(self.u, tmp_u)]
         self.smoke = (self.smoke + -1. * tmp_s).astype(self.dt_)
-        self.smoke = o.map_smoke_psi(1, self.init_smoke, self.smoke, self.psi, self.T, self.dT, dx)
+        self.smoke = pull_s(1, self.init_smoke, self.smoke)
         if self.BFECC_clamp:
             self.u = [o.clamp_u(e, u) for e, u in zip(err_u, self.u)]
         else:

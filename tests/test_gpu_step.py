@@ -20,10 +20,10 @@ def scale_err(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-def make(res, fixed_iters=None, case="cfg1", clamp=None):
+def make(res, fixed_iters=None, case="cfg1", clamp=None, scheme="edge", dtype=np.float32):
     from run_edge import Simulation
     from oracle.step import OracleStep
-    sim = Simulation(case, solver_mode="plain", res=res, num_samples=400)
+    sim = Simulation(case, solver_mode="plain", res=res, num_samples=400, scheme=scheme)
     if clamp is not None:
         sim.BFECC_clamp = clamp
     sim.initialize()
@@ -35,7 +35,7 @@ def make(res, fixed_iters=None, case="cfg1", clamp=None):
         sim.solver.rel_tol = 0.0
         kw = dict(stop_mode=1, rel_tol=0.0, max_iters=fixed_iters)
     ref = OracleStep(u0, s0, sim.dx, sim.reinit_every, sim.CFL, sim.ed_eps, levels=sim.solver.n_mg_levels,
-                     BFECC_clamp=sim.BFECC_clamp, **kw)
+                     BFECC_clamp=sim.BFECC_clamp, scheme=scheme, dtype=dtype, **kw)
     return sim, ref
 
 
@@ -50,6 +50,32 @@ def test_one_step_fixed_iterations(dev, clamp):
         assert scale_err(g, r) < 1e-4
     assert scale_err(np.moveaxis(sim.smoke.to_numpy(), -1, 0), ref.smoke) < 1e-4
     assert scale_err(np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref.psi) < 1e-5
+
+
+@pytest.mark.parametrize("scheme", ["ed8", "ge"])
+def test_ed8_ge_steps_fixed_iterations(dev, scheme):
+    """Two steps of the run_ed8.py / run_ge.py loops.  Their maps carry
+    eps-differences (ED8: up to 1/(8 eps^3)) or finite differences whose FP32
+    rounding differs between any two implementations, so the GPU is judged
+    against the FP64 oracle with the FP32 oracle's own error as the yardstick."""
+    from oracle.step import OracleStep
+    sim, ref32 = make((32, 16, 16), fixed_iters=6, scheme=scheme)
+    u0 = [np.asarray(c, np.float64) for c in ref32.u]
+    ref64 = OracleStep(u0, ref32.smoke.astype(np.float64), sim.dx, sim.reinit_every, sim.CFL, sim.ed_eps,
+                       levels=sim.solver.n_mg_levels, BFECC_clamp=sim.BFECC_clamp, scheme=scheme, dtype=np.float64,
+                       stop_mode=1, rel_tol=0.0, max_iters=6)
+    for _ in range(2):
+        dt = sim.substep()
+        ref32.substep(dt=dt)
+        ref64.substep(dt=dt)
+    torch.cuda.synchronize()
+    for g, r32, r64 in zip(sim.velocity_numpy(), ref32.u, ref64.u):
+        e_gpu, e_32 = scale_err(g, r64), scale_err(r32, r64)
+        assert e_gpu <= max(3.0 * e_32, 1e-4), (e_gpu, e_32)
+    # the map position: ED8's is the mean of 8 marched eps-points (FP32 position noise)
+    e_gpu = scale_err(np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref64.psi)
+    e_32 = scale_err(ref32.psi, ref64.psi)
+    assert e_gpu <= max(3.0 * e_32, 1e-5), (e_gpu, e_32)
 
 
 def test_particle_step_fixed_iterations(dev):
