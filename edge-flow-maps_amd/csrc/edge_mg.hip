@@ -92,6 +92,7 @@ struct Lv {
   int n0, n1, n2;
   const int8_t* bm;   // 1 = solid (null when no solids)
   FDiv f1, f2, fh;    // divisions by n1, n2 and n2/2
+  FDiv f8;            // division by n2/8 (row chunks; n2 a multiple of 8)
 };
 
 inline Lv make_lv(int n0, int n1, int n2) {
@@ -103,8 +104,11 @@ inline Lv make_lv(int n0, int n1, int n2) {
   L.f1 = make_fdiv((uint32_t)n1);
   L.f2 = make_fdiv((uint32_t)n2);
   L.fh = make_fdiv((uint32_t)(n2 > 1 ? n2 / 2 : 1));
+  L.f8 = make_fdiv((uint32_t)(n2 >= 8 ? n2 / 8 : 1));
   return L;
 }
+// row-chunk kernels apply to a level when its rows split into 8-cell chunks
+inline bool chunked(const Lv& L) { return L.n2 % 8 == 0; }
 
 __device__ __forceinline__ int64_t lid(const Lv& L, int i, int j, int k) {
   return ((int64_t)i * L.n1 + j) * L.n2 + k;
@@ -658,6 +662,413 @@ __global__ void __launch_bounds__(BS) k_prolong_black(Lv F, Lv C, int bt, const 
   st(zf, q, (ld(rf, q) + ns) / num_fluid(F, bt, i, j, k));
 }
 
+// ---------------------------------------------------------------------------
+// Row-chunk kernels: one thread owns 8 consecutive cells along k of one row
+// (n2 a multiple of 8): one index decode and 16/32-byte vector loads per 8
+// cells instead of per cell.  Each cell's arithmetic is that of the per-cell
+// kernels above (same neighbour order i+1, i-1, j+1, j-1, k+1, k-1, same
+// boundary terms); with solids every cell takes the per-cell path.
+// ---------------------------------------------------------------------------
+template <class S>
+__device__ __forceinline__ void ld8(const S* p, float v[8]);
+template <>
+__device__ __forceinline__ void ld8<float>(const float* p, float v[8]) {
+  const float4 a = ((const float4*)p)[0], b = ((const float4*)p)[1];
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+template <>
+__device__ __forceinline__ void ld8<__hip_bfloat16>(const __hip_bfloat16* p, float v[8]) {
+  const uint4 u = *(const uint4*)p;
+  const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    v[2 * q] = __uint_as_float(w[q] << 16);
+    v[2 * q + 1] = __uint_as_float(w[q] & 0xffff0000u);
+  }
+}
+template <class S>
+__device__ __forceinline__ void st8(S* p, const float v[8]);
+template <>
+__device__ __forceinline__ void st8<float>(float* p, const float v[8]) {
+  ((float4*)p)[0] = make_float4(v[0], v[1], v[2], v[3]);
+  ((float4*)p)[1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+template <>
+__device__ __forceinline__ void st8<__hip_bfloat16>(__hip_bfloat16* p, const float v[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    w[q] = (uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[2 * q])) |
+           ((uint32_t)__bfloat16_as_ushort(__float2bfloat16(v[2 * q + 1])) << 16);
+  *(uint4*)p = make_uint4(w[0], w[1], w[2], w[3]);
+}
+// the value a store of v to S reads back as
+template <class S>
+__device__ __forceinline__ float through(float v);
+template <>
+__device__ __forceinline__ float through<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float through<__hip_bfloat16>(float v) { return __bfloat162float(__float2bfloat16(v)); }
+
+__device__ __forceinline__ void chunk_ijk(const Lv& L, uint32_t t, int& i, int& j, int& k0) {
+  const uint32_t row = fdiv(t, L.f8);
+  k0 = 8 * (int)(t - row * (uint32_t)(L.n2 >> 3));
+  const uint32_t ii = fdiv(row, L.f1);
+  j = (int)(row - ii * (uint32_t)L.n1);
+  i = (int)ii;
+}
+
+// z at a chunk's 8 cells, its 4 neighbouring rows and the two k-neighbours
+struct Nb8 {
+  float c[8], ip[8], im[8], jp[8], jm[8];
+  float kp, km;
+};
+template <class S>
+__device__ __forceinline__ void load_nb(const Lv& L, const S* z, int i, int j, int k0, int64_t q0, Nb8& N) {
+  const int64_t si = (int64_t)L.n1 * L.n2, sj = L.n2;
+  ld8(z + q0, N.c);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) N.ip[c] = N.im[c] = N.jp[c] = N.jm[c] = 0.f;
+  if (i < L.n0 - 1) ld8(z + q0 + si, N.ip);
+  if (i > 0) ld8(z + q0 - si, N.im);
+  if (j < L.n1 - 1) ld8(z + q0 + sj, N.jp);
+  if (j > 0) ld8(z + q0 - sj, N.jm);
+  N.kp = k0 + 8 < L.n2 ? ld(z, q0 + 8) : 0.f;
+  N.km = k0 > 0 ? ld(z, q0 - 1) : 0.f;
+}
+// neighbor_sum without solids for chunk cell c
+__device__ __forceinline__ float nsum8(const Lv& L, const Nb8& N, int i, int j, int k0, int c) {
+  float ret = 0.f;
+  if (i < L.n0 - 1) ret += N.ip[c];
+  if (i > 0) ret += N.im[c];
+  if (j < L.n1 - 1) ret += N.jp[c];
+  if (j > 0) ret += N.jm[c];
+  const int k = k0 + c;
+  if (k < L.n2 - 1) ret += (c < 7 ? N.c[c + 1] : N.kp);
+  if (k > 0) ret += (c > 0 ? N.c[c - 1] : N.km);
+  return ret;
+}
+// num_fluid without solids
+__device__ __forceinline__ float diag_free(const Lv& L, int bt, int i, int j, int k) {
+  float num = 6.0f;
+  if (i <= 0 && (bt & 1)) num -= 1.0f;
+  if (i >= L.n0 - 1 && ((bt >> 1) & 1)) num -= 1.0f;
+  if (j <= 0 && ((bt >> 2) & 1)) num -= 1.0f;
+  if (j >= L.n1 - 1 && ((bt >> 3) & 1)) num -= 1.0f;
+  if (k <= 0 && ((bt >> 4) & 1)) num -= 1.0f;
+  if (k >= L.n2 - 1 && ((bt >> 5) & 1)) num -= 1.0f;
+  return num;
+}
+
+// One red-black half-sweep (as k_smooth).
+template <class S>
+__global__ void __launch_bounds__(BS) k_smooth8(Lv L, int bt, const S* __restrict__ r, S* __restrict__ z, int phase,
+                                               const double* __restrict__ sc, int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t_ >= (uint32_t)((int64_t)L.n0 * L.n1 * L.n2 >> 3)) return;
+  int i, j, k0;
+  chunk_ijk(L, t_, i, j, k0);
+  const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
+  if (L.bm) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int k = k0 + c;
+      if (((i + j + k) & 1) != phase || L.bm[q0 + c] > 0) continue;
+      float ns = neighbor_sum(L, i, j, k, [&](int a, int b, int cc) { return ld(z, lid(L, a, b, cc)); });
+      st(z, q0 + c, (ld(r, q0 + c) + ns) / num_fluid(L, bt, i, j, k));
+    }
+    return;
+  }
+  Nb8 N;
+  load_nb(L, z, i, j, k0, q0, N);
+  float rv[8], out[8];
+  ld8(r + q0, rv);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int k = k0 + c;
+    out[c] = N.c[c];
+    if (((i + j + k) & 1) == phase) out[c] = (rv[c] + nsum8(L, N, i, j, k0, c)) / diag_free(L, bt, i, j, k);
+  }
+  st8(z + q0, out);
+}
+
+// Level-0 entry (as k_first0).
+template <class S>
+__global__ void __launch_bounds__(BS) k_first08(Lv L, int bt, float* __restrict__ r, S* __restrict__ rl,
+                                               S* __restrict__ z, const double* __restrict__ sc, int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t_ >= (uint32_t)((int64_t)L.n0 * L.n1 * L.n2 >> 3)) return;
+  int i, j, k0;
+  chunk_ijk(L, t_, i, j, k0);
+  const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
+  const float mean = (float)sc[S_MEAN];
+  float v[8], zv[8];
+  ld8(r + q0, v);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int k = k0 + c;
+    v[c] = v[c] - mean;
+    zv[c] = 0.f;
+    if (((i + j + k) & 1) == 0 && !(L.bm && L.bm[q0 + c] > 0))
+      zv[c] = v[c] / (L.bm ? num_fluid(L, bt, i, j, k) : diag_free(L, bt, i, j, k));
+  }
+  st8(r + q0, v);
+  if ((void*)rl != (void*)r) st8(rl + q0, v);
+  st8(z + q0, zv);
+}
+
+// Prolongation + first black half-sweep (as k_prolong_black).
+template <class S>
+__global__ void __launch_bounds__(BS) k_prolong8(Lv F, Lv C, int bt, const S* __restrict__ rf, S* __restrict__ zf,
+                                                const S* __restrict__ zc, const double* __restrict__ sc,
+                                                int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t_ >= (uint32_t)((int64_t)F.n0 * F.n1 * F.n2 >> 3)) return;
+  int i, j, k0;
+  chunk_ijk(F, t_, i, j, k0);
+  const int64_t q0 = ((int64_t)i * F.n1 + j) * F.n2 + k0;
+  auto get = [&](int a, int b, int c) { return ld(zf, lid(F, a, b, c)) + ld(zc, lid(C, a >> 1, b >> 1, c >> 1)); };
+  if (F.bm) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int k = k0 + c;
+      if (((i + j + k) & 1) != 1 || F.bm[q0 + c] > 0) continue;
+      st(zf, q0 + c, (ld(rf, q0 + c) + neighbor_sum(F, i, j, k, get)) / num_fluid(F, bt, i, j, k));
+    }
+    return;
+  }
+  Nb8 N;
+  load_nb(F, zf, i, j, k0, q0, N);
+  // the coarse values under each neighbour: rows (i+-1)>>1, (j+-1)>>1, own row at k0/2-1 .. k0/2+4
+  const int kc0 = k0 >> 1;
+  auto crow = [&](int a, int b, float out[4]) {
+    const int64_t qc = lid(C, a >> 1, b >> 1, kc0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) out[m] = ld(zc, qc + m);
+  };
+  float cip[4] = {0, 0, 0, 0}, cim[4] = {0, 0, 0, 0}, cjp[4] = {0, 0, 0, 0}, cjm[4] = {0, 0, 0, 0}, cc[4];
+  if (i < F.n0 - 1) crow(i + 1, j, cip);
+  if (i > 0) crow(i - 1, j, cim);
+  if (j < F.n1 - 1) crow(i, j + 1, cjp);
+  if (j > 0) crow(i, j - 1, cjm);
+  crow(i, j, cc);
+  const int64_t qc = lid(C, i >> 1, j >> 1, kc0);
+  const float ckp = k0 + 8 < F.n2 ? ld(zc, qc + 4) : 0.f;
+  const float ckm = k0 > 0 ? ld(zc, qc - 1) : 0.f;
+  float rv[8], out[8];
+  ld8(rf + q0, rv);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int k = k0 + c;
+    out[c] = N.c[c];
+    if (((i + j + k) & 1) != 1) continue;
+    float ret = 0.f;
+    if (i < F.n0 - 1) ret += N.ip[c] + cip[c >> 1];
+    if (i > 0) ret += N.im[c] + cim[c >> 1];
+    if (j < F.n1 - 1) ret += N.jp[c] + cjp[c >> 1];
+    if (j > 0) ret += N.jm[c] + cjm[c >> 1];
+    if (k < F.n2 - 1) ret += (c < 7 ? N.c[c + 1] + cc[(c + 1) >> 1] : N.kp + ckp);
+    if (k > 0) ret += (c > 0 ? N.c[c - 1] + cc[(c - 1) >> 1] : N.km + ckm);
+    out[c] = (rv[c] + ret) / diag_free(F, bt, i, j, k);
+  }
+  st8(zf + q0, out);
+}
+
+// Restriction + fill(0) + first red half-sweep of the coarse level (as
+// k_restrict): one thread per 4 coarse cells along k (8 fine cells of 4 fine rows).
+template <class S>
+__global__ void __launch_bounds__(BS) k_restrict8(Lv F, Lv C, int bt, const S* __restrict__ rf, const S* __restrict__ zf,
+                                                 S* __restrict__ rc, S* __restrict__ zc, const double* __restrict__ sc,
+                                                 int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  // chunks of the fine level's rows, but only rows with even i and j
+  const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t_ >= (uint32_t)((int64_t)C.n0 * C.n1 * (C.n2 >> 2))) return;
+  const uint32_t per_row = (uint32_t)(C.n2 >> 2);
+  const uint32_t row = t_ / per_row;
+  const int K0 = 4 * (int)(t_ - row * per_row);
+  const int J = (int)(row % (uint32_t)C.n1), I = (int)(row / (uint32_t)C.n1);
+  const int k0 = 2 * K0;
+  float res[2][2][8];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int i = 2 * I + a, j = 2 * J + b;
+      const int64_t q0 = ((int64_t)i * F.n1 + j) * F.n2 + k0;
+      if (F.bm) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int k = k0 + c;
+          res[a][b][c] = 0.f;
+          if (F.bm[q0 + c] > 0) continue;
+          float ns = neighbor_sum(F, i, j, k, [&](int x, int y, int w) { return ld(zf, lid(F, x, y, w)); });
+          res[a][b][c] = ld(rf, q0 + c) - (num_fluid(F, bt, i, j, k) * ld(zf, q0 + c) - ns);
+        }
+      } else {
+        Nb8 N;
+        load_nb(F, zf, i, j, k0, q0, N);
+        float rv[8];
+        ld8(rf + q0, rv);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          res[a][b][c] = rv[c] - (diag_free(F, bt, i, j, k0 + c) * N.c[c] - nsum8(F, N, i, j, k0, c));
+      }
+    }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int K = K0 + m;
+    const int64_t q = ((int64_t)I * C.n1 + J) * C.n2 + K;
+    float acc = 0.f;
+#pragma unroll
+    for (int c8 = 0; c8 < 8; ++c8) {
+      const int a = c8 >> 2, b = (c8 >> 1) & 1, c = c8 & 1;
+      if (F.bm && F.bm[((int64_t)(2 * I + a) * F.n1 + (2 * J + b)) * F.n2 + 2 * K + c] > 0) continue;
+      acc += res[a][b][2 * m + c] * 1.0f / 2.0f;
+    }
+    st(rc, q, acc);
+    float zv = 0.f;
+    if (((I + J + K) & 1) == 0 && !(C.bm && C.bm[q] > 0)) zv = ld(rc, q) / num_fluid(C, bt, I, J, K);
+    st(zc, q, zv);
+  }
+}
+
+// p = z + beta p on the fly, Ap, p.Ap (as k_p_Ap).
+template <class S>
+__global__ void __launch_bounds__(BS) k_p_Ap8(Lv L, int bt, const S* __restrict__ z, float* __restrict__ p,
+                                             float* __restrict__ Ap, Red R) {
+  EXIT_IF_DONE(R.sc);
+  const float beta = (float)R.sc[S_BETA];
+  double s = 0;
+  const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t_ < (uint32_t)((int64_t)L.n0 * L.n1 * L.n2 >> 3)) {
+    int i, j, k0;
+    chunk_ijk(L, t_, i, j, k0);
+    const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
+    if (L.bm) {
+      auto pn = [&](int a, int b, int c) {
+        int64_t m = lid(L, a, b, c);
+        return ld(z, m) + beta * p[m];
+      };
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (L.bm[q0 + c] > 0) continue;
+        const int k = k0 + c;
+        float pq = pn(i, j, k);
+        float ap = num_fluid(L, bt, i, j, k) * pq - neighbor_sum(L, i, j, k, pn);
+        Ap[q0 + c] = ap;
+        s += (double)pq * (double)ap;
+      }
+    } else {
+      Nb8 Z, P;
+      load_nb(L, z, i, j, k0, q0, Z);
+      load_nb(L, (const float*)p, i, j, k0, q0, P);
+      Nb8 N;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        N.c[c] = Z.c[c] + beta * P.c[c];
+        N.ip[c] = Z.ip[c] + beta * P.ip[c];
+        N.im[c] = Z.im[c] + beta * P.im[c];
+        N.jp[c] = Z.jp[c] + beta * P.jp[c];
+        N.jm[c] = Z.jm[c] + beta * P.jm[c];
+      }
+      N.kp = Z.kp + beta * P.kp;
+      N.km = Z.km + beta * P.km;
+      float out[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float pq = N.c[c];
+        out[c] = diag_free(L, bt, i, j, k0 + c) * pq - nsum8(L, N, i, j, k0, c);
+        s += (double)pq * (double)out[c];
+      }
+      st8(Ap + q0, out);
+    }
+  }
+  block_partial(R, s, 0.0);
+}
+
+// p = z + beta p, x += alpha p, r -= alpha Ap, r.r and sum r (as k_update_xr).
+template <class S>
+__global__ void __launch_bounds__(BS) k_update_xr8(Lv L, const S* __restrict__ z, float* __restrict__ x,
+                                                  float* __restrict__ r, float* __restrict__ p,
+                                                  const float* __restrict__ Ap, Red R) {
+  EXIT_IF_DONE(R.sc);
+  const float alpha = (float)R.sc[S_ALPHA];
+  const float beta = (float)R.sc[S_BETA];
+  double s2 = 0, s1 = 0;
+  const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t_ < (uint32_t)((int64_t)L.n0 * L.n1 * L.n2 >> 3)) {
+    const int64_t q0 = (int64_t)t_ * 8;
+    float zv[8], xv[8], rv[8], pv[8], av[8];
+    ld8(z + q0, zv);
+    ld8(x + q0, xv);
+    ld8(r + q0, rv);
+    ld8(p + q0, pv);
+    ld8(Ap + q0, av);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (!(L.bm && L.bm[q0 + c] > 0)) {
+        const float pn = zv[c] + beta * pv[c];
+        pv[c] = pn;
+        xv[c] += alpha * pn;
+        rv[c] -= alpha * av[c];
+        s2 += (double)rv[c] * (double)rv[c];
+      }
+      s1 += (double)rv[c];
+    }
+    st8(p + q0, pv);
+    st8(x + q0, xv);
+    st8(r + q0, rv);
+  }
+  block_partial(R, s2, s1);
+}
+
+// Last red half-sweep of level 0 + reduce(z, r) over fluid cells (as k_smooth_last_dot).
+template <class S>
+__global__ void __launch_bounds__(BS) k_last_dot8(Lv L, int bt, const S* __restrict__ rl, const float* __restrict__ r,
+                                                 S* __restrict__ z, Red R, int ignore_done) {
+  if (!ignore_done && R.sc[S_DONE] != 0.0) return;
+  double s = 0;
+  const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t_ < (uint32_t)((int64_t)L.n0 * L.n1 * L.n2 >> 3)) {
+    int i, j, k0;
+    chunk_ijk(L, t_, i, j, k0);
+    const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
+    float rv[8];
+    ld8(r + q0, rv);
+    if (L.bm) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (L.bm[q0 + c] > 0) continue;
+        const int k = k0 + c;
+        if (((i + j + k) & 1) == 0) {
+          float ns = neighbor_sum(L, i, j, k, [&](int a, int b, int cc) { return ld(z, lid(L, a, b, cc)); });
+          st(z, q0 + c, (ld(rl, q0 + c) + ns) / num_fluid(L, bt, i, j, k));
+        }
+        s += (double)ld(z, q0 + c) * (double)rv[c];
+      }
+    } else {
+      Nb8 N;
+      load_nb(L, z, i, j, k0, q0, N);
+      float rlv[8], out[8];
+      ld8(rl + q0, rlv);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int k = k0 + c;
+        out[c] = N.c[c];
+        if (((i + j + k) & 1) == 0) out[c] = through<S>((rlv[c] + nsum8(L, N, i, j, k0, c)) / diag_free(L, bt, i, j, k));
+        s += (double)out[c] * (double)rv[c];
+      }
+      st8(z + q0, out);
+    }
+  }
+  block_partial(R, s, 0.0);
+}
+
 // Bottom of the plain V-cycle: the 20 red/black pairs of sim/mgpcg.py:187-189
 // (the first red half-sweep was fused into the restriction) in one workgroup,
 // the level held in LDS; each thread decodes its (at most BOT_CPT) cells and
@@ -1013,24 +1424,60 @@ struct Ops {
 
 int bottom_lds_bytes(const Solver& s) { return (int)(s.nl[s.L - 1] * sizeof(float)); }
 
+int chunk_blocks(int64_t ncells) { return blocks_for(ncells / 8, BS); }
+
+// one red-black half-sweep of level l
+template <class S>
+void smooth(Solver& s, hipStream_t st, int l, const S* r, S* z, int phase, int ig) {
+  const double* sc = s.sc;
+  if (chunked(s.lv[l]))
+    hipLaunchKernelGGL(k_smooth8<S>, dim3(chunk_blocks(s.nl[l])), dim3(BS), 0, st, s.lv[l], s.bt, r, z, phase, sc, ig);
+  else
+    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(s.nl[l] / 2, BS)), dim3(BS), 0, st, s.lv[l], s.bt, r, z, phase, sc,
+                       ig);
+}
+
+// level 0's last red half-sweep + z.r, then its finish action
+template <class S>
+void last_dot(Solver& s, hipStream_t st, Fin fin, int ig, int guard) {
+  const Lv& L0 = s.lv[0];
+  if (chunked(L0)) {
+    hipLaunchKernelGGL(k_last_dot8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r,
+                       (S*)s.zl[0], red(s), ig);
+    if (fin != F_NONE) finish(s, st, chunk_blocks(s.N), fin, guard);
+  } else {
+    hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0],
+                       s.r, (S*)s.zl[0], red(s), ig);
+    if (fin != F_NONE) finish(s, st, red_blocks(s.N / 2), fin, guard);
+  }
+}
+
 template <class S>
 void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
   const int L = s.L;
   const double* sc = s.sc;
   S* r0 = (S*)s.rl[0];
   S* z0 = (S*)s.zl[0];
-  hipLaunchKernelGGL(k_first0<S>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig);
+  if (chunked(s.lv[0]))
+    hipLaunchKernelGGL(k_first08<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig);
+  else
+    hipLaunchKernelGGL(k_first0<S>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig);
   const int lt = tail_level(s);
   for (int l = 0; l < (lt ? lt : L - 1); ++l) {
     S* rl = (S*)s.rl[l];
     S* zl = (S*)s.zl[l];
     int64_t half = s.nl[l] / 2;
+    (void)half;
     // remaining pre-smoothing: black, red, black (sim/mgpcg.py:180-182)
-    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 1, sc, ig);
-    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 0, sc, ig);
-    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 1, sc, ig);
-    hipLaunchKernelGGL(k_restrict<S>, dim3(blocks_for(s.nl[l + 1], BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt,
-                       rl, zl, (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
+    smooth<S>(s, st, l, rl, zl, 1, ig);
+    smooth<S>(s, st, l, rl, zl, 0, ig);
+    smooth<S>(s, st, l, rl, zl, 1, ig);
+    if (chunked(s.lv[l]))
+      hipLaunchKernelGGL(k_restrict8<S>, dim3(blocks_for(s.nl[l + 1] / 4, BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1],
+                         s.bt, rl, zl, (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
+    else
+      hipLaunchKernelGGL(k_restrict<S>, dim3(blocks_for(s.nl[l + 1], BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt,
+                         rl, zl, (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
   }
   const bool dense = s.mode == 1 && s.dense_ok;
   if (lt) {
@@ -1054,22 +1501,22 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
       hipLaunchKernelGGL(k_bottom_rbgs<S>, dim3(1), dim3(BOT_THREADS), bottom_lds_bytes(s), st, s.lv[l], s.bt,
                          (const S*)s.rl[l], (S*)s.zl[l], sc, ig, 20);
     } else {
-      int64_t half = s.nl[l] / 2;
-      for (int it = 0; it < 39; ++it)
-        hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt,
-                           (const S*)s.rl[l], (S*)s.zl[l], (it & 1) == 0 ? 1 : 0, sc, ig);
+      for (int it = 0; it < 39; ++it) smooth<S>(s, st, l, (const S*)s.rl[l], (S*)s.zl[l], (it & 1) == 0 ? 1 : 0, ig);
     }
   }
   for (int l = (lt ? lt : L - 1) - 1; l >= 0; --l) {
     S* rl = (S*)s.rl[l];
     S* zl = (S*)s.zl[l];
     int64_t half = s.nl[l] / 2;
-    hipLaunchKernelGGL(k_prolong_black<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt, rl,
-                       zl, (const S*)s.zl[l + 1], sc, ig);
-    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 0, sc, ig);
-    hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 1, sc, ig);
-    if (l > 0)
-      hipLaunchKernelGGL(k_smooth<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.bt, rl, zl, 0, sc, ig);
+    if (chunked(s.lv[l]))
+      hipLaunchKernelGGL(k_prolong8<S>, dim3(chunk_blocks(s.nl[l])), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt, rl,
+                         zl, (const S*)s.zl[l + 1], sc, ig);
+    else
+      hipLaunchKernelGGL(k_prolong_black<S>, dim3(blocks_for(half, BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt,
+                         rl, zl, (const S*)s.zl[l + 1], sc, ig);
+    smooth<S>(s, st, l, rl, zl, 0, ig);
+    smooth<S>(s, st, l, rl, zl, 1, ig);
+    if (l > 0) smooth<S>(s, st, l, rl, zl, 0, ig);
     // level 0's last red half-sweep is fused with z.r by the caller
   }
 }
@@ -1217,14 +1664,21 @@ template <class S>
 void cg_iteration(Solver& s, hipStream_t st) {
   Red R = red(s);
   const Lv& L0 = s.lv[0];
-  hipLaunchKernelGGL(k_p_Ap<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
-  finish(s, st, red_blocks(s.N), F_ALPHA, 1);
-  hipLaunchKernelGGL(k_update_xr<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, (const S*)s.zl[0], s.x, s.r, s.p, s.Ap, R);
-  finish(s, st, red_blocks(s.N), F_XR, 1);
+  if (chunked(L0)) {
+    hipLaunchKernelGGL(k_p_Ap8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
+    finish(s, st, chunk_blocks(s.N), F_ALPHA, 1);
+    hipLaunchKernelGGL(k_update_xr8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, (const S*)s.zl[0], s.x, s.r, s.p,
+                       s.Ap, R);
+    finish(s, st, chunk_blocks(s.N), F_XR, 1);
+  } else {
+    hipLaunchKernelGGL(k_p_Ap<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
+    finish(s, st, red_blocks(s.N), F_ALPHA, 1);
+    hipLaunchKernelGGL(k_update_xr<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, (const S*)s.zl[0], s.x, s.r, s.p,
+                       s.Ap, R);
+    finish(s, st, red_blocks(s.N), F_XR, 1);
+  }
   Ops<S>::vcycle(s, st, 0);
-  hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r, (S*)s.zl[0],
-                     R, 0);
-  finish(s, st, red_blocks(s.N / 2), F_ZR, 1);
+  last_dot<S>(s, st, F_ZR, 0, 1);
 }
 
 template <class S>
@@ -1269,9 +1723,7 @@ int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* st
   const Lv& L0 = s.lv[0];
   // initial preconditioner application and p, z.r (sim/mgpcg.py:216-228)
   Ops<S>::vcycle(s, st, 1);
-  hipLaunchKernelGGL(k_smooth_last_dot<S>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, L0, s.bt, (const S*)s.rl[0], s.r, (S*)s.zl[0], R,
-                     1);
-  finish(s, st, red_blocks(s.N / 2), F_ZR_INIT, 0);
+  last_dot<S>(s, st, F_ZR_INIT, 1, 0);
   EDGE_CHECK_LAUNCH();
   int it = 0;
   int done = 0;
@@ -1582,14 +2034,12 @@ int edge_mg_vcycle(void* handle, const float* r, float* z, void* stream) {
   EDGE_CHECK_HIP(hipMemcpyAsync(s.r, r, s.N * 4, hipMemcpyDeviceToDevice, st));
   if (s.mode == 1) {
     Ops<__hip_bfloat16>::vcycle(s, st, 1);
-    hipLaunchKernelGGL(k_smooth_last_dot<__hip_bfloat16>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, s.lv[0], s.bt,
-                       (const __hip_bfloat16*)s.rl[0], s.r, (__hip_bfloat16*)s.zl[0], red(s), 1);
+    last_dot<__hip_bfloat16>(s, st, F_NONE, 1, 0);
     hipLaunchKernelGGL(k_to_float<__hip_bfloat16>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N,
                        (const __hip_bfloat16*)s.zl[0], z);
   } else {
     Ops<float>::vcycle(s, st, 1);
-    hipLaunchKernelGGL(k_smooth_last_dot<float>, dim3(red_blocks(s.N / 2)), dim3(BS), 0, st, s.lv[0], s.bt, (const float*)s.rl[0], s.r,
-                       (float*)s.zl[0], red(s), 1);
+    last_dot<float>(s, st, F_NONE, 1, 0);
     EDGE_CHECK_HIP(hipMemcpyAsync(z, s.zl[0], s.N * 4, hipMemcpyDeviceToDevice, st));
   }
   EDGE_CHECK_LAUNCH();
