@@ -1134,7 +1134,7 @@ __global__ void __launch_bounds__(BOT_THREADS) k_bottom_rbgs(Lv L, int bt, const
 // On entry level lt's r and z (first red half-sweep done) are in global
 // memory; on exit level lt's z is.
 constexpr int TAIL_THREADS = 1024;
-constexpr int TAIL_MAX_CELLS = 8192;
+constexpr int TAIL_MAX_CELLS = 1024;   // larger levels are faster as row-chunk launches
 struct Tail {
   int lt, L;
   Lv lv[MAXL];
@@ -1364,7 +1364,7 @@ struct Solver {
   int use_graph = 0;
   hipStream_t cap = nullptr;
   hipGraphExec_t gexec[MAXB + 1] = {};  // one captured batch per size 1..MAXB
-  int hist[2] = {0, 0};                  // iterations of the last two solves (adaptive batches)
+  int hist[4] = {0, 0, 0, 0};            // iterations of the last four solves, newest first (adaptive batches)
   // coarse levels from tail_from on go to one workgroup (0: never)
   int tail_from = 0;
 };
@@ -1706,15 +1706,17 @@ int capture_batch(Solver& s, int batch) {
 }
 
 // CG iterations to run before the next host check.  check_every >= 1: fixed
-// batches (at most MAXB).  0, adaptive: the first batch stops one short of the
-// fewest iterations of the last two solves and later batches are single
-// iterations, so few launches are spent after convergence (the kernels of
-// such an iteration return at once, but each launch still costs its slot).
+// batches (at most MAXB).  0, adaptive: a projection step usually alternates
+// two kinds of solve, so the first batch is predicted from the solves two and
+// four back: their count when they agree, else one short of the smaller;
+// later batches are single iterations.  A host check costs ~30 us, an
+// iteration run after convergence ~150 us of launches (its kernels return at
+// once, but each launch still takes its slot).
 int next_batch(const Solver& s, int check_every, int done_iters) {
   if (check_every >= 1) return std::min(check_every, MAXB);
-  if (done_iters > 0 || s.hist[0] <= 0) return 1;
-  int lo = s.hist[1] > 0 ? std::min(s.hist[0], s.hist[1]) : s.hist[0];
-  return std::max(1, std::min(lo - 1, MAXB));
+  if (done_iters > 0 || s.hist[1] <= 0) return 1;
+  int b = (s.hist[3] > 0 && s.hist[3] == s.hist[1]) ? s.hist[1] : std::min(s.hist[1], s.hist[0] > 0 ? s.hist[0] : s.hist[1]) - 1;
+  return std::max(1, std::min(b, MAXB));
 }
 
 template <class S>
@@ -1749,7 +1751,7 @@ int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* st
     EDGE_CHECK_HIP(hipStreamSynchronize(st));
     done = s.sc_host[S_DONE] != 0.0 || it >= maxit;
   }
-  s.hist[1] = s.hist[0];
+  for (int q = 3; q > 0; --q) s.hist[q] = s.hist[q - 1];
   s.hist[0] = (int)s.sc_host[S_ITERS];
   if (stats) {
     stats[0] = s.sc_host[S_ITERS];
