@@ -5,6 +5,9 @@
 // particles q < num are live, as in the reference.
 #include <algorithm>
 #include <cstring>
+#include <map>
+
+#include <hipcub/hipcub.hpp>
 
 #include "edge_common.h"
 
@@ -82,45 +85,207 @@ __global__ void __launch_bounds__(256) k_lp_vel_diff(Grid g, Mac U, CP3 vel, P3 
   for (int c = 0; c < 3; ++c) diff.v[c][q] = k * (vel.v[c][q] - u[c]);
 }
 
+// ---------------------------------------------------------------------------
 // sim/lp.py:100-156 laden_particles_back_with_computing_particle: the drag
 // diff (r_p / r)^3 spread onto face grid a with the quadratic B-spline
-// weights; particles within 3 dx of a wall are skipped.  Such a particle's
-// three non-zero taps per axis are inside the grid, so no bounds test binds.
-// Accumulated with FP32 atomics (summation order differs from a sequential
-// loop by rounding only).  The faces must be zeroed first.
-__global__ void __launch_bounds__(256) k_lp_p2g(Grid g, P3 drag, CP3 diff, CP3 pos, const float* __restrict__ rr,
-                                               int64_t num, float radius, float Lx, float Ly, float Lz) {
+// weights; particles within 3 dx of a wall are skipped, so a particle's three
+// non-zero taps per axis are inside the grid.
+// Particles are grouped by 8x8x8-cell tile (a radix sort of tile keys); one
+// workgroup per tile accumulates its particles' taps in LDS with LDS atomics,
+// then adds each touched face of the tile's box to the grid with one global
+// atomic.  Summation order differs from a sequential loop by rounding only.
+// ---------------------------------------------------------------------------
+constexpr int TILE = 8;
+constexpr int BOX = TILE + 4;   // face indices c0-2 .. c0+9 of a tile starting at cell c0
+
+// the reference's skip test (NaN positions skipped)
+__device__ __forceinline__ bool p2g_live(const float P[3], float dx, float Lx, float Ly, float Lz) {
+  return P[0] >= 3.f * dx && P[1] >= 3.f * dx && P[2] >= 3.f * dx && P[0] <= Lx - 3.f * dx &&
+         P[1] <= Ly - 3.f * dx && P[2] <= Lz - 3.f * dx;
+}
+
+struct Tiles {
+  int tx, ty, tz;   // tiles per axis
+};
+
+__global__ void __launch_bounds__(256) k_p2g_keys(Grid g, CP3 pos, int64_t num, float Lx, float Ly, float Lz, Tiles T,
+                                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= num) return;
-  const float dx = g.dx;
-  float P[3] = {pos.v[0][q], pos.v[1][q], pos.v[2][q]};
-  // the reference's skip test, written so that a NaN position is skipped too
-  if (!(P[0] >= 3.f * dx && P[1] >= 3.f * dx && P[2] >= 3.f * dx && P[0] <= Lx - 3.f * dx &&
-        P[1] <= Ly - 3.f * dx && P[2] <= Lz - 3.f * dx))
-    return;
-  const float r = rr[q];
-  const float scale = r * r * r / (radius * radius * radius);
-  float s[3] = {P[0] / dx, P[1] / dx, P[2] / dx};
-  const uint32_t ny = g.ny, nz = g.nz;
+  const float P[3] = {pos.v[0][q], pos.v[1][q], pos.v[2][q]};
+  uint32_t key = (uint32_t)T.tx * T.ty * T.tz * (TILE * TILE * TILE);   // sentinel: not scattered
+  if (p2g_live(P, g.dx, Lx, Ly, Lz)) {
+    const int n[3] = {g.nx, g.ny, g.nz};
+    int c[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    // face grid a: offset 0 on axis a, 0.5 on the others
-    Axis3 A = bspline_axis(s[0] - (a == 0 ? 0.f : 0.5f), 1e30f);
-    Axis3 B = bspline_axis(s[1] - (a == 1 ? 0.f : 0.5f), 1e30f);
-    Axis3 C = bspline_axis(s[2] - (a == 2 ? 0.f : 0.5f), 1e30f);
-    const uint32_t d1 = ny + (a == 1), d2 = nz + (a == 2);
-    const float d = diff.v[a][q] * scale;
-    float* __restrict__ f = drag.v[a];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const float wij = A.w[i] * B.w[j];
-        float* row = f + ((uint32_t)(A.i0 + i) * d1 + (uint32_t)(B.i0 + j)) * d2 + (uint32_t)C.i0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) atomicAdd(row + k, d * (wij * C.w[k]));
-      }
+    for (int d = 0; d < 3; ++d) c[d] = min(max((int)floorf(P[d] / g.dx), 0), n[d] - 1);
+    const uint32_t tile = ((uint32_t)(c[0] / TILE) * T.ty + (uint32_t)(c[1] / TILE)) * T.tz + (uint32_t)(c[2] / TILE);
+    key = tile * (TILE * TILE * TILE) + (uint32_t)(((c[0] % TILE) * TILE + c[1] % TILE) * TILE + c[2] % TILE);
   }
+  keys[q] = key;
+  vals[q] = (uint32_t)q;
+}
+
+// start/end of each cell's run in the sorted keys (both zero for empty cells)
+__global__ void __launch_bounds__(256) k_p2g_ranges(const uint32_t* __restrict__ keys, int64_t num, uint32_t ntiles,
+                                                   uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= num) return;
+  const uint32_t k = keys[q];
+  if (k >= ntiles) return;
+  if (q == 0 || keys[q - 1] != k) start[k] = (uint32_t)q;
+  if (q == num - 1 || keys[q + 1] != k) end[k] = (uint32_t)(q + 1);
+}
+
+// One workgroup per tile, one thread per cell: the thread sums its cell's
+// particles into a register window of the faces they reach (per face grid a:
+// 4 faces along a, starting at the cell's index - 1, and 3 along the other
+// axes), adds the window to the tile's LDS box, and the box goes to the grid.
+constexpr int TCELLS = TILE * TILE * TILE;
+__global__ void __launch_bounds__(TCELLS) k_p2g_cells(Grid g, P3 drag, CP3 diff, CP3 pos, const float* __restrict__ rr,
+                                                     float radius, Tiles T, const uint32_t* __restrict__ order,
+                                                     const uint32_t* __restrict__ start,
+                                                     const uint32_t* __restrict__ end) {
+  __shared__ float box[3][BOX][BOX][BOX];
+  const uint32_t tile = blockIdx.x;
+  const int lc = threadIdx.x;
+  const uint32_t key = tile * TCELLS + (uint32_t)lc;
+  const uint32_t b = start[key], e = end[key];
+  if (__syncthreads_count(b < e) == 0) return;
+  const int tz = (int)(tile % (uint32_t)T.tz), ty = (int)((tile / (uint32_t)T.tz) % (uint32_t)T.ty),
+            tx = (int)(tile / ((uint32_t)T.tz * T.ty));
+  const int o[3] = {tx * TILE - 2, ty * TILE - 2, tz * TILE - 2};   // face index of box entry 0
+  const int c[3] = {tx * TILE + (lc >> 6), ty * TILE + ((lc >> 3) & 7), tz * TILE + (lc & 7)};
+  float* flat = &box[0][0][0][0];
+  for (int t = threadIdx.x; t < 3 * BOX * BOX * BOX; t += blockDim.x) flat[t] = 0.f;
+  __syncthreads();
+  float acc[3][36];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int m = 0; m < 36; ++m) acc[a][m] = 0.f;
+  const float dx = g.dx;
+  for (uint32_t idx = b; idx < e; ++idx) {
+    const uint32_t q = order[idx];
+    const float P[3] = {pos.v[0][q], pos.v[1][q], pos.v[2][q]};
+    const float r = rr[q];
+    const float scale = r * r * r / (radius * radius * radius);
+    const float sp[3] = {P[0] / dx, P[1] / dx, P[2] / dx};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const Axis3 X[3] = {bspline_axis(sp[0] - (a == 0 ? 0.f : 0.5f), 1e30f),
+                          bspline_axis(sp[1] - (a == 1 ? 0.f : 0.5f), 1e30f),
+                          bspline_axis(sp[2] - (a == 2 ? 0.f : 0.5f), 1e30f)};
+      const float d = diff.v[a][q] * scale;
+      // window offsets: 0 or 1 along a, 0 along the others
+      int off[3];
+      bool in_window = true;
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax) {
+        off[ax] = X[ax].i0 - (c[ax] - 1);
+        in_window = in_window && (ax == a ? (off[ax] == 0 || off[ax] == 1) : off[ax] == 0);
+      }
+      if (in_window) {
+        // weight of window entry m along axis ax
+        auto wt = [&](int ax, int m) {
+          if (ax != a) return X[ax].w[m];
+          return off[ax] == 0 ? (m < 3 ? X[ax].w[m < 3 ? m : 2] : 0.f) : (m > 0 ? X[ax].w[m > 0 ? m - 1 : 0] : 0.f);
+        };
+        constexpr int W = 3;
+#pragma unroll
+        for (int m0 = 0; m0 < (a == 0 ? 4 : 3); ++m0)
+#pragma unroll
+          for (int m1 = 0; m1 < (a == 1 ? 4 : 3); ++m1) {
+            const float w01 = wt(0, m0) * wt(1, m1);
+#pragma unroll
+            for (int m2 = 0; m2 < (a == 2 ? 4 : 3); ++m2) {
+              const int n1 = (a == 1 ? 4 : W), n2 = (a == 2 ? 4 : W);
+              acc[a][(m0 * n1 + m1) * n2 + m2] += d * (w01 * wt(2, m2));
+            }
+          }
+      } else {
+        // a tap start outside the expected window (rounding at a cell edge): straight to LDS
+        const int li = X[0].i0 - o[0], lj = X[1].i0 - o[1], lk = X[2].i0 - o[2];
+        if (li < 0 || lj < 0 || lk < 0 || li + 2 >= BOX || lj + 2 >= BOX || lk + 2 >= BOX) continue;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+              atomicAdd(&box[a][li + i][lj + j][lk + k], d * (X[0].w[i] * X[1].w[j] * X[2].w[k]));
+      }
+    }
+  }
+  if (b < e) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int n0 = (a == 0 ? 4 : 3), n1 = (a == 1 ? 4 : 3), n2 = (a == 2 ? 4 : 3);
+#pragma unroll
+      for (int m0 = 0; m0 < 4; ++m0)
+#pragma unroll
+        for (int m1 = 0; m1 < 4; ++m1)
+#pragma unroll
+          for (int m2 = 0; m2 < 4; ++m2) {
+            if (m0 >= n0 || m1 >= n1 || m2 >= n2) continue;
+            const float v = acc[a][(m0 * n1 + m1) * n2 + m2];
+            if (v != 0.f) atomicAdd(&box[a][c[0] - 1 + m0 - o[0]][c[1] - 1 + m1 - o[1]][c[2] - 1 + m2 - o[2]], v);
+          }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 3 * BOX * BOX * BOX; t += blockDim.x) {
+    const float v = flat[t];
+    if (v == 0.f) continue;
+    const int a = t / (BOX * BOX * BOX);
+    const int rem = t - a * BOX * BOX * BOX;
+    const int fi = o[0] + rem / (BOX * BOX), fj = o[1] + (rem / BOX) % BOX, fk = o[2] + rem % BOX;
+    const int d0 = g.nx + (a == 0), d1 = g.ny + (a == 1), d2 = g.nz + (a == 2);
+    if (fi < 0 || fj < 0 || fk < 0 || fi >= d0 || fj >= d1 || fk >= d2) continue;
+    atomicAdd(drag.v[a] + ((int64_t)fi * d1 + fj) * d2 + fk, v);
+  }
+}
+
+// Sort workspace per device, grown on demand and kept for the process.
+struct P2GWork {
+  size_t cap = 0, tiles_cap = 0, temp_bytes = 0;
+  uint32_t *k0 = nullptr, *k1 = nullptr, *v0 = nullptr, *v1 = nullptr, *start = nullptr, *end = nullptr;
+  void* temp = nullptr;
+};
+std::map<int, P2GWork> g_p2g;
+
+int p2g_workspace(P2GWork& w, int64_t num, uint32_t ntiles, int end_bit, hipStream_t st) {
+  if ((size_t)num > w.cap) {
+    for (uint32_t* p : {w.k0, w.k1, w.v0, w.v1})
+      if (p) (void)hipFree(p);
+    w.k0 = w.k1 = w.v0 = w.v1 = nullptr;
+    w.cap = 0;
+    const size_t bytes = (size_t)num * sizeof(uint32_t);
+    EDGE_CHECK_HIP(hipMalloc(&w.k0, bytes));
+    EDGE_CHECK_HIP(hipMalloc(&w.k1, bytes));
+    EDGE_CHECK_HIP(hipMalloc(&w.v0, bytes));
+    EDGE_CHECK_HIP(hipMalloc(&w.v1, bytes));
+    w.cap = (size_t)num;
+  }
+  if (ntiles > w.tiles_cap) {
+    if (w.start) (void)hipFree(w.start);
+    if (w.end) (void)hipFree(w.end);
+    w.start = w.end = nullptr;
+    w.tiles_cap = 0;
+    EDGE_CHECK_HIP(hipMalloc(&w.start, (size_t)ntiles * sizeof(uint32_t)));
+    EDGE_CHECK_HIP(hipMalloc(&w.end, (size_t)ntiles * sizeof(uint32_t)));
+    w.tiles_cap = ntiles;
+  }
+  size_t need = 0;
+  EDGE_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, w.k0, w.k1, w.v0, w.v1, (int)num, 0, end_bit, st));
+  if (need > w.temp_bytes) {
+    if (w.temp) (void)hipFree(w.temp);
+    w.temp = nullptr;
+    w.temp_bytes = 0;
+    EDGE_CHECK_HIP(hipMalloc(&w.temp, need));
+    w.temp_bytes = need;
+  }
+  return EDGE_OK;
 }
 
 __global__ void __launch_bounds__(256) k_lp_zero(float* __restrict__ f, int64_t n) {
@@ -254,9 +419,27 @@ int edge_lp_p2g(float* dragx, float* dragy, float* dragz, const float* diff, con
   EDGE_CHECK_LAUNCH();
   int64_t m = std::min(num, n);
   if (m == 0) return EDGE_OK;
-  hipLaunchKernelGGL(k_lp_p2g, dim3(blocks64(m)), dim3(256), 0, st, g, P3{{dragx, dragy, dragz}},
-                     CP3{{diff, diff + n, diff + 2 * n}}, CP3{{pos, pos + n, pos + 2 * n}}, ref_radius, m, radius, Lx,
-                     Ly, Lz);
+  const Tiles T{(nx + TILE - 1) / TILE, (ny + TILE - 1) / TILE, (nz + TILE - 1) / TILE};
+  EDGE_REQUIRE((int64_t)T.tx * T.ty * T.tz * TCELLS < (1LL << 31), "edge_lp_p2g: grid too large for 32-bit cell keys");
+  const uint32_t ntiles = (uint32_t)T.tx * T.ty * T.tz;
+  const uint32_t nkeys = ntiles * TCELLS;   // tile-major cell ids; nkeys = not scattered
+  int end_bit = 1;
+  while ((1u << end_bit) <= nkeys) ++end_bit;
+  int dev = 0;
+  EDGE_CHECK_HIP(hipGetDevice(&dev));
+  P2GWork& w = g_p2g[dev];
+  int rc = p2g_workspace(w, m, nkeys, end_bit, st);
+  if (rc) return rc;
+  const CP3 P{{pos, pos + n, pos + 2 * n}}, D{{diff, diff + n, diff + 2 * n}};
+  hipLaunchKernelGGL(k_p2g_keys, dim3(blocks64(m)), dim3(256), 0, st, g, P, m, Lx, Ly, Lz, T, w.k0, w.v0);
+  EDGE_CHECK_LAUNCH();
+  size_t temp = w.temp_bytes;
+  EDGE_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(w.temp, temp, w.k0, w.k1, w.v0, w.v1, (int)m, 0, end_bit, st));
+  hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(nkeys)), dim3(256), 0, st, (float*)w.start, (int64_t)nkeys);
+  hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(nkeys)), dim3(256), 0, st, (float*)w.end, (int64_t)nkeys);
+  hipLaunchKernelGGL(k_p2g_ranges, dim3(blocks64(m)), dim3(256), 0, st, w.k1, m, nkeys, w.start, w.end);
+  hipLaunchKernelGGL(k_p2g_cells, dim3(ntiles), dim3(TCELLS), 0, st, g, P3{{dragx, dragy, dragz}}, D, P, ref_radius,
+                     radius, T, w.v1, w.start, w.end);
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }
