@@ -91,6 +91,7 @@ __device__ __forceinline__ void st<__hip_bfloat16>(__hip_bfloat16* p, int64_t i,
 struct Lv {
   int n0, n1, n2;
   const int8_t* bm;   // 1 = solid (null when no solids)
+  const uint8_t* nb;  // solid bits per cell for the row-chunk kernels (null when no solids)
   FDiv f1, f2, fh;    // divisions by n1, n2 and n2/2
   FDiv f8;            // division by n2/8 (row chunks; n2 a multiple of 8)
 };
@@ -101,6 +102,7 @@ inline Lv make_lv(int n0, int n1, int n2) {
   L.n1 = n1;
   L.n2 = n2;
   L.bm = nullptr;
+  L.nb = nullptr;
   L.f1 = make_fdiv((uint32_t)n1);
   L.f2 = make_fdiv((uint32_t)n2);
   L.fh = make_fdiv((uint32_t)(n2 > 1 ? n2 / 2 : 1));
@@ -667,7 +669,7 @@ __global__ void __launch_bounds__(BS) k_prolong_black(Lv F, Lv C, int bt, const 
 // (n2 a multiple of 8): one index decode and 16/32-byte vector loads per 8
 // cells instead of per cell.  Each cell's arithmetic is that of the per-cell
 // kernels above (same neighbour order i+1, i-1, j+1, j-1, k+1, k-1, same
-// boundary terms); with solids every cell takes the per-cell path.
+// boundary terms); solids enter through the per-cell bits of k_nbmask.
 // ---------------------------------------------------------------------------
 template <class S>
 __device__ __forceinline__ void ld8(const S* p, float v[8]);
@@ -719,6 +721,17 @@ __device__ __forceinline__ void chunk_ijk(const Lv& L, uint32_t t, int& i, int& 
   i = (int)ii;
 }
 
+// Solid bits of a cell (k_nbmask): bit e set when neighbour e (in the order
+// i+1, i-1, j+1, j-1, k+1, k-1) lies inside the level and is solid, bit 6 when
+// the cell itself is solid.  num_fluid = diag_free - popcount(bits 0..5) and
+// neighbor_sum skips the neighbours whose bit is set.
+constexpr uint32_t NB_SELF = 64u;
+// the bits of a chunk's 8 cells, byte c for cell k0 + c (0 without solids)
+__device__ __forceinline__ uint64_t nb8(const Lv& L, int64_t q0) {
+  return L.nb ? *(const uint64_t*)(L.nb + q0) : 0ull;
+}
+__device__ __forceinline__ uint32_t nbc(uint64_t m, int c) { return (uint32_t)(m >> (8 * c)) & 0xffu; }
+
 // z at a chunk's 8 cells, its 4 neighbouring rows and the two k-neighbours
 struct Nb8 {
   float c[8], ip[8], im[8], jp[8], jm[8];
@@ -737,16 +750,16 @@ __device__ __forceinline__ void load_nb(const Lv& L, const S* z, int i, int j, i
   N.kp = k0 + 8 < L.n2 ? ld(z, q0 + 8) : 0.f;
   N.km = k0 > 0 ? ld(z, q0 - 1) : 0.f;
 }
-// neighbor_sum without solids for chunk cell c
-__device__ __forceinline__ float nsum8(const Lv& L, const Nb8& N, int i, int j, int k0, int c) {
+// neighbor_sum for chunk cell c with its solid bits mb
+__device__ __forceinline__ float nsum8(const Lv& L, const Nb8& N, int i, int j, int k0, int c, uint32_t mb) {
   float ret = 0.f;
-  if (i < L.n0 - 1) ret += N.ip[c];
-  if (i > 0) ret += N.im[c];
-  if (j < L.n1 - 1) ret += N.jp[c];
-  if (j > 0) ret += N.jm[c];
+  if (i < L.n0 - 1 && !(mb & 1u)) ret += N.ip[c];
+  if (i > 0 && !(mb & 2u)) ret += N.im[c];
+  if (j < L.n1 - 1 && !(mb & 4u)) ret += N.jp[c];
+  if (j > 0 && !(mb & 8u)) ret += N.jm[c];
   const int k = k0 + c;
-  if (k < L.n2 - 1) ret += (c < 7 ? N.c[c + 1] : N.kp);
-  if (k > 0) ret += (c > 0 ? N.c[c - 1] : N.km);
+  if (k < L.n2 - 1 && !(mb & 16u)) ret += (c < 7 ? N.c[c + 1] : N.kp);
+  if (k > 0 && !(mb & 32u)) ret += (c > 0 ? N.c[c - 1] : N.km);
   return ret;
 }
 // num_fluid without solids
@@ -760,6 +773,10 @@ __device__ __forceinline__ float diag_free(const Lv& L, int bt, int i, int j, in
   if (k >= L.n2 - 1 && ((bt >> 5) & 1)) num -= 1.0f;
   return num;
 }
+// num_fluid from the solid bits
+__device__ __forceinline__ float diag8(const Lv& L, int bt, int i, int j, int k, uint32_t mb) {
+  return diag_free(L, bt, i, j, k) - (float)__popc(mb & 63u);
+}
 
 // One red-black half-sweep (as k_smooth).
 template <class S>
@@ -771,16 +788,7 @@ __global__ void __launch_bounds__(BS) k_smooth8(Lv L, int bt, const S* __restric
   int i, j, k0;
   chunk_ijk(L, t_, i, j, k0);
   const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
-  if (L.bm) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int k = k0 + c;
-      if (((i + j + k) & 1) != phase || L.bm[q0 + c] > 0) continue;
-      float ns = neighbor_sum(L, i, j, k, [&](int a, int b, int cc) { return ld(z, lid(L, a, b, cc)); });
-      st(z, q0 + c, (ld(r, q0 + c) + ns) / num_fluid(L, bt, i, j, k));
-    }
-    return;
-  }
+  const uint64_t M = nb8(L, q0);
   Nb8 N;
   load_nb(L, z, i, j, k0, q0, N);
   float rv[8], out[8];
@@ -788,8 +796,10 @@ __global__ void __launch_bounds__(BS) k_smooth8(Lv L, int bt, const S* __restric
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int k = k0 + c;
+    const uint32_t mb = nbc(M, c);
     out[c] = N.c[c];
-    if (((i + j + k) & 1) == phase) out[c] = (rv[c] + nsum8(L, N, i, j, k0, c)) / diag_free(L, bt, i, j, k);
+    if (((i + j + k) & 1) == phase && !(mb & NB_SELF))
+      out[c] = (rv[c] + nsum8(L, N, i, j, k0, c, mb)) / diag8(L, bt, i, j, k, mb);
   }
   st8(z + q0, out);
 }
@@ -805,15 +815,16 @@ __global__ void __launch_bounds__(BS) k_first08(Lv L, int bt, float* __restrict_
   chunk_ijk(L, t_, i, j, k0);
   const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
   const float mean = (float)sc[S_MEAN];
+  const uint64_t M = nb8(L, q0);
   float v[8], zv[8];
   ld8(r + q0, v);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int k = k0 + c;
+    const uint32_t mb = nbc(M, c);
     v[c] = v[c] - mean;
     zv[c] = 0.f;
-    if (((i + j + k) & 1) == 0 && !(L.bm && L.bm[q0 + c] > 0))
-      zv[c] = v[c] / (L.bm ? num_fluid(L, bt, i, j, k) : diag_free(L, bt, i, j, k));
+    if (((i + j + k) & 1) == 0 && !(mb & NB_SELF)) zv[c] = v[c] / diag8(L, bt, i, j, k, mb);
   }
   st8(r + q0, v);
   if ((void*)rl != (void*)r) st8(rl + q0, v);
@@ -831,16 +842,7 @@ __global__ void __launch_bounds__(BS) k_prolong8(Lv F, Lv C, int bt, const S* __
   int i, j, k0;
   chunk_ijk(F, t_, i, j, k0);
   const int64_t q0 = ((int64_t)i * F.n1 + j) * F.n2 + k0;
-  auto get = [&](int a, int b, int c) { return ld(zf, lid(F, a, b, c)) + ld(zc, lid(C, a >> 1, b >> 1, c >> 1)); };
-  if (F.bm) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int k = k0 + c;
-      if (((i + j + k) & 1) != 1 || F.bm[q0 + c] > 0) continue;
-      st(zf, q0 + c, (ld(rf, q0 + c) + neighbor_sum(F, i, j, k, get)) / num_fluid(F, bt, i, j, k));
-    }
-    return;
-  }
+  const uint64_t M = nb8(F, q0);
   Nb8 N;
   load_nb(F, zf, i, j, k0, q0, N);
   // the coarse values under each neighbour: rows (i+-1)>>1, (j+-1)>>1, own row at k0/2-1 .. k0/2+4
@@ -864,16 +866,17 @@ __global__ void __launch_bounds__(BS) k_prolong8(Lv F, Lv C, int bt, const S* __
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int k = k0 + c;
+    const uint32_t mb = nbc(M, c);
     out[c] = N.c[c];
-    if (((i + j + k) & 1) != 1) continue;
+    if (((i + j + k) & 1) != 1 || (mb & NB_SELF)) continue;
     float ret = 0.f;
-    if (i < F.n0 - 1) ret += N.ip[c] + cip[c >> 1];
-    if (i > 0) ret += N.im[c] + cim[c >> 1];
-    if (j < F.n1 - 1) ret += N.jp[c] + cjp[c >> 1];
-    if (j > 0) ret += N.jm[c] + cjm[c >> 1];
-    if (k < F.n2 - 1) ret += (c < 7 ? N.c[c + 1] + cc[(c + 1) >> 1] : N.kp + ckp);
-    if (k > 0) ret += (c > 0 ? N.c[c - 1] + cc[(c - 1) >> 1] : N.km + ckm);
-    out[c] = (rv[c] + ret) / diag_free(F, bt, i, j, k);
+    if (i < F.n0 - 1 && !(mb & 1u)) ret += N.ip[c] + cip[c >> 1];
+    if (i > 0 && !(mb & 2u)) ret += N.im[c] + cim[c >> 1];
+    if (j < F.n1 - 1 && !(mb & 4u)) ret += N.jp[c] + cjp[c >> 1];
+    if (j > 0 && !(mb & 8u)) ret += N.jm[c] + cjm[c >> 1];
+    if (k < F.n2 - 1 && !(mb & 16u)) ret += (c < 7 ? N.c[c + 1] + cc[(c + 1) >> 1] : N.kp + ckp);
+    if (k > 0 && !(mb & 32u)) ret += (c > 0 ? N.c[c - 1] + cc[(c - 1) >> 1] : N.km + ckm);
+    out[c] = (rv[c] + ret) / diag8(F, bt, i, j, k, mb);
   }
   st8(zf + q0, out);
 }
@@ -894,45 +897,42 @@ __global__ void __launch_bounds__(BS) k_restrict8(Lv F, Lv C, int bt, const S* _
   const int J = (int)(row % (uint32_t)C.n1), I = (int)(row / (uint32_t)C.n1);
   const int k0 = 2 * K0;
   float res[2][2][8];
+  uint64_t M[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int i = 2 * I + a, j = 2 * J + b;
       const int64_t q0 = ((int64_t)i * F.n1 + j) * F.n2 + k0;
-      if (F.bm) {
+      M[a][b] = nb8(F, q0);
+      Nb8 N;
+      load_nb(F, zf, i, j, k0, q0, N);
+      float rv[8];
+      ld8(rf + q0, rv);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const int k = k0 + c;
-          res[a][b][c] = 0.f;
-          if (F.bm[q0 + c] > 0) continue;
-          float ns = neighbor_sum(F, i, j, k, [&](int x, int y, int w) { return ld(zf, lid(F, x, y, w)); });
-          res[a][b][c] = ld(rf, q0 + c) - (num_fluid(F, bt, i, j, k) * ld(zf, q0 + c) - ns);
-        }
-      } else {
-        Nb8 N;
-        load_nb(F, zf, i, j, k0, q0, N);
-        float rv[8];
-        ld8(rf + q0, rv);
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-          res[a][b][c] = rv[c] - (diag_free(F, bt, i, j, k0 + c) * N.c[c] - nsum8(F, N, i, j, k0, c));
+      for (int c = 0; c < 8; ++c) {
+        const uint32_t mb = nbc(M[a][b], c);
+        res[a][b][c] = 0.f;
+        if (!(mb & NB_SELF)) res[a][b][c] = rv[c] - (diag8(F, bt, i, j, k0 + c, mb) * N.c[c] - nsum8(F, N, i, j, k0, c, mb));
       }
     }
+  const int64_t qc0 = ((int64_t)I * C.n1 + J) * C.n2 + K0;
+  const uint32_t MC = C.nb ? *(const uint32_t*)(C.nb + qc0) : 0u;
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const int K = K0 + m;
-    const int64_t q = ((int64_t)I * C.n1 + J) * C.n2 + K;
+    const int64_t q = qc0 + m;
     float acc = 0.f;
 #pragma unroll
     for (int c8 = 0; c8 < 8; ++c8) {
       const int a = c8 >> 2, b = (c8 >> 1) & 1, c = c8 & 1;
-      if (F.bm && F.bm[((int64_t)(2 * I + a) * F.n1 + (2 * J + b)) * F.n2 + 2 * K + c] > 0) continue;
+      if (nbc(M[a][b], 2 * m + c) & NB_SELF) continue;
       acc += res[a][b][2 * m + c] * 1.0f / 2.0f;
     }
     st(rc, q, acc);
+    const uint32_t mb = (MC >> (8 * m)) & 0xffu;
     float zv = 0.f;
-    if (((I + J + K) & 1) == 0 && !(C.bm && C.bm[q] > 0)) zv = ld(rc, q) / num_fluid(C, bt, I, J, K);
+    if (((I + J + K) & 1) == 0 && !(mb & NB_SELF)) zv = through<S>(acc) / diag8(C, bt, I, J, K, mb);
     st(zc, q, zv);
   }
 }
@@ -949,21 +949,8 @@ __global__ void __launch_bounds__(BS) k_p_Ap8(Lv L, int bt, const S* __restrict_
     int i, j, k0;
     chunk_ijk(L, t_, i, j, k0);
     const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
-    if (L.bm) {
-      auto pn = [&](int a, int b, int c) {
-        int64_t m = lid(L, a, b, c);
-        return ld(z, m) + beta * p[m];
-      };
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        if (L.bm[q0 + c] > 0) continue;
-        const int k = k0 + c;
-        float pq = pn(i, j, k);
-        float ap = num_fluid(L, bt, i, j, k) * pq - neighbor_sum(L, i, j, k, pn);
-        Ap[q0 + c] = ap;
-        s += (double)pq * (double)ap;
-      }
-    } else {
+    const uint64_t M = nb8(L, q0);
+    {
       Nb8 Z, P;
       load_nb(L, z, i, j, k0, q0, Z);
       load_nb(L, (const float*)p, i, j, k0, q0, P);
@@ -981,8 +968,11 @@ __global__ void __launch_bounds__(BS) k_p_Ap8(Lv L, int bt, const S* __restrict_
       float out[8];
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
+        const uint32_t mb = nbc(M, c);
         const float pq = N.c[c];
-        out[c] = diag_free(L, bt, i, j, k0 + c) * pq - nsum8(L, N, i, j, k0, c);
+        out[c] = 0.f;   // solid cells: Ap is never read there
+        if (mb & NB_SELF) continue;
+        out[c] = diag8(L, bt, i, j, k0 + c, mb) * pq - nsum8(L, N, i, j, k0, c, mb);
         s += (double)pq * (double)out[c];
       }
       st8(Ap + q0, out);
@@ -1009,9 +999,10 @@ __global__ void __launch_bounds__(BS) k_update_xr8(Lv L, const S* __restrict__ z
     ld8(r + q0, rv);
     ld8(p + q0, pv);
     ld8(Ap + q0, av);
+    const uint64_t M = nb8(L, q0);
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      if (!(L.bm && L.bm[q0 + c] > 0)) {
+      if (!(nbc(M, c) & NB_SELF)) {
         const float pn = zv[c] + beta * pv[c];
         pv[c] = pn;
         xv[c] += alpha * pn;
@@ -1040,18 +1031,8 @@ __global__ void __launch_bounds__(BS) k_last_dot8(Lv L, int bt, const S* __restr
     const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
     float rv[8];
     ld8(r + q0, rv);
-    if (L.bm) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        if (L.bm[q0 + c] > 0) continue;
-        const int k = k0 + c;
-        if (((i + j + k) & 1) == 0) {
-          float ns = neighbor_sum(L, i, j, k, [&](int a, int b, int cc) { return ld(z, lid(L, a, b, cc)); });
-          st(z, q0 + c, (ld(rl, q0 + c) + ns) / num_fluid(L, bt, i, j, k));
-        }
-        s += (double)ld(z, q0 + c) * (double)rv[c];
-      }
-    } else {
+    const uint64_t M = nb8(L, q0);
+    {
       Nb8 N;
       load_nb(L, z, i, j, k0, q0, N);
       float rlv[8], out[8];
@@ -1059,8 +1040,10 @@ __global__ void __launch_bounds__(BS) k_last_dot8(Lv L, int bt, const S* __restr
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const int k = k0 + c;
+        const uint32_t mb = nbc(M, c);
         out[c] = N.c[c];
-        if (((i + j + k) & 1) == 0) out[c] = through<S>((rlv[c] + nsum8(L, N, i, j, k0, c)) / diag_free(L, bt, i, j, k));
+        if (mb & NB_SELF) continue;
+        if (((i + j + k) & 1) == 0) out[c] = through<S>((rlv[c] + nsum8(L, N, i, j, k0, c, mb)) / diag8(L, bt, i, j, k, mb));
         s += (double)out[c] * (double)rv[c];
       }
       st8(z + q0, out);
@@ -1268,6 +1251,289 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vtail(Tail T, int bt, int part
   }
 }
 
+// ---------------------------------------------------------------------------
+// Halo tiles for the coarse levels.  A level below the finest is a few
+// microseconds of memory traffic, so the seven launches of its two V-cycle
+// legs cost more than their work.  One workgroup per fine tile of 8 x 8 x TK
+// cells loads the tile and a halo of HT_HALO cells into LDS and runs a whole
+// leg there:
+//   down: black, red, black half-sweeps, then the restriction, fill(0) and
+//         first red half-sweep of the coarse level (as k_restrict8);
+//   up:   prolongation + black, red, black, red half-sweeps (as k_prolong8
+//         and three k_smooth8).
+// Half-sweep s (1-based) updates the cells at distance >= s from the box edge;
+// their neighbours are exact after s-1 half-sweeps, so after 3 (down) or 4
+// (up) the tile itself is exact.  Each cell update is the row-chunk kernels'
+// arithmetic (same neighbour order, z rounded through S after every update),
+// so a leg gives the values of the per-level launches.  Loads, sweeps and
+// stores are unrolled over compile-time extents so that each thread keeps
+// several independent accesses in flight (one workgroup's latency is the
+// leg's time).
+// ---------------------------------------------------------------------------
+constexpr int HT_HALO = 4;
+constexpr int HT_T = 8;                       // tile extent along i and j
+constexpr int HT_B = HT_T + 2 * HT_HALO;      // box extent along i and j
+constexpr int HT_COLS = HT_B * HT_B;          // box columns (along k)
+constexpr int HT_KS = 4;                      // threads per box column
+constexpr int HT_THREADS = HT_COLS * HT_KS;
+constexpr int HT_CB = HT_T / 2 + 4;           // coarse box extent along i and j (up leg)
+
+struct HTile {
+  int kp, ckp;    // LDS column strides (elements) of the fine and the coarse box
+  int ntj, ntk;   // tiles along j and k
+  int masked;     // solid bits staged (the level has solids)
+};
+template <int TK>
+struct HTK {
+  static constexpr int BK = TK + 2 * HT_HALO;   // box extent along k
+  static constexpr int CK = TK / 2 + 4;          // coarse box extent along k
+  static constexpr int SPAN = BK / HT_KS;        // box cells per thread along k
+};
+
+template <class S>
+__device__ __forceinline__ S to_s(float v);
+template <>
+__device__ __forceinline__ float to_s<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ __hip_bfloat16 to_s<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+
+template <class S>
+__device__ __forceinline__ float lds_f(const S* p, int q);
+template <>
+__device__ __forceinline__ float lds_f<float>(const float* p, int q) { return p[q]; }
+template <>
+__device__ __forceinline__ float lds_f<__hip_bfloat16>(const __hip_bfloat16* p, int q) {
+  return __bfloat162float(p[q]);
+}
+
+// box origin of this workgroup's tile
+template <int TK>
+__device__ __forceinline__ void ht_origin(const HTile& H, int& o0, int& o1, int& o2) {
+  int b = (int)blockIdx.x;
+  const int tk_ = b % H.ntk;
+  b /= H.ntk;
+  const int tj_ = b % H.ntj, ti_ = b / H.ntj;
+  o0 = ti_ * HT_T - HT_HALO;
+  o1 = tj_ * HT_T - HT_HALO;
+  o2 = tk_ * TK - HT_HALO;
+}
+
+// fine box <- z, r (and the solid bits); consecutive threads along k
+template <class S, int TK>
+__device__ __forceinline__ void ht_load(const Lv& L, const HTile& H, int o0, int o1, int o2, const S* __restrict__ z,
+                                        const S* __restrict__ r, S* zs, S* rs, uint8_t* ms) {
+  constexpr int BK = HTK<TK>::BK, N = HT_COLS * BK, PER = (N + HT_THREADS - 1) / HT_THREADS;
+  S zv[PER], rv[PER];
+  uint8_t mv[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * HT_THREADS;
+    const int col = t / BK, kk = t - col * BK;
+    const int i = o0 + col / HT_B, j = o1 + col % HT_B, k = o2 + kk;
+    const bool in = t < N && i >= 0 && i < L.n0 && j >= 0 && j < L.n1 && k >= 0 && k < L.n2;
+    const int64_t g = in ? lid(L, i, j, k) : 0;   // outside: load cell 0 and drop it
+    const S zg = z[g], rg = r[g];
+    zv[u] = in ? zg : to_s<S>(0.f);
+    rv[u] = in ? rg : to_s<S>(0.f);
+    mv[u] = 0;
+    if (H.masked) {
+      const uint8_t mg = L.nb[g];
+      mv[u] = in ? mg : (uint8_t)0;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * HT_THREADS;
+    if (t >= N) continue;
+    const int col = t / BK, kk = t - col * BK;
+    const int q = col * H.kp + kk;
+    zs[q] = zv[u];
+    rs[q] = rv[u];
+    if (H.masked) ms[q] = mv[u];
+  }
+}
+
+// fine tile -> z
+template <class S, int TK>
+__device__ __forceinline__ void ht_store(const Lv& L, const HTile& H, int o0, int o1, int o2, const S* zs,
+                                         S* __restrict__ z) {
+  constexpr int N = HT_T * HT_T * TK, PER = (N + HT_THREADS - 1) / HT_THREADS;
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int t = threadIdx.x + u * HT_THREADS;
+    if (t >= N) continue;
+    const int c = t / TK, kk = t - c * TK;
+    const int bi = HT_HALO + c / HT_T, bj = HT_HALO + c % HT_T;
+    z[lid(L, o0 + bi, o1 + bj, o2 + HT_HALO + kk)] = zs[(bi * HT_B + bj) * H.kp + HT_HALO + kk];
+  }
+}
+
+// one half-sweep of colour `phase` over the cells at box distance >= dist; PRO
+// adds the coarse correction of each neighbour (prolongation, as k_prolong8).
+// All of a thread's new values are computed before any is stored (cells of
+// one colour do not read each other).
+template <class S, int TK, bool PRO>
+__device__ __forceinline__ void ht_sweep(const Lv& L, int bt, const HTile& H, int o0, int o1, int o2, const S* rs,
+                                         S* zs, const uint8_t* ms, const S* zcs, int phase, int dist) {
+  constexpr int BK = HTK<TK>::BK, SPAN = HTK<TK>::SPAN, NU = (SPAN + 1) / 2;
+  const int col = threadIdx.x % HT_COLS, part = threadIdx.x / HT_COLS;
+  const int bi = col / HT_B, bj = col % HT_B;
+  const int i = o0 + bi, j = o1 + bj;
+  const int dc = min(min(bi, HT_B - 1 - bi), min(bj, HT_B - 1 - bj));
+  if (dc >= dist && i >= 0 && i < L.n0 && j >= 0 && j < L.n1) {
+    const int lo = max(part * SPAN, dist), hi = min((part + 1) * SPAN, BK - dist);
+    const int first = lo + (((i + j + o2 + lo) & 1) != phase);
+    const int SI = HT_B * H.kp, SJ = H.kp;
+    const int c0 = (o0 + HT_HALO) / 2 - 2, c1 = (o1 + HT_HALO) / 2 - 2, c2 = (o2 + HT_HALO) / 2 - 2;
+    const int CI = HT_CB * H.ckp, CJ = H.ckp;
+    auto zc = [&](int a, int b, int c) {
+      return lds_f(zcs, ((a >> 1) - c0) * CI + ((b >> 1) - c1) * CJ + ((c >> 1) - c2));
+    };
+    // Every read is issued unconditionally (the box holds every neighbour of a
+    // cell at distance >= 1, and the padded column one more element) and
+    // masked by selects: a term left out adds +0, which leaves a sum that is
+    // never -0 unchanged.
+    const bool ip = i < L.n0 - 1, im = i > 0, jp = j < L.n1 - 1, jm = j > 0;
+    float nv[NU];
+    bool ok[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int bk = first + 2 * u;
+      const int k = o2 + bk;
+      const int q = col * H.kp + bk;
+      const uint32_t mb = H.masked ? (uint32_t)ms[q] : 0u;
+      ok[u] = bk < hi && k >= 0 && k < L.n2 && !(mb & NB_SELF);
+      float v[6];
+      v[0] = lds_f(zs, q + SI);
+      v[1] = lds_f(zs, q - SI);
+      v[2] = lds_f(zs, q + SJ);
+      v[3] = lds_f(zs, q - SJ);
+      v[4] = lds_f(zs, q + 1);
+      v[5] = lds_f(zs, q - 1);
+      if (PRO) {
+        v[0] += zc(i + 1, j, k);
+        v[1] += zc(i - 1, j, k);
+        v[2] += zc(i, j + 1, k);
+        v[3] += zc(i, j - 1, k);
+        v[4] += zc(i, j, k + 1);
+        v[5] += zc(i, j, k - 1);
+      }
+      const bool use[6] = {ip && !(mb & 1u), im && !(mb & 2u), jp && !(mb & 4u), jm && !(mb & 8u),
+                           k < L.n2 - 1 && !(mb & 16u), k > 0 && !(mb & 32u)};
+      float ret = 0.f;
+#pragma unroll
+      for (int e = 0; e < 6; ++e) ret += use[e] ? v[e] : 0.f;
+      nv[u] = (lds_f(rs, q) + ret) / diag8(L, bt, i, j, k, mb);
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u)
+      if (ok[u]) zs[col * H.kp + first + 2 * u] = to_s<S>(nv[u]);
+  }
+  __syncthreads();
+}
+
+template <class S, int TK>
+__device__ __forceinline__ void ht_carve(const HTile& H, unsigned char* base, S*& zs, S*& rs, uint8_t*& ms, S*& zcs) {
+  zs = (S*)base;
+  rs = zs + HT_COLS * H.kp;
+  zcs = rs + HT_COLS * H.kp;
+  ms = (uint8_t*)(zcs + HT_CB * HT_CB * H.ckp);
+}
+
+// Down leg of level F (>= 1) into coarse level C.
+template <class S, int TK>
+__global__ void __launch_bounds__(HT_THREADS) k_down_tile(Lv F, Lv C, int bt, HTile H, const S* __restrict__ rf,
+                                                         S* __restrict__ zf, S* __restrict__ rc, S* __restrict__ zc,
+                                                         const double* __restrict__ sc, int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  extern __shared__ __align__(16) unsigned char ht_lds[];
+  S *zs, *rs, *zcs;
+  uint8_t* ms;
+  ht_carve<S, TK>(H, ht_lds, zs, rs, ms, zcs);
+  int o0, o1, o2;
+  ht_origin<TK>(H, o0, o1, o2);
+  ht_load<S, TK>(F, H, o0, o1, o2, zf, rf, zs, rs, ms);
+  __syncthreads();
+  ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 1, 1);
+  ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 0, 2);
+  ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 1, 3);
+  ht_store<S, TK>(F, H, o0, o1, o2, zs, zf);
+  // restriction of the tile's residual (its 4 x 4 x TK/2 coarse cells)
+  constexpr int HK = TK / 2;
+  const int SI = HT_B * H.kp, SJ = H.kp;
+  const int t = threadIdx.x;
+  if (t < (HT_T / 2) * (HT_T / 2) * HK) {
+    const int a = t / ((HT_T / 2) * HK), b = (t / HK) % (HT_T / 2), c = t % HK;
+    const int I = (o0 + HT_HALO) / 2 + a, J = (o1 + HT_HALO) / 2 + b, K = (o2 + HT_HALO) / 2 + c;
+    float acc = 0.f;
+#pragma unroll
+    for (int c8 = 0; c8 < 8; ++c8) {
+      const int bi = HT_HALO + 2 * a + (c8 >> 2), bj = HT_HALO + 2 * b + ((c8 >> 1) & 1), bk = HT_HALO + 2 * c + (c8 & 1);
+      const int i = o0 + bi, j = o1 + bj, k = o2 + bk;
+      const int q = (bi * HT_B + bj) * H.kp + bk;
+      const uint32_t mb = H.masked ? (uint32_t)ms[q] : 0u;
+      const float v[6] = {lds_f(zs, q + SI), lds_f(zs, q - SI), lds_f(zs, q + SJ),
+                          lds_f(zs, q - SJ), lds_f(zs, q + 1), lds_f(zs, q - 1)};
+      const bool use[6] = {i < F.n0 - 1 && !(mb & 1u), i > 0 && !(mb & 2u), j < F.n1 - 1 && !(mb & 4u),
+                           j > 0 && !(mb & 8u), k < F.n2 - 1 && !(mb & 16u), k > 0 && !(mb & 32u)};
+      float ns = 0.f;
+#pragma unroll
+      for (int e = 0; e < 6; ++e) ns += use[e] ? v[e] : 0.f;
+      const float res = lds_f(rs, q) - (diag8(F, bt, i, j, k, mb) * lds_f(zs, q) - ns);
+      acc += (mb & NB_SELF) ? 0.f : res * 1.0f / 2.0f;
+    }
+    const int64_t qc = lid(C, I, J, K);
+    st(rc, qc, acc);
+    const uint32_t mbc = C.nb ? (uint32_t)C.nb[qc] : 0u;
+    float zv = 0.f;
+    if (((I + J + K) & 1) == 0 && !(mbc & NB_SELF)) zv = through<S>(acc) / diag8(C, bt, I, J, K, mbc);
+    st(zc, qc, zv);
+  }
+}
+
+// Up leg of level F (>= 1) from the coarse level's z.
+template <class S, int TK>
+__global__ void __launch_bounds__(HT_THREADS) k_up_tile(Lv F, Lv C, int bt, HTile H, const S* __restrict__ rf,
+                                                       S* __restrict__ zf, const S* __restrict__ zc,
+                                                       const double* __restrict__ sc, int ignore_done) {
+  if (!ignore_done && sc[S_DONE] != 0.0) return;
+  extern __shared__ __align__(16) unsigned char ht_lds[];
+  S *zs, *rs, *zcs;
+  uint8_t* ms;
+  ht_carve<S, TK>(H, ht_lds, zs, rs, ms, zcs);
+  int o0, o1, o2;
+  ht_origin<TK>(H, o0, o1, o2);
+  ht_load<S, TK>(F, H, o0, o1, o2, zf, rf, zs, rs, ms);
+  {
+    // coarse box: the parents of the fine box's cells
+    constexpr int CK = HTK<TK>::CK, N = HT_CB * HT_CB * CK, PER = (N + HT_THREADS - 1) / HT_THREADS;
+    const int c0 = (o0 + HT_HALO) / 2 - 2, c1 = (o1 + HT_HALO) / 2 - 2, c2 = (o2 + HT_HALO) / 2 - 2;
+    S v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int t = threadIdx.x + u * HT_THREADS;
+      const int col = t / CK, kk = t - col * CK;
+      const int I = c0 + col / HT_CB, J = c1 + col % HT_CB, K = c2 + kk;
+      const bool in = t < N && I >= 0 && I < C.n0 && J >= 0 && J < C.n1 && K >= 0 && K < C.n2;
+      const S g = zc[in ? lid(C, I, J, K) : 0];
+      v[u] = in ? g : to_s<S>(0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int t = threadIdx.x + u * HT_THREADS;
+      if (t >= N) continue;
+      const int col = t / CK, kk = t - col * CK;
+      zcs[col * H.ckp + kk] = v[u];
+    }
+  }
+  __syncthreads();
+  ht_sweep<S, TK, true>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 1, 1);
+  ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 0, 2);
+  ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 1, 3);
+  ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 0, 4);
+  ht_store<S, TK>(F, H, o0, o1, o2, zs, zf);
+}
+
 // Bottom of the mixed V-cycle: z = B^-1 r over the fluid cells of the coarsest
 // level, B = A + c 1 1^T (SPD; equal to the pseudo-inverse on mean-free r), FP64.
 template <class S>
@@ -1309,6 +1575,23 @@ __global__ void k_downsample(Lv F, Lv C, int8_t* __restrict__ out) {
   out[q] = (int8_t)all;
 }
 
+// the solid bits of every cell of a level (from its mask)
+__global__ void k_nbmask(Lv L, uint8_t* __restrict__ out) {
+  const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  int i, j, k;
+  decode(L, q, i, j, k);
+  uint32_t m = L.bm[q] > 0 ? NB_SELF : 0u;
+  if (i < L.n0 - 1 && solid_at(L, i + 1, j, k)) m |= 1u;
+  if (i > 0 && solid_at(L, i - 1, j, k)) m |= 2u;
+  if (j < L.n1 - 1 && solid_at(L, i, j + 1, k)) m |= 4u;
+  if (j > 0 && solid_at(L, i, j - 1, k)) m |= 8u;
+  if (k < L.n2 - 1 && solid_at(L, i, j, k + 1)) m |= 16u;
+  if (k > 0 && solid_at(L, i, j, k - 1)) m |= 32u;
+  out[q] = (uint8_t)m;
+}
+
 __global__ void k_set_scalars(double* sc, int n, const double* vals) {
   int t = threadIdx.x;
   if (t < n) sc[t] = vals[t];
@@ -1346,6 +1629,7 @@ struct Solver {
   void* rl[MAXL];   // level residual (level 0 aliases r in plain mode)
   void* zl[MAXL];
   int8_t* bm[MAXL];  // null when no solids
+  uint8_t* nbm[MAXL];  // solid bits per cell (k_nbmask), null when no solids
   float *r, *x, *p, *Ap;
   double* partials;
   double* sc;
@@ -1367,6 +1651,8 @@ struct Solver {
   int hist[4] = {0, 0, 0, 0};            // iterations of the last four solves, newest first (adaptive batches)
   // coarse levels from tail_from on go to one workgroup (0: never)
   int tail_from = 0;
+  // halo-tile legs for the levels between the finest and the tail/bottom
+  int use_tiles = 1;
 };
 
 // First level of the one-workgroup tail for the current setup (0 = no tail):
@@ -1426,6 +1712,37 @@ int bottom_lds_bytes(const Solver& s) { return (int)(s.nl[s.L - 1] * sizeof(floa
 
 int chunk_blocks(int64_t ncells) { return blocks_for(ncells / 8, BS); }
 
+// LDS column stride: an odd number of 32-bit words between neighbouring columns
+int ht_stride(int extent, size_t elem) {
+  if (elem == 4) return extent | 1;
+  return extent + ((2 - extent % 4) + 4) % 4;   // 2-byte elements: extent = 2 (mod 4)
+}
+constexpr int HT_MAX_LDS = 128 * 1024;
+
+// Halo-tile plan of level l (between the finest and the coarsest): false when
+// the level's extents do not split into 8 x 8 x tk tiles.
+bool ht_plan(const Solver& s, int l, size_t elem, HTile& H, int& tk_out, int& blocks, int& lds) {
+  if (!s.use_tiles || l < 1 || l >= s.L - 1) return false;
+  const Lv& L = s.lv[l];
+  if (L.n0 % HT_T || L.n1 % HT_T) return false;
+  int tk = 0;
+  for (int c : {32, 16, 8})
+    if (L.n2 % c == 0) {
+      tk = c;
+      break;
+    }
+  if (!tk) return false;
+  tk_out = tk;
+  H.kp = ht_stride(tk + 2 * HT_HALO, elem);
+  H.ckp = ht_stride(tk / 2 + 4, elem);
+  H.ntj = L.n1 / HT_T;
+  H.ntk = L.n2 / tk;
+  H.masked = L.nb != nullptr;
+  blocks = (L.n0 / HT_T) * H.ntj * H.ntk;
+  lds = (int)(elem * (2 * HT_COLS * H.kp + HT_CB * HT_CB * H.ckp)) + (H.masked ? HT_COLS * H.kp : 0);
+  return lds <= HT_MAX_LDS;
+}
+
 // one red-black half-sweep of level l
 template <class S>
 void smooth(Solver& s, hipStream_t st, int l, const S* r, S* z, int phase, int ig) {
@@ -1468,6 +1785,14 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
     S* zl = (S*)s.zl[l];
     int64_t half = s.nl[l] / 2;
     (void)half;
+    HTile H;
+    int tk = 0, hb = 0, hl = 0;
+    if (ht_plan(s, l, sizeof(S), H, tk, hb, hl)) {
+      auto kern = tk == 32 ? k_down_tile<S, 32> : (tk == 16 ? k_down_tile<S, 16> : k_down_tile<S, 8>);
+      hipLaunchKernelGGL(kern, dim3(hb), dim3(HT_THREADS), hl, st, s.lv[l], s.lv[l + 1], s.bt, H, rl, zl,
+                         (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
+      continue;
+    }
     // remaining pre-smoothing: black, red, black (sim/mgpcg.py:180-182)
     smooth<S>(s, st, l, rl, zl, 1, ig);
     smooth<S>(s, st, l, rl, zl, 0, ig);
@@ -1508,6 +1833,14 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
     S* rl = (S*)s.rl[l];
     S* zl = (S*)s.zl[l];
     int64_t half = s.nl[l] / 2;
+    HTile H;
+    int tk = 0, hb = 0, hl = 0;
+    if (ht_plan(s, l, sizeof(S), H, tk, hb, hl)) {
+      auto kern = tk == 32 ? k_up_tile<S, 32> : (tk == 16 ? k_up_tile<S, 16> : k_up_tile<S, 8>);
+      hipLaunchKernelGGL(kern, dim3(hb), dim3(HT_THREADS), hl, st, s.lv[l], s.lv[l + 1], s.bt, H, rl, zl,
+                         (const S*)s.zl[l + 1], sc, ig);
+      continue;
+    }
     if (chunked(s.lv[l]))
       hipLaunchKernelGGL(k_prolong8<S>, dim3(chunk_blocks(s.nl[l])), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt, rl,
                          zl, (const S*)s.zl[l + 1], sc, ig);
@@ -1612,6 +1945,7 @@ void free_solver(Solver* s) {
     if (s->rl[l] && s->rl[l] != (void*)s->r) (void)hipFree(s->rl[l]);
     if (s->zl[l]) (void)hipFree(s->zl[l]);
     if (s->bm[l]) (void)hipFree(s->bm[l]);
+    if (s->nbm[l]) (void)hipFree(s->nbm[l]);
   }
   for (void* p : {(void*)s->r, (void*)s->x, (void*)s->p, (void*)s->Ap, (void*)s->partials,
                   (void*)s->sc, (void*)s->Binv, (void*)s->fidx})
@@ -1644,6 +1978,10 @@ int prepare_masks(Solver& s, hipStream_t st) {
   EDGE_CHECK_LAUNCH();
   for (int l = 1; l < s.L; ++l) {
     hipLaunchKernelGGL(k_downsample, dim3(blocks_for(s.nl[l], BS)), dim3(BS), 0, st, s.lv[l - 1], s.lv[l], s.bm[l]);
+    EDGE_CHECK_LAUNCH();
+  }
+  for (int l = 0; l < s.L; ++l) {
+    hipLaunchKernelGGL(k_nbmask, dim3(blocks_for(s.nl[l], BS)), dim3(BS), 0, st, s.lv[l], s.nbm[l]);
     EDGE_CHECK_LAUNCH();
   }
   if (s.mode == 1) {
@@ -1827,6 +2165,7 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
   std::memset(s->rl, 0, sizeof(s->rl));
   std::memset(s->zl, 0, sizeof(s->zl));
   std::memset(s->bm, 0, sizeof(s->bm));
+  std::memset(s->nbm, 0, sizeof(s->nbm));
   s->r = s->x = s->p = s->Ap = nullptr;
   s->partials = nullptr;
   s->sc = nullptr;
@@ -1878,6 +2217,18 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) s->tail_from = 0;  // per-level kernels instead
   }
+  {
+    const void* kerns[2][6] = {
+        {(const void*)k_down_tile<float, 8>, (const void*)k_down_tile<float, 16>, (const void*)k_down_tile<float, 32>,
+         (const void*)k_up_tile<float, 8>, (const void*)k_up_tile<float, 16>, (const void*)k_up_tile<float, 32>},
+        {(const void*)k_down_tile<__hip_bfloat16, 8>, (const void*)k_down_tile<__hip_bfloat16, 16>,
+         (const void*)k_down_tile<__hip_bfloat16, 32>, (const void*)k_up_tile<__hip_bfloat16, 8>,
+         (const void*)k_up_tile<__hip_bfloat16, 16>, (const void*)k_up_tile<__hip_bfloat16, 32>}};
+    bool ok = true;
+    for (const void* k : kerns[mode == 1 ? 1 : 0])
+      ok = ok && hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, HT_MAX_LDS) == hipSuccess;
+    if (!ok) s->use_tiles = 0;  // per-level kernels instead
+  }
   if (hipDeviceSynchronize() != hipSuccess) return fail("hipDeviceSynchronize");
   *handle = s;
   return EDGE_OK;
@@ -1903,7 +2254,7 @@ int edge_mg_set_solid(void* handle, const int* bm, const float* bvel) {
     drop_graph(s);  // captured launches hold the mask pointers
   if (!bm) {
     s.solid = false;
-    for (int l = 0; l < s.L; ++l) s.lv[l].bm = nullptr;
+    for (int l = 0; l < s.L; ++l) s.lv[l].bm = nullptr, s.lv[l].nb = nullptr;
     s.user_bm = nullptr;
     s.user_bvel = nullptr;
     return EDGE_OK;
@@ -1911,7 +2262,9 @@ int edge_mg_set_solid(void* handle, const int* bm, const float* bvel) {
   EDGE_REQUIRE(bvel, "edge_mg_set_solid: boundary velocity required with a mask");
   for (int l = 0; l < s.L; ++l) {
     if (!s.bm[l]) EDGE_CHECK_HIP(hipMalloc(&s.bm[l], s.nl[l]));
+    if (!s.nbm[l]) EDGE_CHECK_HIP(hipMalloc(&s.nbm[l], s.nl[l]));
     s.lv[l].bm = s.bm[l];
+    s.lv[l].nb = s.nbm[l];
   }
   s.solid = true;
   s.user_bm = bm;
@@ -2066,6 +2419,17 @@ int edge_mg_set_graph(void* handle, int enable) {
   clear_error();
   EDGE_REQUIRE(handle, "edge_mg_set_graph: null handle");
   ((Solver*)handle)->use_graph = enable ? 1 : 0;
+  return EDGE_OK;
+}
+
+// Halo-tile legs on (default) or off for the levels between the finest and
+// the coarsest; off runs every half-sweep as its own launch.
+int edge_mg_set_tiles(void* handle, int enable) {
+  clear_error();
+  EDGE_REQUIRE(handle, "edge_mg_set_tiles: null handle");
+  Solver& s = *(Solver*)handle;
+  if ((enable ? 1 : 0) != s.use_tiles) drop_graph(s);
+  s.use_tiles = enable ? 1 : 0;
   return EDGE_OK;
 }
 

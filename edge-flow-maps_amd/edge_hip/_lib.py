@@ -62,6 +62,7 @@ SIGNATURES = {
     "edge_mg_apply_A": "pppp",
     "edge_mg_dense_bottom": "p",
     "edge_mg_set_graph": "pi",
+    "edge_mg_set_tiles": "pi",
     # learned warm start
     "edge_mg_rhs": "pppppp",
     "edge_conv3d_bf16_fwd": "pipipppiiiiip",

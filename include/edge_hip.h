@@ -188,6 +188,11 @@ int edge_mg_apply_A(void* handle, const float* p, float* Ap, void* stream);
  * instead of launching their kernels one by one.  No
  * counterpart in the reference (its CG loop is host-driven, sim/mgpcg.py:234). */
 int edge_mg_set_graph(void* handle, int enable);
+/* Run each coarse level's V-cycle legs (pre-smoothing + restriction,
+ * prolongation + post-smoothing) as one LDS halo-tile launch (enable=1, the
+ * default) or as one launch per half-sweep (0).  Same values either way.  No
+ * counterpart in the reference (sim/mgpcg.py:177-195 launches per half-sweep). */
+int edge_mg_set_tiles(void* handle, int enable);
 /* 1 if the mixed solver uses the dense coarsest solve (0: red-black bottom). */
 int edge_mg_dense_bottom(void* handle);
 

@@ -260,3 +260,40 @@ def test_adaptive_batches_match_single_steps(dev, graph):
     # a fixed batch larger than the iterations needed stops at the same iteration
     p, st, code = S.poisson([T_(c, dev) for c in u], stop_mode=1, rel_tol=1e-6, check_every=8)
     assert code == 0 and st[0] == st1[0]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_tile_legs_match_per_level_launches(dev, mode):
+    # 64x64x32, 4 levels: levels 1 (32x32x16) and 2 (16x16x8) run as halo-tile legs.
+    # The same cell updates in the same order; only FMA contraction may differ:
+    # FP32 to 1e-6 of scale, BF16 storage to a few BF16 ulps of scale.
+    n = (64, 64, 32)
+    rng = np.random.default_rng(21)
+    r = rng.normal(size=n)
+    r -= r.mean()
+    bm = np.zeros(n, np.int32)
+    bm[20:30, 10:40, 9:20] = 1
+    bm[40:44, 50:64, 0:5] = 1
+    tol = 1e-6 if mode == 0 else 3e-2
+    for solid in (False, True):
+        out = []
+        for tiles in (0, 1):
+            S = Solver(n, 4, mode)
+            S.lib.call("edge_mg_set_tiles", S.h, tiles)
+            rr = r
+            if solid:
+                S.set_solid(torch.from_numpy(bm).to(dev), T_(np.zeros((3,) + n, np.float32), dev))
+                rr = r * (bm == 0)
+            out.append(N_(S.vcycle(T_(rr, dev))))
+        fl = (bm == 0) if solid else np.ones(n, bool)
+        assert scale_err(out[1][fl], out[0][fl]) < tol, (solid, scale_err(out[1][fl], out[0][fl]))
+    # a full solve: same iterations, same pressure
+    u = smooth_velocity(*n, DX, seed=22)
+    res = []
+    for tiles in (0, 1):
+        S = Solver(n, 4, mode)
+        S.lib.call("edge_mg_set_tiles", S.h, tiles)
+        res.append(S.poisson([T_(c, dev) for c in u], stop_mode=1, rel_tol=1e-6))
+    assert res[0][2] == 0 and res[1][2] == 0
+    assert res[1][1][0] == res[0][1][0]
+    assert scale_err(N_(res[1][0]), N_(res[0][0])) < (1e-5 if mode == 0 else 1e-3)
