@@ -46,11 +46,17 @@ __global__ void __launch_bounds__(256) k_map_u_psi(Grid g, Mac U0, Faces U1, con
   int a, i, j, k;
   int64_t l, nf;
   if (!face_of(g, t, a, l, nf, i, j, k)) return;
-  float X[3], psi[3], G[3][3], v[3];
+  float X[3], psi[3], Gc[3], v[3];
   face_coord(a, i, j, k, g.dx, X);
-  map_eval<true>(g, tab, X, psi, G);
+  // only column a of grad psi enters
+  if (a == 0)
+    map_eval_dir<0>(g, tab, X, psi, Gc);
+  else if (a == 1)
+    map_eval_dir<1>(g, tab, X, psi, Gc);
+  else
+    map_eval_dir<2>(g, tab, X, psi, Gc);
   interp_u_mac<false>(g, U0, psi, v, nullptr);
-  U1.f[a][l] = G[0][a] * v[0] + G[1][a] * v[1] + G[2][a] * v[2];
+  U1.f[a][l] = Gc[0] * v[0] + Gc[1] * v[1] + Gc[2] * v[2];
 }
 
 // run_ed8.py:190-223 split_psi_hermite: phi_a = psi(X_a), F_a = column a of grad psi.
@@ -59,13 +65,19 @@ __global__ void __launch_bounds__(256) k_split_psi(Grid g, const float4* __restr
   int a, i, j, k;
   int64_t l, nf;
   if (!face_of(g, t, a, l, nf, i, j, k)) return;
-  float X[3], psi[3], G[3][3];
+  float X[3], psi[3], Gc[3];
   face_coord(a, i, j, k, g.dx, X);
-  map_eval<true, true>(g, tab, X, psi, G);
+  // only column a of grad psi is kept
+  if (a == 0)
+    map_eval_dir<0, true>(g, tab, X, psi, Gc);
+  else if (a == 1)
+    map_eval_dir<1, true>(g, tab, X, psi, Gc);
+  else
+    map_eval_dir<2, true>(g, tab, X, psi, Gc);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     PHI.f[a][c * nf + l] = psi[c];
-    F.f[a][c * nf + l] = G[c][a];
+    F.f[a][c * nf + l] = Gc[c];
   }
 }
 
@@ -210,7 +222,7 @@ __global__ void __launch_bounds__(256) k_advect_sl(Grid g, Mac U0, Faces U1, flo
   int a, i, j, k;
   int64_t l, nf;
   if (!face_of(g, t, a, l, nf, i, j, k)) return;
-  float p1[3], q[3], v1[3], v2[3], v3[3], v4[3], v5[3];
+  float p1[3], q[3], v1[3], v2[3], v3[3], v4[3];
   face_coord(a, i, j, k, g.dx, p1);
   interp_u_mac<false>(g, U0, p1, v1, nullptr);
 #pragma unroll
@@ -224,8 +236,7 @@ __global__ void __launch_bounds__(256) k_advect_sl(Grid g, Mac U0, Faces U1, flo
   interp_u_mac<false>(g, U0, q, v4, nullptr);
 #pragma unroll
   for (int d = 0; d < 3; ++d) q[d] = p1[d] - (v1[d] + 2.0f * v2[d] + 2.0f * v3[d] + v4[d]) / 6.0f * dt;
-  interp_u_mac<false>(g, U0, q, v5, nullptr);
-  U1.f[a][l] = v5[a];
+  U1.f[a][l] = interp_u_comp(g, U0, q, a);   // the face's own component at the departure point
 }
 
 // run_edge.py:295-303 calc_max_speed: max over cells of the centred speed, floor 1e-3.

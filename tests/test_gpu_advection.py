@@ -144,8 +144,13 @@ def test_driver_maps_parity(orc32, orc64, dev):
     # map_u_psi
     out = ops.map_u_psi(gu, tab, dx)
     ref = orc32.map_u_psi(u, psi, T, dT, dx)
+    ref64 = orc64.map_u_psi(u, psi, T, dT, dx)
     for a in range(3):
-        assert scale_err(N_(out[a]), ref[a]) < 1e-5
+        # the FP32 oracle itself is ~1e-5 of scale from FP64 here (sum of
+        # d psi_c/dx_a u_c with cancellation): judge both against FP64
+        e32 = scale_err(ref[a], ref64[a])
+        assert scale_err(N_(out[a]), ref64[a]) <= max(2.0 * e32, 1e-5), (a, e32)
+        assert scale_err(N_(out[a]), ref[a]) < 5e-5
     # smoke maps and clamps through the map
     rng = np.random.default_rng(12)
     s0 = rng.uniform(0, 1, size=(4,) + n)
