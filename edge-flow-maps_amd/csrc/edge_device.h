@@ -300,48 +300,75 @@ EDGE_DEV HPoint hermite_point(const Grid& g, const float p[3]) {
 // Value (and gradient in index units, i.e. before the final 1/dx) of one
 // component from its table.  Corner indices are always inside the grid: the
 // clamp keeps the base in [1, dim-2].
+// The 8 corners x 8 coefficients form one separable 4 x 4 x 4 tensor
+// C[(i,tx)][(j,ty)][(k,tz)] (corner i/j/k, basis F or G per axis), contracted
+// one axis at a time over all corners: z (value and derivative weights), then
+// y, then x: 84 operations for the value, 192 with the gradient (288 corner by
+// corner).
 template <bool GRAD>
 EDGE_DEV float hermite_eval(const Grid& g, Rsrc tab, const HPoint& P, float gr[3]) {
   const uint32_t s1 = (uint32_t)g.nz, s0 = (uint32_t)g.ny * s1;
   const uint32_t vb = 32u * ((uint32_t)P.X.i0 * s0 + (uint32_t)P.Y.i0 * s1 + (uint32_t)P.Z.i0);
+  // per-axis weights of the combined index (corner, basis): [2 * corner + basis]
+  const float Zw[4] = {P.Z.F[0], P.Z.G[0], P.Z.F[1], P.Z.G[1]};
+  const float Yw[4] = {P.Y.F[0], P.Y.G[0], P.Y.F[1], P.Y.G[1]};
+  const float Xw[4] = {P.X.F[0], P.X.G[0], P.X.F[1], P.X.G[1]};
+  const float dZw[4] = {P.Z.dF[0], P.Z.dG[0], P.Z.dF[1], P.Z.dG[1]};
+  const float dYw[4] = {P.Y.dF[0], P.Y.dG[0], P.Y.dF[1], P.Y.dG[1]};
+  const float dXw[4] = {P.X.dF[0], P.X.dG[0], P.X.dF[1], P.X.dG[1]};
   float val = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
+    // the four corners (i, j, k) of this x-layer: c[j][k][tx][ty][tz]
+    float c[2][2][2][2][2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const uint32_t so = 32u * (i * s0 + j * s1);
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        float4 lo = bload4(tab, vb + 32u * k, so), hi = bload4(tab, vb + 32u * k + 16u, so);
-        // coefficient tensor c[a][b][cz] (a,b,cz in {f=0, g=1})
-        float cfff = lo.x, cgff = lo.y, cfgf = lo.z, cffg = lo.w;
-        float cfgg = hi.x, cgfg = hi.y, cggf = hi.z, cggg = hi.w;
-        float Zf = P.Z.F[k], Zg = P.Z.G[k];
-        // contract z
-        float sff = cfff * Zf + cffg * Zg;
-        float sfg = cfgf * Zf + cfgg * Zg;
-        float sgf = cgff * Zf + cgfg * Zg;
-        float sgg = cggf * Zf + cggg * Zg;
-        float Yf = P.Y.F[j], Yg = P.Y.G[j];
-        float tf = sff * Yf + sfg * Yg;
-        float tg = sgf * Yf + sgg * Yg;
-        float Xf = P.X.F[i], Xg = P.X.G[i];
-        val += tf * Xf + tg * Xg;
-        if (GRAD) {
-          float dZf = P.Z.dF[k], dZg = P.Z.dG[k];
-          float dff = cfff * dZf + cffg * dZg;
-          float dfg = cfgf * dZf + cfgg * dZg;
-          float dgf = cgff * dZf + cgfg * dZg;
-          float dgg = cggf * dZf + cggg * dZg;
-          float dYf = P.Y.dF[j], dYg = P.Y.dG[j];
-          float ty_f = sff * dYf + sfg * dYg;
-          float ty_g = sgf * dYf + sgg * dYg;
-          float tz_f = dff * Yf + dfg * Yg;
-          float tz_g = dgf * Yf + dgg * Yg;
-          gx += tf * P.X.dF[i] + tg * P.X.dG[i];
-          gy += ty_f * Xf + ty_g * Xg;
-          gz += tz_f * Xf + tz_g * Xg;
+        const float4 lo = bload4(tab, vb + 32u * k, so), hi = bload4(tab, vb + 32u * k + 16u, so);
+        c[j][k][0][0][0] = lo.x;
+        c[j][k][1][0][0] = lo.y;
+        c[j][k][0][1][0] = lo.z;
+        c[j][k][0][0][1] = lo.w;
+        c[j][k][0][1][1] = hi.x;
+        c[j][k][1][0][1] = hi.y;
+        c[j][k][1][1][0] = hi.z;
+        c[j][k][1][1][1] = hi.w;
+      }
+    }
+#pragma unroll
+    for (int tx = 0; tx < 2; ++tx) {
+      float t = 0.f, ty = 0.f, tz = 0.f;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int tyb = 0; tyb < 2; ++tyb) {
+          // contract z over K = (k, tz)
+          float sz = 0.f, dsz = 0.f;
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+#pragma unroll
+            for (int tzb = 0; tzb < 2; ++tzb) {
+              const float cv = c[j][k][tx][tyb][tzb];
+              sz += cv * Zw[2 * k + tzb];
+              if (GRAD) dsz += cv * dZw[2 * k + tzb];
+            }
+          }
+          const int J = 2 * j + tyb;
+          t += sz * Yw[J];
+          if (GRAD) {
+            ty += sz * dYw[J];
+            tz += dsz * Yw[J];
+          }
         }
+      }
+      const int I = 2 * i + tx;
+      val += t * Xw[I];
+      if (GRAD) {
+        gx += t * dXw[I];
+        gy += ty * Xw[I];
+        gz += tz * Xw[I];
       }
     }
   }

@@ -1725,12 +1725,14 @@ bool ht_plan(const Solver& s, int l, size_t elem, HTile& H, int& tk_out, int& bl
   if (!s.use_tiles || l < 1 || l >= s.L - 1) return false;
   const Lv& L = s.lv[l];
   if (L.n0 % HT_T || L.n1 % HT_T) return false;
+  // the deepest tile that still gives one workgroup per CU, else the shallowest
+  // (a leg's time is one workgroup's latency, which grows with the tile)
   int tk = 0;
-  for (int c : {32, 16, 8})
-    if (L.n2 % c == 0) {
-      tk = c;
-      break;
-    }
+  for (int c : {32, 16, 8}) {
+    if (L.n2 % c) continue;
+    tk = c;
+    if ((int64_t)(L.n0 / HT_T) * (L.n1 / HT_T) * (L.n2 / c) >= 256) break;
+  }
   if (!tk) return false;
   tk_out = tk;
   H.kp = ht_stride(tk + 2 * HT_HALO, elem);
