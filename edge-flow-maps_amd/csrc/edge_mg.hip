@@ -1647,7 +1647,7 @@ struct Solver {
   // CG iteration batches captured as a HIP graph (replayed on the caller's stream)
   int use_graph = 0;
   hipStream_t cap = nullptr;
-  hipGraphExec_t gexec[MAXB + 1] = {};  // one captured batch per size 1..MAXB
+  hipGraphExec_t gexec[MAXB + 1] = {};  // [0]: the first iteration; [b]: a batch of b iterations
   int hist[4] = {0, 0, 0, 0};            // iterations of the last four solves, newest first (adaptive batches)
   // coarse levels from tail_from on go to one workgroup (0: never)
   int tail_from = 0;
@@ -1999,9 +1999,10 @@ int prepare_masks(Solver& s, hipStream_t st) {
   return EDGE_OK;
 }
 
-// One CG iteration after the first preconditioning (sim/mgpcg.py:234-278).
+// The update half of a CG iteration: Ap, alpha, x, r, r.r and the stop test
+// (sim/mgpcg.py:234-259).
 template <class S>
-void cg_iteration(Solver& s, hipStream_t st) {
+void cg_update(Solver& s, hipStream_t st) {
   Red R = red(s);
   const Lv& L0 = s.lv[0];
   if (chunked(L0)) {
@@ -2017,16 +2018,35 @@ void cg_iteration(Solver& s, hipStream_t st) {
                        s.Ap, R);
     finish(s, st, red_blocks(s.N), F_XR, 1);
   }
-  Ops<S>::vcycle(s, st, 0);
-  last_dot<S>(s, st, F_ZR, 0, 1);
 }
 
+// A CG iteration, rotated so that it ends with the stop test: z = M r, beta
+// and p (sim/mgpcg.py:260-278 of the previous iteration), then the update.  A
+// batch that ends at convergence then launches nothing after the test.
+template <class S>
+void cg_iteration(Solver& s, hipStream_t st) {
+  Ops<S>::vcycle(s, st, 0);
+  last_dot<S>(s, st, F_ZR, 0, 1);
+  cg_update<S>(s, st);
+}
+
+// The first iteration: the initial preconditioning, p = z and z.r
+// (sim/mgpcg.py:216-228), then the update.
+template <class S>
+void cg_first(Solver& s, hipStream_t st) {
+  Ops<S>::vcycle(s, st, 1);
+  last_dot<S>(s, st, F_ZR_INIT, 1, 0);
+  cg_update<S>(s, st);
+}
+
+// batch 0: the first iteration (cg_first); b >= 1: b iterations
 template <class S>
 int capture_batch(Solver& s, int batch) {
   if (s.gexec[batch]) return EDGE_OK;
   if (!s.cap) EDGE_CHECK_HIP(hipStreamCreateWithFlags(&s.cap, hipStreamNonBlocking));
   hipGraph_t g = nullptr;
   EDGE_CHECK_HIP(hipStreamBeginCapture(s.cap, hipStreamCaptureModeThreadLocal));
+  if (batch == 0) cg_first<S>(s, s.cap);
   for (int b = 0; b < batch; ++b) cg_iteration<S>(s, s.cap);
   hipError_t le = hipGetLastError();
   hipError_t e = hipStreamEndCapture(s.cap, &g);
@@ -2050,7 +2070,7 @@ int capture_batch(Solver& s, int batch) {
 // two kinds of solve, so the first batch is predicted from the solves two and
 // four back: their count when they agree, else one short of the smaller;
 // later batches are single iterations.  A host check costs ~30 us, an
-// iteration run after convergence ~150 us of launches (its kernels return at
+// iteration run after convergence ~100 us of launches (its kernels return at
 // once, but each launch still takes its slot).
 int next_batch(const Solver& s, int check_every, int done_iters) {
   if (check_every >= 1) return std::min(check_every, MAXB);
@@ -2061,28 +2081,33 @@ int next_batch(const Solver& s, int check_every, int done_iters) {
 
 template <class S>
 int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* stats) {
-  Red R = red(s);
-  const Lv& L0 = s.lv[0];
-  // initial preconditioner application and p, z.r (sim/mgpcg.py:216-228)
-  Ops<S>::vcycle(s, st, 1);
-  last_dot<S>(s, st, F_ZR_INIT, 1, 0);
-  EDGE_CHECK_LAUNCH();
   int it = 0;
   int done = 0;
   const int maxit = max_iters < 0 ? (1 << 30) : max_iters;
   while (!done) {
     int batch = std::min(next_batch(s, check_every, it), std::max(1, maxit - it));
-    // a batch of b iterations: the captured graphs of sizes in b's binary expansion
+    // a batch: the first iteration's graph when it = 0, then the captured
+    // graphs of sizes in the binary expansion of the remaining count
     if (s.use_graph) {
+      int rest = batch;
+      if (it == 0) {
+        int rc = capture_batch<S>(s, 0);
+        if (rc) return rc;
+        EDGE_CHECK_HIP(hipGraphLaunch(s.gexec[0], st));
+        --rest;
+      }
       for (int b = MAXB; b >= 1; b >>= 1)
-        if (batch & b) {
+        if (rest & b) {
           int rc = capture_batch<S>(s, b);
           if (rc) return rc;
           EDGE_CHECK_HIP(hipGraphLaunch(s.gexec[b], st));
         }
     } else {
       for (int b = 0; b < batch; ++b) {
-        cg_iteration<S>(s, st);
+        if (it + b == 0)
+          cg_first<S>(s, st);
+        else
+          cg_iteration<S>(s, st);
         EDGE_CHECK_LAUNCH();
       }
     }
