@@ -1265,7 +1265,10 @@ __global__ void __launch_bounds__(TAIL_THREADS) k_vtail(Tail T, int bt, int part
 // their neighbours are exact after s-1 half-sweeps, so after 3 (down) or 4
 // (up) the tile itself is exact.  Each cell update is the row-chunk kernels'
 // arithmetic (same neighbour order, z rounded through S after every update),
-// so a leg gives the values of the per-level launches.  Loads, sweeps and
+// so a leg gives the values of the per-level launches.  A leg reads z and
+// writes another buffer (down: zl -> zt, up: zt -> zl): a tile's halo is its
+// neighbours' interior, which must not change while the launch runs (a
+// launch holds more tiles than fit on the GPU at once).  Loads, sweeps and
 // stores are unrolled over compile-time extents so that each thread keeps
 // several independent accesses in flight (one workgroup's latency is the
 // leg's time).
@@ -1443,7 +1446,8 @@ __device__ __forceinline__ void ht_carve(const HTile& H, unsigned char* base, S*
 // Down leg of level F (>= 1) into coarse level C.
 template <class S, int TK>
 __global__ void __launch_bounds__(HT_THREADS) k_down_tile(Lv F, Lv C, int bt, HTile H, const S* __restrict__ rf,
-                                                         S* __restrict__ zf, S* __restrict__ rc, S* __restrict__ zc,
+                                                         const S* __restrict__ zf, S* __restrict__ zout,
+                                                         S* __restrict__ rc, S* __restrict__ zc,
                                                          const double* __restrict__ sc, int ignore_done) {
   if (!ignore_done && sc[S_DONE] != 0.0) return;
   extern __shared__ __align__(16) unsigned char ht_lds[];
@@ -1457,7 +1461,7 @@ __global__ void __launch_bounds__(HT_THREADS) k_down_tile(Lv F, Lv C, int bt, HT
   ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 1, 1);
   ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 0, 2);
   ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 1, 3);
-  ht_store<S, TK>(F, H, o0, o1, o2, zs, zf);
+  ht_store<S, TK>(F, H, o0, o1, o2, zs, zout);
   // restriction of the tile's residual (its 4 x 4 x TK/2 coarse cells)
   constexpr int HK = TK / 2;
   const int SI = HT_B * H.kp, SJ = H.kp;
@@ -1494,8 +1498,9 @@ __global__ void __launch_bounds__(HT_THREADS) k_down_tile(Lv F, Lv C, int bt, HT
 // Up leg of level F (>= 1) from the coarse level's z.
 template <class S, int TK>
 __global__ void __launch_bounds__(HT_THREADS) k_up_tile(Lv F, Lv C, int bt, HTile H, const S* __restrict__ rf,
-                                                       S* __restrict__ zf, const S* __restrict__ zc,
-                                                       const double* __restrict__ sc, int ignore_done) {
+                                                       const S* __restrict__ zf, S* __restrict__ zout,
+                                                       const S* __restrict__ zc, const double* __restrict__ sc,
+                                                       int ignore_done) {
   if (!ignore_done && sc[S_DONE] != 0.0) return;
   extern __shared__ __align__(16) unsigned char ht_lds[];
   S *zs, *rs, *zcs;
@@ -1531,7 +1536,7 @@ __global__ void __launch_bounds__(HT_THREADS) k_up_tile(Lv F, Lv C, int bt, HTil
   ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 0, 2);
   ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 1, 3);
   ht_sweep<S, TK, false>(F, bt, H, o0, o1, o2, rs, zs, ms, zcs, 0, 4);
-  ht_store<S, TK>(F, H, o0, o1, o2, zs, zf);
+  ht_store<S, TK>(F, H, o0, o1, o2, zs, zout);
 }
 
 // Bottom of the mixed V-cycle: z = B^-1 r over the fluid cells of the coarsest
@@ -1628,6 +1633,7 @@ struct Solver {
   int64_t nl[MAXL];
   void* rl[MAXL];   // level residual (level 0 aliases r in plain mode)
   void* zl[MAXL];
+  void* zt[MAXL];   // levels >= 1: z after the tile down leg (the up leg reads it, writes zl)
   int8_t* bm[MAXL];  // null when no solids
   uint8_t* nbm[MAXL];  // solid bits per cell (k_nbmask), null when no solids
   float *r, *x, *p, *Ap;
@@ -1722,7 +1728,7 @@ constexpr int HT_MAX_LDS = 128 * 1024;
 // Halo-tile plan of level l (between the finest and the coarsest): false when
 // the level's extents do not split into 8 x 8 x tk tiles.
 bool ht_plan(const Solver& s, int l, size_t elem, HTile& H, int& tk_out, int& blocks, int& lds) {
-  if (!s.use_tiles || l < 1 || l >= s.L - 1) return false;
+  if (!s.use_tiles || l < 1 || l >= s.L - 1 || !s.zt[l]) return false;
   const Lv& L = s.lv[l];
   if (L.n0 % HT_T || L.n1 % HT_T) return false;
   // the deepest tile that still gives one workgroup per CU, else the shallowest
@@ -1792,7 +1798,7 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
     if (ht_plan(s, l, sizeof(S), H, tk, hb, hl)) {
       auto kern = tk == 32 ? k_down_tile<S, 32> : (tk == 16 ? k_down_tile<S, 16> : k_down_tile<S, 8>);
       hipLaunchKernelGGL(kern, dim3(hb), dim3(HT_THREADS), hl, st, s.lv[l], s.lv[l + 1], s.bt, H, rl, zl,
-                         (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
+                         (S*)s.zt[l], (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
       continue;
     }
     // remaining pre-smoothing: black, red, black (sim/mgpcg.py:180-182)
@@ -1839,8 +1845,8 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
     int tk = 0, hb = 0, hl = 0;
     if (ht_plan(s, l, sizeof(S), H, tk, hb, hl)) {
       auto kern = tk == 32 ? k_up_tile<S, 32> : (tk == 16 ? k_up_tile<S, 16> : k_up_tile<S, 8>);
-      hipLaunchKernelGGL(kern, dim3(hb), dim3(HT_THREADS), hl, st, s.lv[l], s.lv[l + 1], s.bt, H, rl, zl,
-                         (const S*)s.zl[l + 1], sc, ig);
+      hipLaunchKernelGGL(kern, dim3(hb), dim3(HT_THREADS), hl, st, s.lv[l], s.lv[l + 1], s.bt, H, rl,
+                         (const S*)s.zt[l], zl, (const S*)s.zl[l + 1], sc, ig);
       continue;
     }
     if (chunked(s.lv[l]))
@@ -1857,6 +1863,11 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
 }
 
 
+// Largest coarsest level (fluid cells) for the dense solve: the host Cholesky
+// is O(n^3) (about a second at 2048) and reruns whenever the solid mask of the
+// coarsest level changes.
+constexpr int DENSE_MAX = 2048;
+
 // Build B^-1 for the coarsest level on the host (FP64 Cholesky).  Returns
 // false when B is not positive definite (e.g. a disconnected fluid region).
 bool build_dense(Solver& s, const std::vector<int8_t>& cm) {
@@ -1870,7 +1881,7 @@ bool build_dense(Solver& s, const std::vector<int8_t>& cm) {
       fl.push_back(q);
     }
   int nf = (int)fl.size();
-  if (nf == 0 || nf > 4096) return false;
+  if (nf == 0 || nf > DENSE_MAX) return false;
   std::vector<double> A((size_t)nf * nf, 0.0);
   bool all_neu = (s.bt == 63);
   auto solid = [&](int i, int j, int k) { return s.solid && cm[(i * n1 + j) * n2 + k] > 0; };
@@ -1946,6 +1957,7 @@ void free_solver(Solver* s) {
   for (int l = 0; l < s->L; ++l) {
     if (s->rl[l] && s->rl[l] != (void*)s->r) (void)hipFree(s->rl[l]);
     if (s->zl[l]) (void)hipFree(s->zl[l]);
+    if (s->zt[l]) (void)hipFree(s->zt[l]);
     if (s->bm[l]) (void)hipFree(s->bm[l]);
     if (s->nbm[l]) (void)hipFree(s->nbm[l]);
   }
@@ -2191,6 +2203,7 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
   s->elem = mode == 1 ? 2 : 4;
   std::memset(s->rl, 0, sizeof(s->rl));
   std::memset(s->zl, 0, sizeof(s->zl));
+  std::memset(s->zt, 0, sizeof(s->zt));
   std::memset(s->bm, 0, sizeof(s->bm));
   std::memset(s->nbm, 0, sizeof(s->nbm));
   s->r = s->x = s->p = s->Ap = nullptr;
@@ -2217,6 +2230,8 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
       return fail("hipMalloc level r");
     }
     if (hipMalloc(&s->zl[l], s->nl[l] * s->elem) != hipSuccess) return fail("hipMalloc level z");
+    if (l >= 1 && l < levels - 1 && hipMalloc(&s->zt[l], s->nl[l] * s->elem) != hipSuccess)
+      return fail("hipMalloc level z (tiles)");
   }
   if (hipMalloc(&s->partials, (size_t)(red_blocks(s->N) + 1) * 2 * sizeof(double)) != hipSuccess)
     return fail("hipMalloc partials");

@@ -297,3 +297,33 @@ def test_tile_legs_match_per_level_launches(dev, mode):
     assert res[0][2] == 0 and res[1][2] == 0
     assert res[1][1][0] == res[0][1][0]
     assert scale_err(N_(res[1][0]), N_(res[0][0])) < (1e-5 if mode == 0 else 1e-3)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_tile_legs_with_more_tiles_than_fit_at_once(dev, mode):
+    # 512x256x128, 6 levels: level 1 (256x128x64) runs as 1024 tiles, more than
+    # the GPU holds at once, so tiles of one launch start after others have
+    # finished; their halos must still be the values from before the launch.
+    n = (512, 256, 128)
+    rng = np.random.default_rng(31)
+    r = rng.normal(size=n).astype(np.float32)
+    r -= r.mean()
+    outs = []
+    for tiles in (0, 1, 1):
+        S = Solver(n, 6, mode)
+        S.lib.call("edge_mg_set_tiles", S.h, tiles)
+        outs.append(N_(S.vcycle(T_(r, dev))))
+        del S
+    tol = 1e-6 if mode == 0 else 3e-2
+    assert scale_err(outs[1], outs[0]) < tol
+    assert np.array_equal(outs[1], outs[2])  # the same values on every run
+    u = smooth_velocity(*n, 1.0 / 128, seed=32)
+    its = []
+    for tiles in (0, 1):
+        S = Solver(n, 6, mode)
+        S.lib.call("edge_mg_set_tiles", S.h, tiles)
+        p, st, code = S.poisson([T_(c, dev) for c in u], stop_mode=1, rel_tol=1e-6)
+        assert code == 0
+        its.append(st[0])
+        del S
+    assert its[1] == its[0]
