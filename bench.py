@@ -27,11 +27,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# algorithmic work per unit (minimal separable counts; FMA = 2 flops; bytes =
-# compulsory FP32 reads + writes)
-MARCH_FLOP_PER_CELL = 33600.0      # fused EDGE march: 5 points x (RK4 with Jacobian + Hermite compose)
+# algorithmic work per unit.  Flops: the FP32 arithmetic of the implemented
+# separable formulation (FMA = 2), i.e. 64 lanes x (2 FMA + ADD + MUL) wave
+# instructions per launch from SQ_INSTS_VALU_{FMA,ADD,MUL}_F32, divided by the
+# cells (256x128x128, four rings); the reference's formulation (64 weight
+# products per Hermite evaluation, 27 per B-spline) is 33,600 flop/cell for
+# EDGE.  Bytes: compulsory FP32 reads + writes.
+MARCH_FLOP_PER_CELL = 22250.0      # fused EDGE march: 5 points x (RK4 with Jacobian + Hermite compose)
 # ED8: 8 points x (RK4 of the point + full-Hermite value); GE: 1 point (RK4 with Jacobian + Hermite gradient)
-SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 21400.0, "ge": 6800.0}
+SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 16550.0, "ge": 4430.0}
 SCHEME_BYTES_PER_CELL = {"edge": 180.0, "ed8": 204.0, "ge": 324.0}
 MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0}  # one CG iteration
 MARCH_BYTES_PER_CELL = 180.0       # u 12 + old map 84 + new map 84 (compulsory FP32)
@@ -253,7 +257,7 @@ def main():
         sim.solver = make_solver(sim, "plain")
         run_steps(sim, 1, torch)  # warm the plain solver
         pw, pph = run_steps(sim, a.steps, torch)
-        plain = dict(ms_per_step=round(statistics.mean(pw), 3),
+        plain = dict(ms_per_step=round(statistics.mean(pw), 3), step_ms=[round(w, 3) for w in pw],
                      phases_ms={k: round(statistics.median(p.get(k, 0.0) for p in pph), 3) for k in pph[0]},
                      poisson_iters=[s["iters"] for s in sim.poisson_stats[-2 * a.steps:]])
         sim.solver = mixed_solver
@@ -320,6 +324,7 @@ def main():
                    "solver": "mixed (BF16 multigrid, FP64 dots, dense FP64 coarsest)",
                    **({"particles": sim.laden_particle_num} if sim.lp is not None else {})},
         "phases_ms": {k: round(v, 3) for k, v in ph.items()},
+        "step_ms": [round(w, 3) for w in walls],
         "poisson_iters_mixed": mixed_iters,
         "plain": plain,
         "poisson_same_rhs": pcmp,
