@@ -29,9 +29,13 @@ __device__ __forceinline__ void particle_u(const Grid& g, const Mac& U, const CP
 
 // sim/lp.py:6-29 update_particles_acc: acc = (g e_y - k (v - u(p))) / (1 + k dt), k = 1/St;
 // acc = 0 for q >= num (the reference's acc.fill(0)).
+// order (nullable): a permutation of the first num particles to visit them in
+// (cell order from the last scatter): neighbouring threads then gather nearby
+// velocities.  Each particle's result does not depend on it.
 __global__ void __launch_bounds__(256) k_lp_acc(Grid g, Mac U, CP3 vel, CP3 pos, P3 acc, int64_t n, int64_t num,
-                                               float dt, float St, float gravity) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                               float dt, float St, float gravity, const uint32_t* __restrict__ order) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t q = (order && t < num) ? (int64_t)order[t] : t;
   if (q >= n) return;
   if (q >= num) {
 #pragma unroll
@@ -74,9 +78,11 @@ __global__ void __launch_bounds__(256) k_lp_vel(CP3 acc, P3 vel, CP3 pos, int64_
 
 // sim/lp.py:73-97 calculate_velocity_diff: k (v - u(p)), k = density_ratio / (St r^3).
 __global__ void __launch_bounds__(256) k_lp_vel_diff(Grid g, Mac U, CP3 vel, P3 diff, CP3 pos, int64_t num, float St,
-                                                    float density_ratio, float radius_ratio) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= num) return;
+                                                    float density_ratio, float radius_ratio,
+                                                    const uint32_t* __restrict__ order) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= num) return;
+  const int64_t q = order ? (int64_t)order[t] : t;
   const float er = 1.0f / (radius_ratio * radius_ratio * radius_ratio);
   const float k = 1.0f / St * density_ratio * er;
   float u[3];
@@ -251,11 +257,23 @@ struct P2GWork {
   size_t cap = 0, tiles_cap = 0, temp_bytes = 0;
   uint32_t *k0 = nullptr, *k1 = nullptr, *v0 = nullptr, *v1 = nullptr, *start = nullptr, *end = nullptr;
   void* temp = nullptr;
+  int64_t perm_n = 0;   // v1 holds a permutation of 0..perm_n-1 (the last scatter's cell order)
 };
 std::map<int, P2GWork> g_p2g;
 
+// The last scatter's cell order when it covers exactly m particles (else null):
+// the gathers visit particles in it.  Stream-ordered: a gather sees the order of
+// the last scatter launched before it on the same stream.
+const uint32_t* cell_order(int64_t m) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  auto it = g_p2g.find(dev);
+  return (it != g_p2g.end() && it->second.perm_n == m && m > 0) ? it->second.v1 : nullptr;
+}
+
 int p2g_workspace(P2GWork& w, int64_t num, uint32_t ntiles, int end_bit, hipStream_t st) {
   if ((size_t)num > w.cap) {
+    w.perm_n = 0;
     for (uint32_t* p : {w.k0, w.k1, w.v0, w.v1})
       if (p) (void)hipFree(p);
     w.k0 = w.k1 = w.v0 = w.v1 = nullptr;
@@ -365,7 +383,7 @@ int edge_lp_acc(const float* ux, const float* uy, const float* uz, int nx, int n
   Grid g = make_grid(nx, ny, nz, dx);
   hipLaunchKernelGGL(k_lp_acc, dim3(blocks64(n)), dim3(256), 0, (hipStream_t)stream, g, Mac{ux, uy, uz},
                      CP3{{vel, vel + n, vel + 2 * n}}, CP3{{pos, pos + n, pos + 2 * n}}, P3{{acc, acc + n, acc + 2 * n}},
-                     (int64_t)n, (int64_t)std::min(num, n), dt, St, gravity);
+                     (int64_t)n, (int64_t)std::min(num, n), dt, St, gravity, cell_order(std::min((int64_t)num, (int64_t)n)));
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }
@@ -394,7 +412,7 @@ int edge_lp_vel_diff(const float* ux, const float* uy, const float* uz, int nx, 
   Grid g = make_grid(nx, ny, nz, dx);
   hipLaunchKernelGGL(k_lp_vel_diff, dim3(blocks64(m)), dim3(256), 0, (hipStream_t)stream, g, Mac{ux, uy, uz},
                      CP3{{vel, vel + n, vel + 2 * n}}, P3{{diff, diff + n, diff + 2 * n}},
-                     CP3{{pos, pos + n, pos + 2 * n}}, m, St, density_ratio, radius_ratio);
+                     CP3{{pos, pos + n, pos + 2 * n}}, m, St, density_ratio, radius_ratio, cell_order(m));
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }
@@ -435,6 +453,7 @@ int edge_lp_p2g(float* dragx, float* dragy, float* dragz, const float* diff, con
   EDGE_CHECK_LAUNCH();
   size_t temp = w.temp_bytes;
   EDGE_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(w.temp, temp, w.k0, w.k1, w.v0, w.v1, (int)m, 0, end_bit, st));
+  w.perm_n = m;
   hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(nkeys)), dim3(256), 0, st, (float*)w.start, (int64_t)nkeys);
   hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(nkeys)), dim3(256), 0, st, (float*)w.end, (int64_t)nkeys);
   hipLaunchKernelGGL(k_p2g_ranges, dim3(blocks64(m)), dim3(256), 0, st, w.k1, m, nkeys, w.start, w.end);

@@ -69,8 +69,22 @@ CASES = {
 }
 
 
-def torus_particles(n, centre, R, tube, seed=0):
-    """n points uniform in the solid torus (axis x) around `centre`: (3, n) float32."""
+def cell_order(pos, dx, res, tile=8):
+    """Permutation putting particles (3, n) in 8x8x8-cell-tile-major cell order
+    (the order the drag scatter groups them in): particles that share cells
+    are stored together, so the per-particle gathers and the scatter read
+    contiguous memory.  Particle results do not depend on the order."""
+    res = np.asarray(res)
+    c = np.clip(np.floor(pos.astype(np.float64) / dx), 0, (res - 1)[:, None]).astype(np.int64)
+    t = c // tile
+    nt = (res + tile - 1) // tile
+    key = ((t[0] * nt[1] + t[1]) * nt[2] + t[2]) * tile ** 3 + ((c[0] % tile) * tile + c[1] % tile) * tile + c[2] % tile
+    return np.argsort(key, kind="stable")
+
+
+def torus_particles(n, centre, R, tube, seed=0, dx=None, res=None):
+    """n points uniform in the solid torus (axis x) around `centre`: (3, n) float32,
+    in cell order (cell_order) when dx and res are given."""
     rng = np.random.default_rng(seed)
     out = np.empty((3, 0), np.float32)
     c = np.asarray(centre, np.float64)
@@ -81,7 +95,10 @@ def torus_particles(n, centre, R, tube, seed=0):
         rho = np.sqrt(p[1] ** 2 + p[2] ** 2)
         keep = p[0] ** 2 + (rho - R) ** 2 < tube ** 2
         out = np.concatenate([out, (p[:, keep] + c[:, None]).astype(np.float32)], axis=1)
-    return np.ascontiguousarray(out[:, :n])
+    out = out[:, :n]
+    if dx is not None:
+        out = out[:, cell_order(out, dx, res)]
+    return np.ascontiguousarray(out)
 
 
 class Simulation:
@@ -215,7 +232,8 @@ class Simulation:
             ring = dict(w=1.0, rad=R, delta=tube, c=tuple(c), ex=(0.0, 0.0, -1.0), ey=(0.0, 1.0, 0.0))
             init_rings(self.X, self.u, self.smoke, self.tmp_smoke, [ring], num_samples=self.num_samples)
             n = self.lp["max_num"]
-            pos = torus_particles(n, c, R, tube, seed=self.seed)
+            pos = torus_particles(n, c, R, tube, seed=self.seed, dx=self.dx,
+                                  res=(self.res_x, self.res_y, self.res_z))
             self.laden_particles_pos.from_numpy(pos.T)
             self.laden_particles_vel.fill(0.0)
             self.laden_particle_ref_radius.fill(self.lp["laden_radius"])

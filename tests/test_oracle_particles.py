@@ -114,3 +114,25 @@ def test_viscous_force_quadratic_and_edge_clamp(orc64):
         np.testing.assert_allclose(v2[1:-1], 0.0, atol=1e-9)
         np.testing.assert_allclose(v2[0], 2.0 / DX ** 2 / Re, rtol=1e-10)
         np.testing.assert_allclose(v2[-1], -2.0 / DX ** 2 / Re, rtol=1e-10)
+
+
+def test_cell_order_is_a_tile_major_permutation():
+    """run_edge.cell_order: a permutation that sorts particles by the scatter's
+    key (8x8x8 tiles, then cells inside a tile, k fastest)."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "edge-flow-maps_amd"))
+    from run_edge import cell_order, torus_particles
+    rng = np.random.default_rng(4)
+    res, dx = (40, 24, 16), 1.0 / 16
+    pos = np.stack([rng.uniform(0, r * dx, 5000) for r in res]).astype(np.float32)
+    order = cell_order(pos, dx, res)
+    assert np.array_equal(np.sort(order), np.arange(5000))
+    c = np.floor(pos[:, order] / dx).astype(np.int64)
+    t = c // 8
+    key = ((t[0] * 3 + t[1]) * 2 + t[2]) * 512 + ((c[0] % 8) * 8 + c[1] % 8) * 8 + c[2] % 8
+    assert np.all(np.diff(key) >= 0)
+    # the seeded torus is the same point set, reordered
+    a = torus_particles(3000, (1.0, 0.7, 0.5), 0.25, 0.08, seed=1)
+    b = torus_particles(3000, (1.0, 0.7, 0.5), 0.25, 0.08, seed=1, dx=dx, res=res)
+    assert np.array_equal(np.sort(a, axis=1), np.sort(b, axis=1))
