@@ -288,7 +288,7 @@ def main():
         it_ms = (ph.get("poisson1", 0.0) + ph.get("poisson2", 0.0)) / max(sum(mixed_iters[-2:]), 1)
     mg_gbs = MG_BYTES_PER_CELL["mixed"] * ncell / (it_ms * 1e-3) / 1e9
     roof_mg = dict(bound="hbm", achieved=round(mg_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                   frac=round(mg_gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="mixed CG iteration (~30 launches)",
+                   frac=round(mg_gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="mixed CG iteration (22 launches)",
                    time_ms=round(it_ms, 4))
 
     if a.copy_probe:

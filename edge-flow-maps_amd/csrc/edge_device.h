@@ -27,6 +27,9 @@
 #ifndef EDGE_WPE_MARCH
 #define EDGE_WPE_MARCH 0
 #endif
+#ifndef EDGE_WPE_ED8
+#define EDGE_WPE_ED8 0
+#endif
 #ifndef EDGE_WPE_FACES
 #define EDGE_WPE_FACES 0
 #endif
@@ -34,6 +37,11 @@
 #define EDGE_MARCH_ATTR __attribute__((amdgpu_waves_per_eu(EDGE_WPE_MARCH, 8)))
 #else
 #define EDGE_MARCH_ATTR
+#endif
+#if EDGE_WPE_ED8
+#define EDGE_ED8_ATTR __attribute__((amdgpu_waves_per_eu(EDGE_WPE_ED8, 8)))
+#else
+#define EDGE_ED8_ATTR
 #endif
 #if EDGE_WPE_FACES
 #define EDGE_FACES_ATTR __attribute__((amdgpu_waves_per_eu(EDGE_WPE_FACES, 8)))

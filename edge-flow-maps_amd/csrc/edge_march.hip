@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) EDGE_MARCH_ATTR k_march_edge(Grid g, Mac 
 // (bit 0 = x sign, bit 1 = y, bit 2 = z; sim/hermite.py:704-725), each
 // advance_map_0 (-dt) then interp_psi_hermite through the old map (full
 // Hermite), then calc_ed8_3 (sim/hermite.py:1098-1151).
-__global__ void __launch_bounds__(256) k_march_ed8(Grid g, Mac U, const float4* __restrict__ tab,
+__global__ void __launch_bounds__(256) EDGE_ED8_ATTR k_march_ed8(Grid g, Mac U, const float4* __restrict__ tab,
                                                    const float* __restrict__ X, MapOut o, float dt, float eps) {
   const int64_t N = ncell(g);
   const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
