@@ -882,59 +882,59 @@ __global__ void __launch_bounds__(BS) k_prolong8(Lv F, Lv C, int bt, const S* __
 }
 
 // Restriction + fill(0) + first red half-sweep of the coarse level (as
-// k_restrict): one thread per 4 coarse cells along k (8 fine cells of 4 fine rows).
+// k_restrict).  Four lanes per 4 coarse cells along k: lane (a, b) computes
+// the residual of the 8 fine cells of fine row (2I + a, 2J + b); the block's
+// residuals meet in LDS and one thread per coarse cell adds its 8 children in
+// the order of k_restrict.
 template <class S>
 __global__ void __launch_bounds__(BS) k_restrict8(Lv F, Lv C, int bt, const S* __restrict__ rf, const S* __restrict__ zf,
                                                  S* __restrict__ rc, S* __restrict__ zc, const double* __restrict__ sc,
                                                  int ignore_done) {
   if (!ignore_done && sc[S_DONE] != 0.0) return;
-  // chunks of the fine level's rows, but only rows with even i and j
+  __shared__ float res[BS / 4][4][8];
   const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t_ >= (uint32_t)((int64_t)C.n0 * C.n1 * (C.n2 >> 2))) return;
   const uint32_t per_row = (uint32_t)(C.n2 >> 2);
-  const uint32_t row = t_ / per_row;
-  const int K0 = 4 * (int)(t_ - row * per_row);
-  const int J = (int)(row % (uint32_t)C.n1), I = (int)(row / (uint32_t)C.n1);
-  const int k0 = 2 * K0;
-  float res[2][2][8];
-  uint64_t M[2][2];
+  const uint32_t nchunk = (uint32_t)C.n0 * (uint32_t)C.n1 * per_row;
+  const uint32_t u = t_ >> 2;           // coarse chunk: 4 cells along k
+  const int ab = (int)(t_ & 3u);        // fine row (a, b) here, coarse cell K0 + ab below
+  const bool live = u < nchunk;
+  const int lc = threadIdx.x >> 2;
+  int I = 0, J = 0, K0 = 0;
+  if (live) {
+    const uint32_t row = u / per_row;
+    K0 = 4 * (int)(u - row * per_row);
+    J = (int)(row % (uint32_t)C.n1);
+    I = (int)(row / (uint32_t)C.n1);
+    const int a = ab >> 1, b = ab & 1;
+    const int i = 2 * I + a, j = 2 * J + b, k0 = 2 * K0;
+    const int64_t q0 = ((int64_t)i * F.n1 + j) * F.n2 + k0;
+    const uint64_t M = nb8(F, q0);
+    Nb8 N;
+    load_nb(F, zf, i, j, k0, q0, N);
+    float rv[8];
+    ld8(rf + q0, rv);
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int i = 2 * I + a, j = 2 * J + b;
-      const int64_t q0 = ((int64_t)i * F.n1 + j) * F.n2 + k0;
-      M[a][b] = nb8(F, q0);
-      Nb8 N;
-      load_nb(F, zf, i, j, k0, q0, N);
-      float rv[8];
-      ld8(rf + q0, rv);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const uint32_t mb = nbc(M[a][b], c);
-        res[a][b][c] = 0.f;
-        if (!(mb & NB_SELF)) res[a][b][c] = rv[c] - (diag8(F, bt, i, j, k0 + c, mb) * N.c[c] - nsum8(F, N, i, j, k0, c, mb));
-      }
+    for (int c = 0; c < 8; ++c) {
+      const uint32_t mb = nbc(M, c);
+      // a solid fine cell adds +0 (k_restrict skips it: the same sum)
+      res[lc][ab][c] = (mb & NB_SELF) ? 0.f
+                                      : (rv[c] - (diag8(F, bt, i, j, k0 + c, mb) * N.c[c] - nsum8(F, N, i, j, k0, c, mb))) *
+                                            1.0f / 2.0f;
     }
-  const int64_t qc0 = ((int64_t)I * C.n1 + J) * C.n2 + K0;
-  const uint32_t MC = C.nb ? *(const uint32_t*)(C.nb + qc0) : 0u;
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int K = K0 + m;
-    const int64_t q = qc0 + m;
-    float acc = 0.f;
-#pragma unroll
-    for (int c8 = 0; c8 < 8; ++c8) {
-      const int a = c8 >> 2, b = (c8 >> 1) & 1, c = c8 & 1;
-      if (nbc(M[a][b], 2 * m + c) & NB_SELF) continue;
-      acc += res[a][b][2 * m + c] * 1.0f / 2.0f;
-    }
-    st(rc, q, acc);
-    const uint32_t mb = (MC >> (8 * m)) & 0xffu;
-    float zv = 0.f;
-    if (((I + J + K) & 1) == 0 && !(mb & NB_SELF)) zv = through<S>(acc) / diag8(C, bt, I, J, K, mb);
-    st(zc, q, zv);
   }
+  __syncthreads();
+  if (!live) return;
+  const int m = ab;
+  float acc = 0.f;
+#pragma unroll
+  for (int c8 = 0; c8 < 8; ++c8) acc += res[lc][c8 >> 1][2 * m + (c8 & 1)];
+  const int K = K0 + m;
+  const int64_t q = ((int64_t)I * C.n1 + J) * C.n2 + K;
+  st(rc, q, acc);
+  const uint32_t mbc = C.nb ? (uint32_t)C.nb[q] : 0u;
+  float zv = 0.f;
+  if (((I + J + K) & 1) == 0 && !(mbc & NB_SELF)) zv = through<S>(acc) / diag8(C, bt, I, J, K, mbc);
+  st(zc, q, zv);
 }
 
 // p = z + beta p on the fly, Ap, p.Ap (as k_p_Ap).
@@ -1806,7 +1806,7 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
     smooth<S>(s, st, l, rl, zl, 0, ig);
     smooth<S>(s, st, l, rl, zl, 1, ig);
     if (chunked(s.lv[l]))
-      hipLaunchKernelGGL(k_restrict8<S>, dim3(blocks_for(s.nl[l + 1] / 4, BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1],
+      hipLaunchKernelGGL(k_restrict8<S>, dim3(blocks_for(s.nl[l + 1], BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1],
                          s.bt, rl, zl, (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
     else
       hipLaunchKernelGGL(k_restrict<S>, dim3(blocks_for(s.nl[l + 1], BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt,
