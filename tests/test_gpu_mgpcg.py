@@ -327,3 +327,30 @@ def test_tile_legs_with_more_tiles_than_fit_at_once(dev, mode):
         its.append(st[0])
         del S
     assert its[1] == its[0]
+
+
+def test_dense_coarsest_solve_is_the_fp64_inverse(dev):
+    # two levels, mixed, 24^3 (coarsest 12^3 = 1728 cells, solved densely):
+    # the V-cycle must equal the reference V-cycle with its bottom solved
+    # exactly, i.e. the numpy FP64 V-cycle with the 20 bottom pairs replaced by
+    # 3000 (converged; 6000 gives the same digits).  Compared without the
+    # constant (all-wall Neumann null space: the dense solve returns the
+    # mean-free solution, Gauss-Seidel from zero another one, and the constant
+    # passes through prolongation and smoothing unchanged).  The reference's
+    # own 20 pairs are 42% away from the exact bottom here, BF16 storage of
+    # the result alone 0.17%, so 2% separates the two.
+    from edge_hip import _lib
+    from oracle import np_ref
+    n = (24, 24, 24)
+    rng = np.random.default_rng(41)
+    r = rng.normal(size=n)
+    r -= r.mean()
+    S = Solver(n, 2, 1)
+    assert _lib.lib().edge_mg_dense_bottom(S.h) == 1
+    z = N_(S.vcycle(T_(r, dev))).astype(np.float64)
+    want = np_ref.vcycle(r, 2, bottom=3000)
+    z20 = np_ref.vcycle(r, 2, bottom=20)
+    mf = lambda x: x - x.mean()
+    nrm = np.linalg.norm(mf(want))
+    assert np.linalg.norm(mf(z20) - mf(want)) > 0.2 * nrm
+    assert np.linalg.norm(mf(z) - mf(want)) <= 2e-2 * nrm
