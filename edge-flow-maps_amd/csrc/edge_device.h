@@ -393,6 +393,10 @@ EDGE_DEV float hermite_eval(const Grid& g, Rsrc tab, const HPoint& P, float gr[3
 // 2 more operations per corner (16 per corner instead of 36 for the full
 // gradient).  Coefficient c[tx][ty][tz] (t = 0: F basis, 1: G basis) of a
 // corner is the table row of hermite_eval.
+// A corner whose weights on axis A1 or A2 are both exactly zero adds exact
+// zeros and is skipped (neither read nor multiplied): at a face centre
+// (fraction 0 on the two axes across the face) only the 2 corners along D
+// remain, and the value is that of the 8-corner loop.
 template <int D>
 EDGE_DEV float hermite_eval_dir(const Grid& g, Rsrc tab, const HPoint& P, float& dval) {
   constexpr int A1 = D == 2 ? 0 : 2;   // contracted first
@@ -400,6 +404,8 @@ EDGE_DEV float hermite_eval_dir(const Grid& g, Rsrc tab, const HPoint& P, float&
   const uint32_t s1 = (uint32_t)g.nz, s0 = (uint32_t)g.ny * s1;
   const uint32_t vb = 32u * ((uint32_t)P.X.i0 * s0 + (uint32_t)P.Y.i0 * s1 + (uint32_t)P.Z.i0);
   const HAxis* ax[3] = {&P.X, &P.Y, &P.Z};
+  const bool z1 = ax[A1]->F[1] == 0.f && ax[A1]->G[1] == 0.f;  // corner 1 of A1 weighs nothing
+  const bool z2 = ax[A2]->F[1] == 0.f && ax[A2]->G[1] == 0.f;
   float val = 0.f, dv = 0.f;
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -408,6 +414,8 @@ EDGE_DEV float hermite_eval_dir(const Grid& g, Rsrc tab, const HPoint& P, float&
       const uint32_t so = 32u * (i * s0 + j * s1);
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
+        const int cr[3] = {i, j, k};
+        if ((cr[A1] == 1 && z1) || (cr[A2] == 1 && z2)) continue;
         const float4 lo = bload4(tab, vb + 32u * k, so), hi = bload4(tab, vb + 32u * k + 16u, so);
         float c[2][2][2];
         c[0][0][0] = lo.x;
@@ -418,7 +426,6 @@ EDGE_DEV float hermite_eval_dir(const Grid& g, Rsrc tab, const HPoint& P, float&
         c[1][0][1] = hi.y;
         c[1][1][0] = hi.z;
         c[1][1][1] = hi.w;
-        const int cr[3] = {i, j, k};
         auto coef = [&](int t1, int t2, int td) {
           int t[3];
           t[A1] = t1;

@@ -59,26 +59,51 @@ __global__ void __launch_bounds__(256) k_map_u_psi(Grid g, Mac U0, Faces U1, con
   U1.f[a][l] = Gc[0] * v[0] + Gc[1] * v[1] + Gc[2] * v[2];
 }
 
-// run_ed8.py:190-223 split_psi_hermite: phi_a = psi(X_a), F_a = column a of grad psi.
-__global__ void __launch_bounds__(256) k_split_psi(Grid g, const float4* __restrict__ tab, Faces PHI, Faces F) {
-  int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  int a, i, j, k;
-  int64_t l, nf;
-  if (!face_of(g, t, a, l, nf, i, j, k)) return;
+// Face (i, j, k) of face grid D: phi = psi(X_D), F = column D of grad psi.
+template <int D>
+__device__ __forceinline__ void split_face(const Grid& g, const float4* __restrict__ tab, int i, int j, int k,
+                                           const Faces& PHI, const Faces& F) {
   float X[3], psi[3], Gc[3];
-  face_coord(a, i, j, k, g.dx, X);
-  // only column a of grad psi is kept
-  if (a == 0)
-    map_eval_dir<0, true>(g, tab, X, psi, Gc);
-  else if (a == 1)
-    map_eval_dir<1, true>(g, tab, X, psi, Gc);
-  else
-    map_eval_dir<2, true>(g, tab, X, psi, Gc);
+  face_coord(D, i, j, k, g.dx, X);
+  map_eval_dir<D, true>(g, tab, X, psi, Gc);
+  const int64_t nf = g.nf[D];
+  const int64_t d1 = g.ny + (D == 1), d2 = g.nz + (D == 2);
+  const int64_t l = ((int64_t)i * d1 + j) * d2 + k;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    PHI.f[a][c * nf + l] = psi[c];
-    F.f[a][c * nf + l] = Gc[c];
+    PHI.f[D][c * nf + l] = psi[c];
+    F.f[D][c * nf + l] = Gc[c];
   }
+}
+
+// run_ed8.py:190-223 split_psi_hermite: phi_a = psi(X_a), F_a = column a of grad psi.
+// Thread t < N takes cell t's three low faces (x at i, y at j, z at k), so one
+// pass reads each table row once from memory and its neighbours from cache;
+// the threads after N take the three top planes (x at nx, y at ny, z at nz).
+__global__ void __launch_bounds__(256) k_split_psi(Grid g, const float4* __restrict__ tab, Faces PHI, Faces F) {
+  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t N = ncell(g);
+  if (t < N) {
+    int i, j, k;
+    cell_ijk(g, t, i, j, k);
+    split_face<0>(g, tab, i, j, k, PHI, F);
+    split_face<1>(g, tab, i, j, k, PHI, F);
+    split_face<2>(g, tab, i, j, k, PHI, F);
+    return;
+  }
+  int64_t r = t - N;
+  const int64_t px = (int64_t)g.ny * g.nz, py = (int64_t)g.nx * g.nz, pz = (int64_t)g.nx * g.ny;
+  if (r < px) {
+    split_face<0>(g, tab, g.nx, (int)(r / g.nz), (int)(r % g.nz), PHI, F);
+    return;
+  }
+  r -= px;
+  if (r < py) {
+    split_face<1>(g, tab, (int)(r / g.nz), g.ny, (int)(r % g.nz), PHI, F);
+    return;
+  }
+  r -= py;
+  if (r < pz) split_face<2>(g, tab, (int)(r / g.ny), (int)(r % g.ny), g.nz, PHI, F);
 }
 
 // util/taichi_utils.py:171-227: trilinear value or 8-tap min/max of a
@@ -400,8 +425,8 @@ int edge_split_psi_hermite(const float* tab, float* phix, float* phiy, float* ph
   EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_split_psi_hermite: bad dims");
   EDGE_REQUIRE(tab && phix && phiy && phiz && Fx && Fy && Fz, "edge_split_psi_hermite: null pointer");
   Grid g = make_grid(nx, ny, nz, dx);
-  int64_t nf = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
-  hipLaunchKernelGGL(k_split_psi, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g,
+  const int64_t nt = ncell(g) + (int64_t)ny * nz + (int64_t)nx * nz + (int64_t)nx * ny;
+  hipLaunchKernelGGL(k_split_psi, dim3(blocks_for(nt, 256)), dim3(256), 0, (hipStream_t)stream, g,
                      (const float4*)tab, Faces{{phix, phiy, phiz}}, Faces{{Fx, Fy, Fz}});
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
