@@ -385,6 +385,49 @@ __global__ void __launch_bounds__(256) k_split_central(Grid g, const float* __re
   U.f[a][l] = 0.5f * (r + lft);
 }
 
+// util/taichi_utils.py:95-100 get_central_vector: vc_a = mean of the cell's two
+// a-faces.  vc: 3 planes of N cells.
+__global__ void __launch_bounds__(256) k_central_vector(Grid g, Mac U, float* __restrict__ vc) {
+  const int64_t N = ncell(g);
+  const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  int i, j, k;
+  cell_ijk(g, q, i, j, k);
+  const int64_t qx = q;  // x-face (i, j, k); (i + 1, j, k) is one (ny nz) plane further
+  const int64_t qy = ((int64_t)i * (g.ny + 1) + j) * g.nz + k;
+  const int64_t qz = ((int64_t)i * g.ny + j) * (g.nz + 1) + k;
+  vc[q] = 0.5f * (U.ux[qx + (int64_t)g.ny * g.nz] + U.ux[qx]);
+  vc[N + q] = 0.5f * (U.uy[qy + g.nz] + U.uy[qy]);
+  vc[2 * N + q] = 0.5f * (U.uz[qz + 1] + U.uz[qz]);
+}
+
+// util/taichi_utils.py:70-92 curl: central differences of a cell-centred vector
+// field with sample()'s clamp of the neighbour index to the grid.
+__global__ void __launch_bounds__(256) k_curl(Grid g, const float* __restrict__ vf, float* __restrict__ cf,
+                                              float inv_dist) {
+  const int64_t N = ncell(g);
+  const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  int i, j, k;
+  cell_ijk(g, q, i, j, k);
+  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+  const int64_t r = i + 1 < g.nx ? q + sx : q, l = i > 0 ? q - sx : q;   // vr, vl
+  const int64_t t = j + 1 < g.ny ? q + sy : q, b = j > 0 ? q - sy : q;   // vt, vb
+  const int64_t c = k + 1 < g.nz ? q + 1 : q, a = k > 0 ? q - 1 : q;     // vc, va
+  const float* vx = vf;
+  const float* vy = vf + N;
+  const float* vz = vf + 2 * N;
+  const float d_vx_dz = inv_dist * (vx[c] - vx[a]);
+  const float d_vx_dy = inv_dist * (vx[t] - vx[b]);
+  const float d_vy_dx = inv_dist * (vy[r] - vy[l]);
+  const float d_vy_dz = inv_dist * (vy[c] - vy[a]);
+  const float d_vz_dx = inv_dist * (vz[r] - vz[l]);
+  const float d_vz_dy = inv_dist * (vz[t] - vz[b]);
+  cf[q] = d_vz_dy - d_vy_dz;
+  cf[N + q] = d_vx_dz - d_vz_dx;
+  cf[2 * N + q] = d_vy_dx - d_vx_dy;
+}
+
 }  // namespace
 
 extern "C" {
@@ -563,6 +606,32 @@ int edge_vortex_ring(float w, float rad, float delta, const float* c, const floa
                      delta, make_float3(c[0], c[1], c[2]), make_float3(ex[0], ex[1], ex[2]),
                      make_float3(ey[0], ey[1], ey[2]), pf, vf, smokef, make_float3(color[0], color[1], color[2]),
                      num_samples);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_get_central_vector(const float* ux, const float* uy, const float* uz, float* vc, int nx, int ny, int nz,
+                            void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_get_central_vector: bad dims");
+  EDGE_REQUIRE(ux && uy && uz && vc, "edge_get_central_vector: null pointer");
+  Grid g = make_grid(nx, ny, nz, 1.0f);
+  hipLaunchKernelGGL(k_central_vector, dim3(blocks_for(ncell(g), 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     Mac{ux, uy, uz}, vc);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_curl(const float* vf, float* cf, int nx, int ny, int nz, float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_curl: bad dims");
+  EDGE_REQUIRE(vf && cf && vf != cf, "edge_curl: null or aliased pointers");
+  EDGE_REQUIRE(dx > 0.f, "edge_curl: dx must be positive");
+  Grid g = make_grid(nx, ny, nz, dx);
+  // 1/(2 dx) in f32, as the kernel computes it from its f32 argument
+  const float inv_dist = 1.0f / (2.0f * dx);
+  hipLaunchKernelGGL(k_curl, dim3(blocks_for(ncell(g), 256)), dim3(256), 0, (hipStream_t)stream, g, vf, cf,
+                     inv_dist);
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }

@@ -52,6 +52,8 @@ SIGNATURES = {
     "edge_fill": "pflp",
     "edge_vortex_ring": "fffpppppppiiiip",
     "edge_split_central_vector": "ppppiiip",
+    "edge_get_central_vector": "ppppiiip",
+    "edge_curl": "ppiiifp",
     # projection
     "edge_mg_create": "iiiiipp",
     "edge_mg_destroy": "p",

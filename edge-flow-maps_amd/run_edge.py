@@ -24,6 +24,7 @@ import sys
 import time
 
 import numpy as np
+import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 if _HERE not in sys.path:
@@ -40,8 +41,9 @@ from sim.init_conditions import (init_delta, init_four_vorts, init_rings, init_v
                                  seeded_rings, set_boundary_mask_delta)
 from sim.mgpcg import MGPCG_3  # noqa: E402
 from sim.mgpcg_solid import MGPCG_Solid_3  # noqa: E402
-from util.taichi_utils import (add_fields, center_coords_func, scale_field, split_central_vector,  # noqa: E402
-                               x_coords_func, y_coords_func, z_coords_func)
+from util import io_utils  # noqa: E402
+from util.taichi_utils import (add_fields, center_coords_func, curl, get_central_vector, scale_field,  # noqa: E402
+                               split_central_vector, x_coords_func, y_coords_func, z_coords_func)
 
 CASES = {
     # reference case 0 (run_edge.py:25-42)
@@ -156,6 +158,7 @@ class Simulation:
         self.F_x, self.F_y, self.F_z = V(3, float, fx), V(3, float, fy), V(3, float, fz)
         self.phi_x, self.phi_y, self.phi_z = V(3, float, fx), V(3, float, fy), V(3, float, fz)
         self.u = V(3, float, cell)
+        self.w = V(3, float, cell)   # vorticity of the frame output (run_edge.py:719-723)
         self.u_x, self.u_y, self.u_z = F(float, fx), F(float, fy), F(float, fz)
         self.init_u_x, self.init_u_y, self.init_u_z = F(float, fx), F(float, fy), F(float, fz)
         self.err_u_x, self.err_u_y, self.err_u_z = F(float, fx), F(float, fy), F(float, fz)
@@ -194,6 +197,8 @@ class Simulation:
         self.seed = seed
         self.i = -1
         self.sub_t = 0.0
+        self.frame_idx = 0
+        self.output_frame = False
         self.timing = None   # dict name -> list of (start, end) torch.cuda.Event pairs, when enabled
         self.rhs_snapshots = None  # list: velocities handed to each projection, when enabled
 
@@ -270,11 +275,15 @@ class Simulation:
         if self.sub_t + curr_dt >= self.visualize_dt:
             curr_dt = self.visualize_dt - self.sub_t
             self.sub_t = 0.
+            self.frame_idx += 1
+            self.output_frame = True
         elif self.sub_t + curr_dt * 2 >= self.visualize_dt:
             curr_dt = (self.visualize_dt - self.sub_t) / 2
             self.sub_t += curr_dt
+            self.output_frame = False
         else:
             self.sub_t += curr_dt
+            self.output_frame = False
         self.curr_dt = curr_dt
 
         if j == 0:
@@ -417,6 +426,115 @@ class Simulation:
 
     def velocity_numpy(self):
         return [self.u_x.to_numpy(), self.u_y.to_numpy(), self.u_z.to_numpy()]
+
+    # -- frame output and checkpoints (run_edge.py:557-581, 719-737) ----------------
+    def vorticity_norm(self):
+        """|curl u| at the cells (run_edge.py:720-723): face means, curl on the GPU, norm on the host."""
+        get_central_vector(self.u_x, self.u_y, self.u_z, self.u)
+        curl(self.u, self.w, self.dx)
+        return np.linalg.norm(self.w.to_numpy(), axis=-1)
+
+    def write_frame(self, vtkdir, particle_vtkdir=None, frame=None):
+        """The visualization block of run_edge.py:719-727: field_<frame>.vti with
+        "vorticity" and the 4-channel "smoke", and the particles' field_<frame>.vtu."""
+        frame = self.frame_idx if frame is None else frame
+        io_utils.write_vtks_4channel_smoke(self.vorticity_norm(), self.smoke.to_numpy(), vtkdir, frame)
+        if self.lp is not None and particle_vtkdir is not None:
+            pos = self.laden_particles_pos.to_numpy()[:self.laden_particle_num, :]
+            io_utils.write_to_vtks_lp(pos, {}, {}, os.path.join(particle_vtkdir, "field_{:03d}".format(frame)))
+
+    def _state_fields(self):
+        """Fields a step reads before it writes them, besides u and smoke: the two maps,
+        the fields at the last reinitialisation and the particles' state."""
+        names = ["init_u_x", "init_u_y", "init_u_z", "init_smoke", "phi_x", "phi_y", "phi_z", "F_x", "F_y", "F_z",
+                 "psi_garm", "T_x_garm", "T_y_garm", "T_z_garm", "dT_x_garm", "dT_y_garm", "dT_z_garm"]
+        if self.scheme != "edge":
+            names += ["d2T_x", "d2T_y", "d2T_z"]
+        if self.lp is not None:
+            names += ["laden_particles_pos", "laden_particles_vel", "laden_particles_acc",
+                      "laden_particles_vel_diff", "laden_particle_ref_radius", "vis_x", "vis_y", "vis_z"]
+        return names
+
+    def save_checkpoint(self, ckptdir, frame=None, state=True):
+        """vel_{x,y,z}_numpy_<frame>.npy (run_edge.py:576-581, 731-737), smoke_numpy_<frame>.npy,
+        and with state=True edge_state_<frame>.npz: the maps, the fields at the last
+        reinitialisation, the particles and the step counters, so that a run resumes
+        mid-reinit and continues bit for bit.  Returns the files written."""
+        frame = self.frame_idx if frame is None else frame
+        os.makedirs(ckptdir, exist_ok=True)
+        paths = io_utils.velocity_paths(ckptdir, frame)
+        for p, f in zip(paths, (self.u_x, self.u_y, self.u_z)):
+            np.save(p, f.to_numpy())
+        np.save(io_utils.smoke_path(ckptdir, frame), self.smoke.to_numpy())
+        paths.append(io_utils.smoke_path(ckptdir, frame))
+        if state:
+            sp = io_utils.state_path(ckptdir, frame)
+            arrays = {k: getattr(self, k).data.detach().cpu().numpy() for k in self._state_fields()}
+            io_utils.save_state(sp, arrays, dict(i=self.i, sub_t=float(self.sub_t), frame_idx=self.frame_idx,
+                                                 reinit_every=self.reinit_every))
+            paths.append(sp)
+        return paths
+
+    def load_checkpoint(self, ckptdir, frame):
+        """Resume from frame `frame` after initialize() (run_edge.py:557-566).  Velocity from
+        the .npy files; smoke when its file exists (the reference loads it, but never writes
+        it); with edge_state_<frame>.npz the whole map state and the step counters, else,
+        as the reference, the next step reinitialises the maps.  Returns True when the
+        map state was restored."""
+        for p, f in zip(io_utils.velocity_paths(ckptdir, frame), (self.u_x, self.u_y, self.u_z)):
+            f.from_numpy(np.load(p, allow_pickle=False))
+        sm = io_utils.smoke_path(ckptdir, frame)
+        if os.path.exists(sm):
+            self.smoke.from_numpy(np.load(sm, allow_pickle=False))
+        self.frame_idx = int(frame)
+        self.sub_t = 0.0
+        self.i = -1
+        sp = io_utils.state_path(ckptdir, frame)
+        if not os.path.exists(sp):
+            return False
+        arrays, counters = io_utils.load_state(sp)
+        if counters.get("reinit_every", self.reinit_every) != self.reinit_every:
+            raise ValueError(f"{sp}: saved with reinit_every {counters['reinit_every']}, "
+                             f"this run uses {self.reinit_every}")
+        names = self._state_fields()
+        missing = [k for k in names if k not in arrays]
+        if missing:
+            raise ValueError(f"{sp}: missing fields {missing}")
+        for k in names:
+            f = getattr(self, k)
+            a = arrays[k]
+            if tuple(a.shape) != tuple(f.data.shape):
+                raise ValueError(f"{sp}: field {k} has shape {a.shape}, expected {tuple(f.data.shape)}")
+            f.data.copy_(torch.from_numpy(a).to(f.data.dtype))
+            f.touch()
+        self.i = int(counters["i"])
+        self.sub_t = float(counters["sub_t"])
+        self.frame_idx = int(counters["frame_idx"])
+        return True
+
+    def run(self, total_frames, logsdir, from_frame=0, ckpt_every=1, state=True, log=print):
+        """The frame loop of run_edge.py:522-745: a first frame (written at from_frame), then
+        steps until frame total_frames, writing vtks/field_<frame>.vti (and particle/ files)
+        on every output frame and ckpts/ every ckpt_every frames."""
+        vtkdir, ckptdir = os.path.join(logsdir, "vtks"), os.path.join(logsdir, "ckpts")
+        pdir = os.path.join(logsdir, "particle") if self.lp is not None else None
+        for d in (vtkdir, ckptdir) + ((pdir,) if pdir else ()):
+            os.makedirs(d, exist_ok=True)
+        self.initialize()
+        if from_frame > 0:
+            self.load_checkpoint(ckptdir, from_frame)
+        self.write_frame(vtkdir, pdir, self.frame_idx)
+        self.save_checkpoint(ckptdir, self.frame_idx, state=state)
+        while self.frame_idx < total_frames:
+            t0 = time.perf_counter()
+            dt = self.substep()
+            log(f"[Simulate] Done step {self.i} dt {dt:.5g} poisson iters {self.poisson_stats[-1]['iters']}")
+            if self.output_frame:
+                self.write_frame(vtkdir, pdir)
+                if self.frame_idx % ckpt_every == 0:
+                    self.save_checkpoint(ckptdir, state=state)
+                ti.sync()
+                log(f"[Simulate] Done Frame {self.frame_idx} time {time.perf_counter() - t0:.2f}s")
 
 
 def main(argv=None):

@@ -60,3 +60,21 @@ def split_central_vector(vc, vx, vy, vz):
     _lib.call("edge_split_central_vector", as_ptr(vc), as_ptr(vx), as_ptr(vy), as_ptr(vz), nx, ny, nz,
               stream_handle())
     touch(vx, vy, vz)
+
+
+def get_central_vector(vx, vy, vz, vc):
+    """util/taichi_utils.py:95-100: MAC faces -> cell-centred vector (face means)."""
+    nx, ny, nz = vc.shape
+    assert vx.shape == (nx + 1, ny, nz) and vy.shape == (nx, ny + 1, nz) and vz.shape == (nx, ny, nz + 1)
+    _lib.call("edge_get_central_vector", as_ptr(vx), as_ptr(vy), as_ptr(vz), as_ptr(vc), nx, ny, nz,
+              stream_handle())
+    touch(vc)
+
+
+def curl(vf, cf, dx):
+    """util/taichi_utils.py:70-92: cf = curl of the cell-centred vector field vf
+    (central differences, neighbour index clamped to the grid as sample() does)."""
+    nx, ny, nz = cf.shape
+    assert vf.shape == cf.shape and vf.n == cf.n == 3
+    _lib.call("edge_curl", as_ptr(vf), as_ptr(cf), nx, ny, nz, float(dx), stream_handle())
+    touch(cf)

@@ -142,6 +142,11 @@ int edge_fill(float* p, float v, long n, void* stream);
 int edge_vortex_ring(float w, float rad, float delta, const float* c, const float* ex, const float* ey,
                      const float* pf, float* vf, float* smokef, const float* color, int num_samples, int nx, int ny,
                      int nz, void* stream);
+/* get_central_vector (util/taichi_utils.py:95-100): vc = 3 planes of nx*ny*nz cells. */
+int edge_get_central_vector(const float* ux, const float* uy, const float* uz, float* vc, int nx, int ny, int nz,
+                            void* stream);
+/* curl (util/taichi_utils.py:70-92): vf, cf = 3 planes of nx*ny*nz cells, must differ. */
+int edge_curl(const float* vf, float* cf, int nx, int ny, int nz, float dx, void* stream);
 /* split_central_vector (util/taichi_utils.py:103-116). */
 int edge_split_central_vector(const float* vc, float* ux, float* uy, float* uz, int nx, int ny, int nz,
                               void* stream);

@@ -372,3 +372,62 @@ void particle_u(const Mac<R>& U, const R* pos, long n, long q, R dx, R out[3]) {
     }                                                                                               \
   }
 ORC_BOTH(DEF_LP)
+
+// Frame output (run_edge.py:719-725): cell-centred velocity and its curl.
+namespace {
+
+// util/taichi_utils.py:95-100 get_central_vector: each component the mean of
+// the cell's two faces on its axis.
+template <class R>
+void central_vector(const Mac<R>& U, const Fld<R>& V) {
+#pragma omp parallel for collapse(2)
+  for (int i = 0; i < V.n0; ++i)
+    for (int j = 0; j < V.n1; ++j)
+      for (int k = 0; k < V.n2; ++k) {
+        V.at(0, i, j, k) = R(0.5) * (U.ux.at(0, i + 1, j, k) + U.ux.at(0, i, j, k));
+        V.at(1, i, j, k) = R(0.5) * (U.uy.at(0, i, j + 1, k) + U.uy.at(0, i, j, k));
+        V.at(2, i, j, k) = R(0.5) * (U.uz.at(0, i, j, k + 1) + U.uz.at(0, i, j, k));
+      }
+}
+
+// util/taichi_utils.py:70-92 curl: inv_dist = 1/(2 dx) in R; the six
+// neighbours through sample()'s index clamp.
+template <class R>
+void curl(const Fld<R>& vf, const Fld<R>& cf, R dx) {
+  const R inv_dist = R(1) / (R(2) * dx);
+#pragma omp parallel for collapse(2)
+  for (int i = 0; i < cf.n0; ++i)
+    for (int j = 0; j < cf.n1; ++j)
+      for (int k = 0; k < cf.n2; ++k) {
+        R vr[3], vl[3], vt[3], vb[3], vc[3], va[3];
+        for (int c = 0; c < 3; ++c) {
+          vr[c] = sample(vf, c, i + 1, j, k);
+          vl[c] = sample(vf, c, i - 1, j, k);
+          vt[c] = sample(vf, c, i, j + 1, k);
+          vb[c] = sample(vf, c, i, j - 1, k);
+          vc[c] = sample(vf, c, i, j, k + 1);
+          va[c] = sample(vf, c, i, j, k - 1);
+        }
+        const R d_vx_dz = inv_dist * (vc[0] - va[0]);
+        const R d_vx_dy = inv_dist * (vt[0] - vb[0]);
+        const R d_vy_dx = inv_dist * (vr[1] - vl[1]);
+        const R d_vy_dz = inv_dist * (vc[1] - va[1]);
+        const R d_vz_dx = inv_dist * (vr[2] - vl[2]);
+        const R d_vz_dy = inv_dist * (vt[2] - vb[2]);
+        cf.at(0, i, j, k) = d_vz_dy - d_vy_dz;
+        cf.at(1, i, j, k) = d_vx_dz - d_vz_dx;
+        cf.at(2, i, j, k) = d_vy_dx - d_vx_dy;
+      }
+}
+
+}  // namespace
+
+#define DEF_OUT(R, S)                                                                              \
+  extern "C" void orc_central_vector_##S(const R* ux, const R* uy, const R* uz, R* vc, int nx,      \
+                                         int ny, int nz) {                                          \
+    central_vector(mkmac(ux, uy, uz, nx, ny, nz), mk((const R*)vc, nx, ny, nz));                   \
+  }                                                                                                 \
+  extern "C" void orc_curl_##S(const R* vf, R* cf, int nx, int ny, int nz, R dx) {                 \
+    curl(mk(vf, nx, ny, nz), mk((const R*)cf, nx, ny, nz), dx);                                     \
+  }
+ORC_BOTH(DEF_OUT)

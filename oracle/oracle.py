@@ -268,6 +268,21 @@ class Oracle:
                                self.r(dx), self.r(dt))
         return out
 
+    def central_vector(self, u):
+        """util/taichi_utils.py:95-100: (3, nx, ny, nz) face means of the MAC velocity."""
+        u = self.mac(u)
+        nx, ny, nz = self.cell_dims(u)
+        vc = np.zeros((3, nx, ny, nz), self.dt)
+        self.fn("central_vector")(*[_p(c) for c in u], _p(vc), nx, ny, nz)
+        return vc
+
+    def curl(self, vf, dx):
+        """util/taichi_utils.py:70-92: curl of a (3, nx, ny, nz) cell-centred field."""
+        vf = self.a(vf)
+        cf = np.zeros_like(vf)
+        self.fn("curl")(_p(vf), _p(cf), *vf.shape[1:], self.r(dx))
+        return cf
+
     def max_speed(self, u):
         u = self.mac(u)
         return float(self.fn("max_speed")(*[_p(c) for c in u], *self.cell_dims(u)))
