@@ -94,6 +94,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--poisson-reps", type=int, default=3, help="0 skips the same-RHS solver comparison")
+    ap.add_argument("--no-io", action="store_true", help="skip timing one frame output and one checkpoint")
     ap.add_argument("--case", default="cfg2", choices=["cfg2", "cfg3", "cfg5"],
                     help="cfg2: the headline EDGE step; cfg3: delta-wing plate with the solid projection; "
                          "cfg5: 512x256x256 with 16M laden particles")
@@ -191,6 +192,43 @@ def poisson_compare(sim, reps, torch):
         res["speedup_learned_vs_plain"] = round(res["plain"]["ms"] / res["learned"]["ms"], 3)
         res["learned_note"] = "mixed solver started from the U-Net's p0 (inference included in ms)"
     return res
+
+
+def io_timing(sim, torch):
+    """One frame output (vorticity on the GPU, .vti) and one checkpoint save and load
+    (velocity and smoke .npy, map state .npz) into a temporary directory, after the timed
+    steps.  Host-side file work: reported beside the step, never part of it."""
+    import shutil
+    import tempfile
+    from edge_hip import _lib
+    from edge_hip.field import stream_handle
+    d = tempfile.mkdtemp(prefix="edge_io_")
+    try:
+        wn, sm, _, _ = sim._frame_buffers()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()   # the frame kernel alone (|curl u| and smoke in VTK order), on torch's stream
+        _lib.call("edge_vtk_frame", sim.u_x.ptr, sim.u_y.ptr, sim.u_z.ptr, sim.smoke.ptr, 4, wn.data_ptr(),
+                  sm.data_ptr(), sim.res_x, sim.res_y, sim.res_z, float(sim.dx), stream_handle())
+        e1.record()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.write_frame(d, frame=0)
+        t1 = time.perf_counter()
+        files = sim.save_checkpoint(d, frame=0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        restored = sim.load_checkpoint(d, 0)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        fb = os.path.getsize(os.path.join(d, "field_0000.vti"))
+        cb = sum(os.path.getsize(f) for f in files)
+        return dict(frame_kernel_ms=round(e0.elapsed_time(e1), 3), frame_ms=round((t1 - t0) * 1e3, 1),
+                    frame_bytes=fb, checkpoint_save_ms=round((t2 - t1) * 1e3, 1), checkpoint_bytes=cb,
+                    checkpoint_load_ms=round((t3 - t2) * 1e3, 1), state_restored=bool(restored),
+                    where="a temporary directory on the host's local disk")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def cpu_baseline(torch):
@@ -302,6 +340,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    io_rec = None if a.no_io else io_timing(sim, torch)
     cpu = None if a.no_cpu else cpu_baseline(torch)
 
     line = {
@@ -330,6 +369,7 @@ def main():
         "poisson_same_rhs": pcmp,
         "roofline": roof_march,
         "roofline_cg_iteration": roof_mg,
+        "io": io_rec,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -428,6 +428,56 @@ __global__ void __launch_bounds__(256) k_curl(Grid g, const float* __restrict__ 
   cf[2 * N + q] = d_vy_dx - d_vx_dy;
 }
 
+// The device part of a frame output (run_edge.py:720-725 with
+// util/io_utils.py:53-75): |curl u| of the cell-centred velocity and the smoke,
+// both in VTK point order (x fastest), the smoke's nc channels interleaved per
+// point.  Thread t is VTK point t.  The central vector and the curl are
+// k_central_vector's and k_curl's expressions; the norm is numpy's float32
+// norm over the last axis, sqrt((x*x + y*y) + z*z), without contraction.
+__global__ void __launch_bounds__(256) k_vtk_frame(Grid g, Mac U, const float* __restrict__ smoke, int nc,
+                                                   float* __restrict__ wn, float* __restrict__ sm, float inv_dist) {
+  const int64_t N = ncell(g);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N) return;
+  const uint32_t t32 = (uint32_t)t;
+  const uint32_t rest = t32 / (uint32_t)g.nx;
+  const int i = (int)(t32 - rest * (uint32_t)g.nx);
+  const int k = (int)(rest / (uint32_t)g.ny);
+  const int j = (int)(rest - (uint32_t)k * (uint32_t)g.ny);
+  const int64_t sx = (int64_t)g.ny * g.nz;
+  // component a of the central vector at cell (ii, jj, kk), clamped like sample()
+  auto vcomp = [&](int a, int ii, int jj, int kk) {
+    ii = max(0, min(ii, g.nx - 1));
+    jj = max(0, min(jj, g.ny - 1));
+    kk = max(0, min(kk, g.nz - 1));
+    if (a == 0) {
+      const int64_t q = ((int64_t)ii * g.ny + jj) * g.nz + kk;
+      return 0.5f * (U.ux[q + sx] + U.ux[q]);
+    }
+    if (a == 1) {
+      const int64_t q = ((int64_t)ii * (g.ny + 1) + jj) * g.nz + kk;
+      return 0.5f * (U.uy[q + g.nz] + U.uy[q]);
+    }
+    const int64_t q = ((int64_t)ii * g.ny + jj) * (g.nz + 1) + kk;
+    return 0.5f * (U.uz[q + 1] + U.uz[q]);
+  };
+  const float d_vx_dz = inv_dist * (vcomp(0, i, j, k + 1) - vcomp(0, i, j, k - 1));
+  const float d_vx_dy = inv_dist * (vcomp(0, i, j + 1, k) - vcomp(0, i, j - 1, k));
+  const float d_vy_dx = inv_dist * (vcomp(1, i + 1, j, k) - vcomp(1, i - 1, j, k));
+  const float d_vy_dz = inv_dist * (vcomp(1, i, j, k + 1) - vcomp(1, i, j, k - 1));
+  const float d_vz_dx = inv_dist * (vcomp(2, i + 1, j, k) - vcomp(2, i - 1, j, k));
+  const float d_vz_dy = inv_dist * (vcomp(2, i, j + 1, k) - vcomp(2, i, j - 1, k));
+  const float wx = d_vz_dy - d_vy_dz, wy = d_vx_dz - d_vz_dx, wz = d_vy_dx - d_vx_dy;
+  float w2;
+  {
+#pragma clang fp contract(off)
+    w2 = (wx * wx + wy * wy) + wz * wz;
+  }
+  wn[t] = sqrtf(w2);   // correctly rounded (hipcc's default for f32 sqrt)
+  const int64_t q = ((int64_t)i * g.ny + j) * g.nz + k;
+  for (int c = 0; c < nc; ++c) sm[t * nc + c] = smoke[c * N + q];
+}
+
 }  // namespace
 
 extern "C" {
@@ -632,6 +682,20 @@ int edge_curl(const float* vf, float* cf, int nx, int ny, int nz, float dx, void
   const float inv_dist = 1.0f / (2.0f * dx);
   hipLaunchKernelGGL(k_curl, dim3(blocks_for(ncell(g), 256)), dim3(256), 0, (hipStream_t)stream, g, vf, cf,
                      inv_dist);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_vtk_frame(const float* ux, const float* uy, const float* uz, const float* smoke, int nc, float* wn,
+                   float* sm, int nx, int ny, int nz, float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_vtk_frame: bad dims");
+  EDGE_REQUIRE(ux && uy && uz && wn && (nc == 0 || (smoke && sm)), "edge_vtk_frame: null pointer");
+  EDGE_REQUIRE(nc >= 0 && nc <= 4, "edge_vtk_frame: nc must be in 0..4 (got %d)", nc);
+  EDGE_REQUIRE(dx > 0.f, "edge_vtk_frame: dx must be positive");
+  Grid g = make_grid(nx, ny, nz, dx);
+  hipLaunchKernelGGL(k_vtk_frame, dim3(blocks_for(ncell(g), 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     Mac{ux, uy, uz}, smoke, nc, wn, sm, 1.0f / (2.0f * dx));
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }

@@ -54,6 +54,7 @@ SIGNATURES = {
     "edge_split_central_vector": "ppppiiip",
     "edge_get_central_vector": "ppppiiip",
     "edge_curl": "ppiiifp",
+    "edge_vtk_frame": "ppppippiiifp",
     # projection
     "edge_mg_create": "iiiiipp",
     "edge_mg_destroy": "p",

@@ -30,7 +30,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 if _HERE not in sys.path:
     sys.path.insert(0, _HERE)
 
-from edge_hip.field import Field, ti  # noqa: E402
+from edge_hip import _lib  # noqa: E402
+from edge_hip.field import Field, stream_handle, ti  # noqa: E402
 from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smoke_psi, clamp_u3,  # noqa: E402
                              map_smoke, map_smoke_psi, map_u, map_u_psi, march_phi_grid, reset_to_identity)
 from sim import lp as lp_kernels  # noqa: E402
@@ -434,11 +435,29 @@ class Simulation:
         curl(self.u, self.w, self.dx)
         return np.linalg.norm(self.w.to_numpy(), axis=-1)
 
+    def _frame_buffers(self):
+        if getattr(self, "_fbuf", None) is None:
+            n = self.res_x * self.res_y * self.res_z
+            dev = self.u_x.data.device
+            self._fbuf = (torch.empty(n, device=dev), torch.empty(4 * n, device=dev),
+                          torch.empty(n, pin_memory=True), torch.empty(4 * n, pin_memory=True))
+        return self._fbuf
+
     def write_frame(self, vtkdir, particle_vtkdir=None, frame=None):
         """The visualization block of run_edge.py:719-727: field_<frame>.vti with
-        "vorticity" and the 4-channel "smoke", and the particles' field_<frame>.vtu."""
+        "vorticity" (|curl u|) and the 4-channel "smoke", and the particles'
+        field_<frame>.vtu.  The norm and the VTK point order are made on the GPU
+        (one kernel), then one copy per array to pinned host memory."""
         frame = self.frame_idx if frame is None else frame
-        io_utils.write_vtks_4channel_smoke(self.vorticity_norm(), self.smoke.to_numpy(), vtkdir, frame)
+        wn, sm, hw, hs = self._frame_buffers()
+        _lib.call("edge_vtk_frame", self.u_x.ptr, self.u_y.ptr, self.u_z.ptr, self.smoke.ptr, 4, wn.data_ptr(),
+                  sm.data_ptr(), self.res_x, self.res_y, self.res_z, float(self.dx), stream_handle())
+        hw.copy_(wn, non_blocking=True)
+        hs.copy_(sm, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        io_utils.write_vti_points(os.path.join(vtkdir, "field_{:04d}.vti".format(frame)),
+                                  (self.res_x, self.res_y, self.res_z),
+                                  [("vorticity", hw.numpy()), ("smoke", hs.numpy().reshape(-1, 4))], "vorticity")
         if self.lp is not None and particle_vtkdir is not None:
             pos = self.laden_particles_pos.to_numpy()[:self.laden_particle_num, :]
             io_utils.write_to_vtks_lp(pos, {}, {}, os.path.join(particle_vtkdir, "field_{:03d}".format(frame)))

@@ -70,6 +70,12 @@ def _data_array(name, arr, ncomp, app):
     return f"        <DataArray type=\"{t}\" Name=\"{name}\"{comp} format=\"appended\" offset=\"{off}\"/>\n"
 
 
+def write_vti_points(path, dims, arrays, scalars):
+    """An ImageData of dims (origin 0, spacing 1) with point arrays already in VTK
+    point order: arrays = [(name, (n_points,) or (n_points, ncomp) array)]."""
+    _write_image(path, dims, arrays, scalars)
+
+
 def _write_image(path, dims, arrays, scalars):
     """arrays: list of (name, (n_points, ncomp) or (n_points,) array in x-fastest order)."""
     ext = f"0 {dims[0] - 1} 0 {dims[1] - 1} 0 {dims[2] - 1}"

@@ -147,6 +147,12 @@ int edge_get_central_vector(const float* ux, const float* uy, const float* uz, f
                             void* stream);
 /* curl (util/taichi_utils.py:70-92): vf, cf = 3 planes of nx*ny*nz cells, must differ. */
 int edge_curl(const float* vf, float* cf, int nx, int ny, int nz, float dx, void* stream);
+/* No counterpart in the reference (it runs get_central_vector, curl, numpy's norm and
+ * vtk's writer, run_edge.py:720-725, util/io_utils.py:53-75): the device part of a frame
+ * output.  wn = |curl u| (nx*ny*nz floats) and sm = the nc smoke planes interleaved per
+ * point (nx*ny*nz*nc floats), both in VTK point order (x fastest); nc in 0..4. */
+int edge_vtk_frame(const float* ux, const float* uy, const float* uz, const float* smoke, int nc, float* wn,
+                   float* sm, int nx, int ny, int nz, float dx, void* stream);
 /* split_central_vector (util/taichi_utils.py:103-116). */
 int edge_split_central_vector(const float* vc, float* ux, float* uy, float* uz, int nx, int ny, int nz,
                               void* stream);

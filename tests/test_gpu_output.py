@@ -56,6 +56,36 @@ def test_frame_vti_holds_vorticity_and_smoke(orc32, dev, tmp_path):
     assert d["_dims"] == (64, 32, 32)
     assert scale_err(d["vorticity"], np.linalg.norm(w, axis=0)) < 1e-5
     assert np.array_equal(d["smoke"], sim.smoke.to_numpy())
+    # the fused frame kernel against the reference's sequence on the GPU
+    # (get_central_vector, curl, numpy's float32 norm): the same values up to
+    # the contraction of the curl's products, i.e. within a few ulp
+    ref = sim.vorticity_norm()
+    assert ref.dtype == np.float32
+    assert np.max(np.abs(d["vorticity"] - ref) / np.maximum(ref, 1e-30)) < 5e-7
+
+
+def test_vtk_frame_one_channel_and_no_smoke(dev):
+    from edge_hip import _lib
+    from edge_hip.field import stream_handle
+    rng = np.random.default_rng(8)
+    n, dx = (12, 10, 8), 0.1
+    u = [torch.from_numpy(rng.normal(size=s).astype(np.float32)).to(dev)
+         for s in ((n[0] + 1, n[1], n[2]), (n[0], n[1] + 1, n[2]), (n[0], n[1], n[2] + 1))]
+    sm0 = torch.from_numpy(rng.random(n).astype(np.float32)).to(dev)
+    wn = torch.empty(int(np.prod(n)), device=dev)
+    sm = torch.empty(int(np.prod(n)), device=dev)
+    _lib.call("edge_vtk_frame", *[c.data_ptr() for c in u], sm0.data_ptr(), 1, wn.data_ptr(), sm.data_ptr(), *n,
+              float(dx), stream_handle())
+    got_w = wn.cpu().numpy().reshape(n, order="F")
+    assert np.array_equal(sm.cpu().numpy().reshape(n, order="F"), sm0.cpu().numpy())
+    wn2 = torch.zeros_like(wn)
+    _lib.call("edge_vtk_frame", *[c.data_ptr() for c in u], None, 0, wn2.data_ptr(), None, *n, float(dx),
+              stream_handle())
+    assert torch.equal(wn, wn2)
+    from oracle.oracle import Oracle
+    o = Oracle(np.float64)
+    w = o.curl(o.central_vector([c.cpu().numpy().astype(np.float64) for c in u]), dx)
+    assert scale_err(got_w, np.linalg.norm(w, axis=0)) < 1e-6
 
 
 @pytest.mark.parametrize("case", ["cfg1", "cfg5"])
