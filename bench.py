@@ -36,6 +36,8 @@ FP32_PEAK_TFLOPS = 157.3
 MARCH_FLOP_PER_CELL = 22250.0      # fused EDGE march: 5 points x (RK4 with Jacobian + Hermite compose)
 # ED8: 8 points x (RK4 of the point + full-Hermite value); GE: 1 point (RK4 with Jacobian + Hermite gradient)
 SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 16550.0, "ge": 4430.0}
+# the face march (RK4 of a face point and its column, k_advance_faces): EFM's dominant kernel
+FACE_MARCH_FLOP_PER_FACE, FACE_MARCH_BYTES_PER_FACE = 2860.0, 52.0
 SCHEME_BYTES_PER_CELL = {"edge": 180.0, "ed8": 204.0, "ge": 324.0}
 MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0}  # one CG iteration
 MARCH_BYTES_PER_CELL = 180.0       # u 12 + old map 84 + new map 84 (compulsory FP32)
@@ -99,7 +101,7 @@ def parse():
                     help="cfg2: the headline EDGE step; cfg3: delta-wing plate with the solid projection; "
                          "cfg5: 512x256x256 with 16M laden particles")
     ap.add_argument("--particles", type=int, default=None, help="particle count for --case cfg5")
-    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge"],
+    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge", "efm"],
                     help="backward-map march: EDGE (run_edge.py), ED8 (run_ed8.py) or GE (run_ge.py)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, one plain copy kernel (k_scale, alpha 1) over MIB MiB: the known-size "
@@ -282,7 +284,8 @@ def main():
     torch.cuda.synchronize()
     total_ms = (time.perf_counter() - t0) * 1e3
     timing.enable(False)
-    march_kernel = {"edge": "k_march_edge", "ed8": "k_march_ed8", "ge": "k_march_ge"}[a.scheme]
+    march_kernel = {"edge": "k_march_edge", "ed8": "k_march_ed8", "ge": "k_march_ge",
+                    "efm": "k_advance_faces"}[a.scheme]
     march_launch_ms = timing.durations(march_kernel)
     ms_step = total_ms / a.steps
     ph = {k: statistics.median(p.get(k, 0.0) for p in phases) for k in phases[0]}
@@ -307,8 +310,13 @@ def main():
     # the dominant kernel: the fused EDGE march, one launch per step, timed per launch with
     # HIP events on its stream; FP32 vector-ALU bound (no matrix op on this path)
     march_ms = statistics.mean(march_launch_ms)
-    flop_cell = SCHEME_FLOP_PER_CELL[a.scheme]
-    march_tflops = flop_cell * ncell / (march_ms * 1e-3) / 1e12
+    if a.scheme == "efm":
+        nface = 3 * ncell + sim.res_y * sim.res_z + sim.res_x * sim.res_z + sim.res_x * sim.res_y
+        flop_launch, bytes_launch = FACE_MARCH_FLOP_PER_FACE * nface, FACE_MARCH_BYTES_PER_FACE * nface
+    else:
+        flop_launch = SCHEME_FLOP_PER_CELL[a.scheme] * ncell
+        bytes_launch = SCHEME_BYTES_PER_CELL[a.scheme] * ncell
+    march_tflops = flop_launch / (march_ms * 1e-3) / 1e12
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and a.scheme == "edge":
         rec = json.load(open(TRAFFIC_FILE)).get("k_march_edge", {})
@@ -317,7 +325,7 @@ def main():
     roof_march = dict(bound="mfma", pipe="FP32 vector ALU; 157.3 TFLOP/s is MI355X's FP32 peak",
                       achieved=round(march_tflops, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                       frac=round(march_tflops / FP32_PEAK_TFLOPS, 4), traffic=traffic, kernel=march_kernel,
-                      flop_per_launch=flop_cell * ncell, algorithmic_bytes_per_launch=SCHEME_BYTES_PER_CELL[a.scheme] * ncell,
+                      flop_per_launch=flop_launch, algorithmic_bytes_per_launch=bytes_launch,
                       avg_launch_ms=round(march_ms, 4), launches=len(march_launch_ms))
     # the projection's CG iteration (many kernels), HBM-bound: bytes per iteration / ms per iteration
     if pcmp is not None:

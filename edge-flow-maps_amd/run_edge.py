@@ -109,10 +109,13 @@ class Simulation:
 
     def __init__(self, case="cfg1", solver_mode="plain", stop="relative", rel_tol=1e-6, res=None,
                  reinit_every=None, check_every=None, num_samples=2000, seed=0, num_particles=None, scheme="edge"):
-        """scheme: "edge" (run_edge.py), "ed8" (run_ed8.py) or "ge" (run_ge.py): the backward
-        map's march; ED8 and GE keep a full Hermite map and pull back through its face values."""
-        if scheme not in ("edge", "ed8", "ge"):
-            raise ValueError(f"scheme must be edge, ed8 or ge, got {scheme!r}")
+        """scheme: "edge" (run_edge.py), "ed8" (run_ed8.py), "ge" (run_ge.py) or "efm"
+        (run_efm.py): the backward map's march; ED8 and GE keep a full Hermite map and pull
+        back through its face values; EFM keeps no cell map: it caches the velocity of every
+        step since the last reinitialisation and re-marches the face map back through all
+        of them each step."""
+        if scheme not in ("edge", "ed8", "ge", "efm"):
+            raise ValueError(f"scheme must be edge, ed8, ge or efm, got {scheme!r}")
         self.scheme = scheme
         cfg = dict(CASES[case])
         if res is not None:
@@ -171,11 +174,17 @@ class Simulation:
         self.psi_garm = V(3, float, cell)
         self.T_x_garm, self.T_y_garm, self.T_z_garm = V(3, float, cell), V(3, float, cell), V(3, float, cell)
         self.dT_x_garm, self.dT_y_garm, self.dT_z_garm = V(3, float, cell), V(3, float, cell), V(3, float, cell)
-        if scheme != "edge":
+        if scheme in ("ed8", "ge"):
             # run_ed8.py / run_ge.py: the map also holds d2T; psi and T at the faces (split_psi_hermite)
             self.d2T_x, self.d2T_y, self.d2T_z = F(float, cell), F(float, cell), F(float, cell)
+        if scheme != "edge":
             self.psi_x, self.psi_y, self.psi_z = V(3, float, fx), V(3, float, fy), V(3, float, fz)
             self.T_x, self.T_y, self.T_z = V(3, float, fx), V(3, float, fy), V(3, float, fz)
+        if scheme == "efm":
+            # run_efm.py:104-109: the velocity after each step's first projection since the
+            # last reinitialisation, and that step's dt
+            self.u_cache = [(F(float, fx), F(float, fy), F(float, fz)) for _ in range(self.reinit_every)]
+            self.dts = [0.0] * self.reinit_every
         # scratch of the reference's fused march (accepted, unused by the in-place kernel)
         self.psi_tmp = self.T_x_tmp = self.T_y_tmp = self.T_z_tmp = None
         self.dT_x_tmp = self.dT_y_tmp = self.dT_z_tmp = None
@@ -292,7 +301,7 @@ class Simulation:
             if self.scheme == "edge":
                 reset_to_identity_2(self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm,
                                     self.dT_y_garm, self.dT_z_garm, self.X)
-            else:
+            elif self.scheme != "efm":
                 reset_to_identity_3(self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm,
                                     self.dT_y_garm, self.dT_z_garm, self.d2T_x, self.d2T_y, self.d2T_z, self.X)
             self.init_u_x.copy_from(self.u_x)
@@ -320,6 +329,8 @@ class Simulation:
                                               self.T_y_garm, self.T_z_garm, self.dT_x_garm, self.dT_y_garm,
                                               self.dT_z_garm, self.u_x, self.u_y, self.u_z, self.X, dx, curr_dt,
                                               self.ed_eps)
+        elif self.scheme == "efm":
+            self.backtrack_psi_grid(j, curr_dt)
         else:
             self.march_full_map(curr_dt)
         self._mark("phi_march")
@@ -386,6 +397,21 @@ class Simulation:
             self.poisson()
         self._mark("end")
         return curr_dt
+
+    def backtrack_psi_grid(self, j, curr_dt):
+        """run_efm.py:170-185 after caching u (run_efm.py:355-357): the face map psi_*, T_*
+        from the identity, marched back (-dt) through the current velocity and then through
+        the cached velocities of steps j-1 .. 0 (the reference first copies each cached
+        field into tmp_u; the march reads the cache directly)."""
+        dx = self.dx
+        self.dts[j] = curr_dt
+        for c, f in zip(self.u_cache[j], (self.u_x, self.u_y, self.u_z)):
+            c.copy_from(f)
+        faces = (self.psi_x, self.psi_y, self.psi_z, self.T_x, self.T_y, self.T_z)
+        reset_to_identity(*faces, dx)
+        march_phi_grid(*faces, self.u_x, self.u_y, self.u_z, dx, -curr_dt)
+        for step in reversed(range(j)):
+            march_phi_grid(*faces, *self.u_cache[step], dx, -self.dts[step])
 
     def march_full_map(self, curr_dt):
         """run_ed8.py:226-230 / run_ge.py:220-231: the ED8 or GE march of the full
@@ -465,14 +491,23 @@ class Simulation:
     def _state_fields(self):
         """Fields a step reads before it writes them, besides u and smoke: the two maps,
         the fields at the last reinitialisation and the particles' state."""
-        names = ["init_u_x", "init_u_y", "init_u_z", "init_smoke", "phi_x", "phi_y", "phi_z", "F_x", "F_y", "F_z",
-                 "psi_garm", "T_x_garm", "T_y_garm", "T_z_garm", "dT_x_garm", "dT_y_garm", "dT_z_garm"]
-        if self.scheme != "edge":
+        names = ["init_u_x", "init_u_y", "init_u_z", "init_smoke", "phi_x", "phi_y", "phi_z", "F_x", "F_y", "F_z"]
+        if self.scheme != "efm":
+            names += ["psi_garm", "T_x_garm", "T_y_garm", "T_z_garm", "dT_x_garm", "dT_y_garm", "dT_z_garm"]
+        if self.scheme in ("ed8", "ge"):
             names += ["d2T_x", "d2T_y", "d2T_z"]
+        if self.scheme == "efm":
+            names += [f"u_{a}_cache_{s}" for s in range(self.reinit_every) for a in "xyz"]
         if self.lp is not None:
             names += ["laden_particles_pos", "laden_particles_vel", "laden_particles_acc",
                       "laden_particles_vel_diff", "laden_particle_ref_radius", "vis_x", "vis_y", "vis_z"]
         return names
+
+    def _field(self, name):
+        if name.startswith("u_") and "_cache_" in name:   # u_{x,y,z}_cache_<step> (EFM)
+            a, step = name[2], int(name.rsplit("_", 1)[1])
+            return self.u_cache[step]["xyz".index(a)]
+        return getattr(self, name)
 
     def save_checkpoint(self, ckptdir, frame=None, state=True):
         """vel_{x,y,z}_numpy_<frame>.npy (run_edge.py:576-581, 731-737), smoke_numpy_<frame>.npy,
@@ -488,7 +523,9 @@ class Simulation:
         paths.append(io_utils.smoke_path(ckptdir, frame))
         if state:
             sp = io_utils.state_path(ckptdir, frame)
-            arrays = {k: getattr(self, k).data.detach().cpu().numpy() for k in self._state_fields()}
+            arrays = {k: self._field(k).data.detach().cpu().numpy() for k in self._state_fields()}
+            if self.scheme == "efm":
+                arrays["dts"] = np.asarray(self.dts, np.float64)
             io_utils.save_state(sp, arrays, dict(i=self.i, sub_t=float(self.sub_t), frame_idx=self.frame_idx,
                                                  reinit_every=self.reinit_every))
             paths.append(sp)
@@ -520,12 +557,14 @@ class Simulation:
         if missing:
             raise ValueError(f"{sp}: missing fields {missing}")
         for k in names:
-            f = getattr(self, k)
+            f = self._field(k)
             a = arrays[k]
             if tuple(a.shape) != tuple(f.data.shape):
                 raise ValueError(f"{sp}: field {k} has shape {a.shape}, expected {tuple(f.data.shape)}")
             f.data.copy_(torch.from_numpy(a).to(f.data.dtype))
             f.touch()
+        if self.scheme == "efm":
+            self.dts = [float(v) for v in arrays["dts"]]
         self.i = int(counters["i"])
         self.sub_t = float(counters["sub_t"])
         self.frame_idx = int(counters["frame_idx"])
@@ -563,7 +602,7 @@ def main(argv=None):
     ap.add_argument("--solver", default="plain", choices=["plain", "mixed"])
     ap.add_argument("--res", type=int, nargs=3, default=None)
     ap.add_argument("--particles", type=int, default=None, help="particle count of a particle case")
-    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge"])
+    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge", "efm"])
     a = ap.parse_args(argv)
     case = int(a.case) if a.case.isdigit() else a.case
     sim = Simulation(case, solver_mode=a.solver, res=a.res, num_particles=a.particles, scheme=a.scheme)

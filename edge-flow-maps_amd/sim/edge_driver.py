@@ -9,6 +9,7 @@ import torch
 
 from edge_hip import _lib
 from edge_hip.field import as_ptr, stream_handle, touch
+from edge_hip import timing
 from edge_hip.tables import hermite_table, table_version
 
 
@@ -55,9 +56,10 @@ def reset_to_identity(psi_x, psi_y, psi_z, T_x, T_y, T_z, dx):
 def march_phi_grid(phi_x, phi_y, phi_z, F_x, F_y, F_z, u_x, u_y, u_z, dx, dt):
     """run_edge.py:330-357: advance_map on the three face grids (+dt), one launch."""
     nx, ny, nz = _cells_of_mac(u_x)
-    _lib.call("edge_advance_map_faces", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), as_ptr(phi_x), as_ptr(phi_y),
-              as_ptr(phi_z), as_ptr(F_x), as_ptr(F_y), as_ptr(F_z), nx, ny, nz, float(dx), float(dt),
-              stream_handle())
+    with timing.span("k_advance_faces"):
+        _lib.call("edge_advance_map_faces", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), as_ptr(phi_x), as_ptr(phi_y),
+                  as_ptr(phi_z), as_ptr(F_x), as_ptr(F_y), as_ptr(F_z), nx, ny, nz, float(dx), float(dt),
+                  stream_handle())
     touch(phi_x, phi_y, phi_z, F_x, F_y, F_z)
 
 

@@ -35,7 +35,9 @@ class OracleStep:
         self.i = -1
         self.iters = []
         self.lp = None
-        self.scheme = scheme   # "edge" (run_edge.py), "ed8" (run_ed8.py), "ge" (run_ge.py)
+        self.scheme = scheme   # "edge" (run_edge.py), "ed8" (run_ed8.py), "ge" (run_ge.py), "efm" (run_efm.py)
+        self.u_cache = [None] * reinit_every   # EFM: u after each step's first projection
+        self.dts = [0.0] * reinit_every
 
     def set_particles(self, pos, vel, ref_radius, num, params):
         """Laden particles (run_edge.py:235-254): pos/vel (3, n); params as the driver's lp dict."""
@@ -66,6 +68,21 @@ class OracleStep:
             self.psi, self.T = p.reshape((3,) + n), Tm.reshape((3, 3) + n)
             self.dT, self.d2T = o.fd3(self.T, None, dx, want_dT=True)
         self.psif, self.Tf = o.split_psi_hermite(HK_FULL, self.psi, self.T, self.dT, self.d2T, dx)
+
+    def backtrack_psi_grid(self, j, dt):
+        """run_efm.py:170-185, 355-360: cache u, then the face map from the identity
+        marched back through the current and the cached velocities."""
+        o, dx, n = self.o, self.dx, (self.nx, self.ny, self.nz)
+        self.u_cache[j] = [c.copy() for c in self.u]
+        self.dts[j] = dt
+        psif = [face_coords(a, *n, dx).astype(self.dt_) for a in range(3)]
+        Tf = [np.broadcast_to(np.eye(3)[a][:, None, None, None], psif[a].shape).astype(self.dt_) for a in range(3)]
+        for a in range(3):
+            psif[a], Tf[a] = o.advance_map_col(psif[a], Tf[a], self.u, dx, -dt)
+        for step in reversed(range(j)):
+            for a in range(3):
+                psif[a], Tf[a] = o.advance_map_col(psif[a], Tf[a], self.u_cache[step], dx, -self.dts[step])
+        self.psif, self.Tf = psif, Tf
 
     def particle_forces(self, dt):
         """run_edge.py:627-641 with the oracle's sim/lp.py restatement."""
@@ -113,6 +130,8 @@ class OracleStep:
         self.poisson()
         if self.scheme == "edge":
             self.psi, self.T, self.dT = o.march_edge(self.psi, self.T, self.dT, self.u, self.X, dx, dt, self.eps)
+        elif self.scheme == "efm":
+            self.backtrack_psi_grid(j, dt)
         else:
             self.march_full_map(dt)
         for a in range(3):

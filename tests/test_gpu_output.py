@@ -41,7 +41,7 @@ def _sim(case="cfg1", res=(64, 32, 32), **kw):
 
 def _state(sim):
     names = ["u_x", "u_y", "u_z", "smoke"] + sim._state_fields()
-    return {k: getattr(sim, k).data.detach().cpu().numpy().copy() for k in names}
+    return {k: sim._field(k).data.detach().cpu().numpy().copy() for k in names}
 
 
 def test_frame_vti_holds_vorticity_and_smoke(orc32, dev, tmp_path):
@@ -88,9 +88,10 @@ def test_vtk_frame_one_channel_and_no_smoke(dev):
     assert scale_err(got_w, np.linalg.norm(w, axis=0)) < 1e-6
 
 
-@pytest.mark.parametrize("case", ["cfg1", "cfg5"])
+@pytest.mark.parametrize("case", ["cfg1", "cfg5", "efm"])
 def test_resume_mid_reinit_continues_bit_for_bit(dev, tmp_path, case):
-    kw = dict(case="cfg5", res=(32, 16, 16), num_particles=3000) if case == "cfg5" else {}
+    kw = {"cfg5": dict(case="cfg5", res=(32, 16, 16), num_particles=3000),
+          "efm": dict(scheme="efm", reinit_every=4)}.get(case, {})
     a = _sim(**kw)
     a.initialize()
     for _ in range(3):
@@ -108,7 +109,7 @@ def test_resume_mid_reinit_continues_bit_for_bit(dev, tmp_path, case):
         b.substep()
     got = _state(b)
     for k in want:
-        if case == "cfg1":
+        if case != "cfg5":
             assert np.array_equal(got[k], want[k]), k
         else:
             # the drag scatter adds tile borders with FP32 atomics, in an order that

@@ -20,10 +20,10 @@ def scale_err(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
 
 
-def make(res, fixed_iters=None, case="cfg1", clamp=None, scheme="edge", dtype=np.float32):
+def make(res, fixed_iters=None, case="cfg1", clamp=None, scheme="edge", dtype=np.float32, reinit_every=None):
     from run_edge import Simulation
     from oracle.step import OracleStep
-    sim = Simulation(case, solver_mode="plain", res=res, num_samples=400, scheme=scheme)
+    sim = Simulation(case, solver_mode="plain", res=res, num_samples=400, scheme=scheme, reinit_every=reinit_every)
     if clamp is not None:
         sim.BFECC_clamp = clamp
     sim.initialize()
@@ -76,6 +76,22 @@ def test_ed8_ge_steps_fixed_iterations(dev, scheme):
     e_gpu = scale_err(np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref64.psi)
     e_32 = scale_err(ref32.psi, ref64.psi)
     assert e_gpu <= max(3.0 * e_32, 1e-5), (e_gpu, e_32)
+
+
+def test_efm_steps_fixed_iterations(dev):
+    """Six steps of the run_efm.py loop with reinit every 4: the face map is
+    re-marched through 1, 2, 3, 4, then 1, 2 cached velocities."""
+    sim, ref = make((32, 16, 16), fixed_iters=6, scheme="efm", reinit_every=4, clamp=True)
+    for _ in range(6):
+        dt = sim.substep()
+        ref.substep(dt=dt)
+        torch.cuda.synchronize()
+        for a, (p, t) in enumerate(zip((sim.psi_x, sim.psi_y, sim.psi_z), (sim.T_x, sim.T_y, sim.T_z))):
+            assert scale_err(np.moveaxis(p.to_numpy(), -1, 0), ref.psif[a]) < 1e-5
+            assert scale_err(np.moveaxis(t.to_numpy(), -1, 0), ref.Tf[a]) < 1e-4
+    for g, r in zip(sim.velocity_numpy(), ref.u):
+        assert scale_err(g, r) < 1e-4
+    assert scale_err(np.moveaxis(sim.smoke.to_numpy(), -1, 0), ref.smoke) < 1e-4
 
 
 def test_particle_step_fixed_iterations(dev):
