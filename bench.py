@@ -280,12 +280,13 @@ def main():
     timing.enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    i_first = sim.i + 1
     walls, phases = run_steps(sim, a.steps, torch)
     torch.cuda.synchronize()
     total_ms = (time.perf_counter() - t0) * 1e3
     timing.enable(False)
     march_kernel = {"edge": "k_march_edge", "ed8": "k_march_ed8", "ge": "k_march_ge",
-                    "efm": "k_advance_faces"}[a.scheme]
+                    "efm": "k_backtrack_faces"}[a.scheme]
     march_launch_ms = timing.durations(march_kernel)
     ms_step = total_ms / a.steps
     ph = {k: statistics.median(p.get(k, 0.0) for p in phases) for k in phases[0]}
@@ -311,8 +312,12 @@ def main():
     # HIP events on its stream; FP32 vector-ALU bound (no matrix op on this path)
     march_ms = statistics.mean(march_launch_ms)
     if a.scheme == "efm":
+        # step i re-marches every face through (i mod reinit_every) + 1 velocities in one launch:
+        # the work per launch is the mean over the timed steps
         nface = 3 * ncell + sim.res_y * sim.res_z + sim.res_x * sim.res_z + sim.res_x * sim.res_y
-        flop_launch, bytes_launch = FACE_MARCH_FLOP_PER_FACE * nface, FACE_MARCH_BYTES_PER_FACE * nface
+        marches = statistics.mean((i % sim.reinit_every) + 1 for i in range(i_first, i_first + a.steps))
+        flop_launch = FACE_MARCH_FLOP_PER_FACE * nface * marches
+        bytes_launch = FACE_MARCH_BYTES_PER_FACE * nface * marches
     else:
         flop_launch = SCHEME_FLOP_PER_CELL[a.scheme] * ncell
         bytes_launch = SCHEME_BYTES_PER_CELL[a.scheme] * ncell

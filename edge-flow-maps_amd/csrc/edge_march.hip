@@ -1,6 +1,8 @@
 // Backward-map marches (sim/hermite.py) and the forward face march
 // (run_edge.py:330-357), one thread per cell or face, everything between the
 // reads of the old map and the writes of the new one kept in registers.
+#include <algorithm>
+
 #include "edge_common.h"
 
 using namespace edge;
@@ -193,6 +195,60 @@ __global__ void __launch_bounds__(256) EDGE_FACES_ATTR k_advance_faces(Grid g, M
   float p[3] = {phi[t], phi[nf + t], phi[2 * nf + t]};
   float f[3] = {F[t], F[nf + t], F[2 * nf + t]};
   rk4_phi_F(g, U, p, f, dt);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    phi[c * nf + t] = p[c];
+    F[c * nf + t] = f[c];
+  }
+}
+
+// A sequence of velocity fields for the EFM backtrack, passed by value (one
+// launch takes up to EFM_SEQ of them; longer sequences chain launches).
+constexpr int EFM_SEQ = 32;
+struct VelSeq {
+  const float* ux[EFM_SEQ];
+  const float* uy[EFM_SEQ];
+  const float* uz[EFM_SEQ];
+  float dt[EFM_SEQ];
+  int n;
+};
+
+// run_efm.py:170-185 backtrack_psi_grid fused: every face point and its column
+// (from the identity when `from_identity`, else from psi/T in memory) marched
+// by advance_map through velocity 0, 1, .., n-1 with dt[s], kept in registers
+// between the marches and written once.  The same per-face operations, in the
+// same order, as one k_reset_faces and n k_advance_faces launches.
+__global__ void __launch_bounds__(256) EDGE_FACES_ATTR k_backtrack_faces(Grid g, VelSeq vs, FaceMap fm,
+                                                                        int from_identity) {
+  const int64_t nf0 = (int64_t)(g.nx + 1) * g.ny * g.nz;
+  const int64_t nf1 = (int64_t)g.nx * (g.ny + 1) * g.nz;
+  const int64_t nf2 = (int64_t)g.nx * g.ny * (g.nz + 1);
+  int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  int a;
+  int64_t nf;
+  if (t < nf0) { a = 0; nf = nf0; }
+  else if (t < nf0 + nf1) { a = 1; t -= nf0; nf = nf1; }
+  else if (t < nf0 + nf1 + nf2) { a = 2; t -= nf0 + nf1; nf = nf2; }
+  else return;
+  float* phi = fm.phi[a];
+  float* F = fm.F[a];
+  float p[3], f[3];
+  if (from_identity) {
+    int i, j, k;
+    face_ijk(g, a, (uint32_t)t, i, j, k);
+    face_coord(a, i, j, k, g.dx, p);
+    f[0] = a == 0 ? 1.f : 0.f;
+    f[1] = a == 1 ? 1.f : 0.f;
+    f[2] = a == 2 ? 1.f : 0.f;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      p[c] = phi[c * nf + t];
+      f[c] = F[c * nf + t];
+    }
+  }
+#pragma unroll 1
+  for (int s = 0; s < vs.n; ++s) rk4_phi_F(g, Mac{vs.ux[s], vs.uy[s], vs.uz[s]}, p, f, vs.dt[s]);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     phi[c * nf + t] = p[c];
@@ -540,6 +596,39 @@ int edge_advance_map_faces(const float* ux, const float* uy, const float* uz, fl
   hipLaunchKernelGGL(k_advance_faces, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g,
                      Mac{ux, uy, uz}, fm, dt);
   EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_backtrack_faces(const float* const* ux, const float* const* uy, const float* const* uz, const float* dts,
+                         int n, float* phix, float* phiy, float* phiz, float* Fx, float* Fy, float* Fz, int nx, int ny,
+                         int nz, float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_backtrack_faces: bad dims");
+  EDGE_REQUIRE(n >= 0 && (n == 0 || (ux && uy && uz && dts)), "edge_backtrack_faces: bad velocity list");
+  EDGE_REQUIRE(phix && phiy && phiz && Fx && Fy && Fz, "edge_backtrack_faces: null pointer");
+  for (int s = 0; s < n; ++s) EDGE_MAC_OK(ux[s], uy[s], uz[s]);
+  Grid g = make_grid(nx, ny, nz, dx);
+  int64_t nf = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
+  FaceMap fm{{phix, phiy, phiz}, {Fx, Fy, Fz}};
+  int s0 = 0;
+  do {  // n == 0: one launch that only writes the identity
+    VelSeq vs;
+    vs.n = std::min(EFM_SEQ, n - s0);
+    for (int s = 0; s < vs.n; ++s) {
+      vs.ux[s] = ux[s0 + s];
+      vs.uy[s] = uy[s0 + s];
+      vs.uz[s] = uz[s0 + s];
+      vs.dt[s] = dts[s0 + s];
+    }
+    for (int s = vs.n; s < EFM_SEQ; ++s) {
+      vs.ux[s] = vs.uy[s] = vs.uz[s] = nullptr;
+      vs.dt[s] = 0.f;
+    }
+    hipLaunchKernelGGL(k_backtrack_faces, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g, vs, fm,
+                       s0 == 0 ? 1 : 0);
+    EDGE_CHECK_LAUNCH();
+    s0 += vs.n;
+  } while (s0 < n);
   return EDGE_OK;
 }
 

@@ -55,6 +55,7 @@ SIGNATURES = {
     "edge_get_central_vector": "ppppiiip",
     "edge_curl": "ppiiifp",
     "edge_vtk_frame": "ppppippiiifp",
+    "edge_backtrack_faces": "ppppippppppiiifp",
     # projection
     "edge_mg_create": "iiiiipp",
     "edge_mg_destroy": "p",

@@ -35,7 +35,7 @@ from edge_hip.field import Field, stream_handle, ti  # noqa: E402
 from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smoke_psi, clamp_u3,  # noqa: E402
                              map_smoke, map_smoke_psi, map_u, map_u_psi, march_phi_grid, reset_to_identity)
 from sim import lp as lp_kernels  # noqa: E402
-from sim.edge_driver import clamp_smoke, split_psi_hermite  # noqa: E402
+from sim.edge_driver import backtrack_faces, clamp_smoke, split_psi_hermite  # noqa: E402
 from sim.hermite import (advect_ps_garm, advect_ps_hermite_ed8, advect_psi_garm_ed4_fusion_unroll,  # noqa: E402
                          advect_u_semi_lagrangian, reset_to_identity_2, reset_to_identity_3)
 from sim.init_conditions import (init_delta, init_four_vorts, init_rings, init_vorts_leapfrog,  # noqa: E402
@@ -185,6 +185,7 @@ class Simulation:
             # last reinitialisation, and that step's dt
             self.u_cache = [(F(float, fx), F(float, fy), F(float, fz)) for _ in range(self.reinit_every)]
             self.dts = [0.0] * self.reinit_every
+            self.efm_fused = True   # False: one reset and j+1 face-march launches, as the reference
         # scratch of the reference's fused march (accepted, unused by the in-place kernel)
         self.psi_tmp = self.T_x_tmp = self.T_y_tmp = self.T_z_tmp = None
         self.dT_x_tmp = self.dT_y_tmp = self.dT_z_tmp = None
@@ -408,9 +409,15 @@ class Simulation:
         for c, f in zip(self.u_cache[j], (self.u_x, self.u_y, self.u_z)):
             c.copy_from(f)
         faces = (self.psi_x, self.psi_y, self.psi_z, self.T_x, self.T_y, self.T_z)
+        steps = list(reversed(range(j)))
+        if self.efm_fused:
+            # one kernel: each face marched through every velocity in registers
+            backtrack_faces(*faces, [(self.u_x, self.u_y, self.u_z)] + [self.u_cache[s] for s in steps],
+                            [-curr_dt] + [-self.dts[s] for s in steps], dx)
+            return
         reset_to_identity(*faces, dx)
         march_phi_grid(*faces, self.u_x, self.u_y, self.u_z, dx, -curr_dt)
-        for step in reversed(range(j)):
+        for step in steps:
             march_phi_grid(*faces, *self.u_cache[step], dx, -self.dts[step])
 
     def march_full_map(self, curr_dt):

@@ -63,6 +63,23 @@ def march_phi_grid(phi_x, phi_y, phi_z, F_x, F_y, F_z, u_x, u_y, u_z, dx, dt):
     touch(phi_x, phi_y, phi_z, F_x, F_y, F_z)
 
 
+def backtrack_faces(psi_x, psi_y, psi_z, T_x, T_y, T_z, velocities, dts, dx):
+    """run_efm.py:170-185 backtrack_psi_grid in one pass: reset_to_identity, then
+    advance_map through velocities[s] = (u_x, u_y, u_z) with dt dts[s], s = 0, 1, ..
+    (the reference passes the current velocity first, then its cache newest first)."""
+    import ctypes
+    n = len(velocities)
+    assert len(dts) == n
+    nx, ny, nz = _cells_of_mac(psi_x)
+    arr = [(ctypes.c_void_p * max(n, 1))(*[as_ptr(v[c]) for v in velocities]) for c in range(3)]
+    dta = (ctypes.c_float * max(n, 1))(*[float(d) for d in dts])
+    with timing.span("k_backtrack_faces"):
+        _lib.call("edge_backtrack_faces", *[ctypes.cast(a, ctypes.c_void_p) for a in arr],
+                  ctypes.cast(dta, ctypes.c_void_p), n, as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z), as_ptr(T_x),
+                  as_ptr(T_y), as_ptr(T_z), nx, ny, nz, float(dx), stream_handle())
+    touch(psi_x, psi_y, psi_z, T_x, T_y, T_z)
+
+
 def map_u(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, T_x, T_y, T_z, psi_x, psi_y, psi_z, dx):
     """run_edge.py:360-373."""
     nx, ny, nz = _cells_of_mac(u_x0)

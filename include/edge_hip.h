@@ -70,6 +70,12 @@ int edge_march_ed4(const float* ux, const float* uy, const float* uz, const floa
 /* march_phi_grid -> advance_map on the three face grids (run_edge.py:330-357). */
 int edge_advance_map_faces(const float* ux, const float* uy, const float* uz, float* phix, float* phiy, float* phiz,
                            float* Fx, float* Fy, float* Fz, int nx, int ny, int nz, float dx, float dt, void* stream);
+/* backtrack_psi_grid (run_efm.py:170-185) in one pass: the face maps (phi, F) from the
+ * identity, marched by advance_map through velocity s = 0 .. n-1 with dt dts[s].
+ * ux, uy, uz: host arrays of n device pointers (MAC velocity fields); dts: host array. */
+int edge_backtrack_faces(const float* const* ux, const float* const* uy, const float* const* uz, const float* dts,
+                         int n, float* phix, float* phiy, float* phiz, float* Fx, float* Fy, float* Fz, int nx, int ny,
+                         int nz, float dx, void* stream);
 /* mode 0: advance_map_0 (sim/hermite.py:786-803); mode 1:
  * advance_map_1_transpose (sim/hermite.py:863-918).  pts: 3 planes of n. */
 int edge_advance_points(int mode, const float* ux, const float* uy, const float* uz, float* pts, float* Tx,

@@ -264,3 +264,40 @@ def test_vortex_ring_init(orc32, dev):
     ops.vortex_ring(*args, T_(X, dev), gv, gs, (1, 0, 0), 500)
     assert scale_err(N_(gv), vf) < 1e-4
     assert np.mean(N_(gs) != sm) < 1e-3  # threshold flips only at |s - 0.002| ~ rounding
+
+
+@pytest.mark.parametrize("nvel", [0, 3, 35])
+def test_backtrack_faces_equals_sequential_marches(dev, nvel):
+    """run_efm.py:170-185: the one-pass backtrack (faces kept in registers
+    across the velocity list, 32 velocities per launch) against a reset and one
+    face-march launch per velocity: the same values bit for bit."""
+    from edge_hip import ti
+    from sim.edge_driver import backtrack_faces, march_phi_grid, reset_to_identity
+    n, dx = (20, 12, 12), 1.0 / 12
+    fs = [(n[0] + 1, n[1], n[2]), (n[0], n[1] + 1, n[2]), (n[0], n[1], n[2] + 1)]
+    vels = []
+    for s in range(nvel):
+        u = smooth_velocity(*n, dx, seed=40 + s % 5, amp=0.5)
+        f = [ti.field(float, shape=sh) for sh in fs]
+        for c, a in zip(f, u):
+            c.from_numpy(a.astype(np.float32))
+        vels.append(tuple(f))
+    dts = [-(0.3 + 0.01 * s) * dx for s in range(nvel)]
+    A = [ti.Vector.field(3, float, shape=sh) for sh in fs] + [ti.Vector.field(3, float, shape=sh) for sh in fs]
+    B = [ti.Vector.field(3, float, shape=sh) for sh in fs] + [ti.Vector.field(3, float, shape=sh) for sh in fs]
+    backtrack_faces(*A, vels, dts, dx)
+    reset_to_identity(*B, dx)
+    for v, d in zip(vels, dts):
+        march_phi_grid(*B, *v, dx, d)
+    torch.cuda.synchronize()
+    for a, b in zip(A, B):
+        assert np.array_equal(a.to_numpy(), b.to_numpy())
+    if nvel:
+        assert not np.array_equal(A[0].to_numpy(), np.moveaxis(face_coords_np(0, n, dx), 0, -1))
+
+
+def face_coords_np(a, n, dx):
+    d = [n[0] + (a == 0), n[1] + (a == 1), n[2] + (a == 2)]
+    i, j, k = np.meshgrid(*[np.arange(m) for m in d], indexing="ij")
+    o = [0.0 if a == b else 0.5 for b in range(3)]
+    return np.stack([(i + o[0]) * dx, (j + o[1]) * dx, (k + o[2]) * dx]).astype(np.float32)

@@ -94,6 +94,20 @@ def test_efm_steps_fixed_iterations(dev):
     assert scale_err(np.moveaxis(sim.smoke.to_numpy(), -1, 0), ref.smoke) < 1e-4
 
 
+def test_efm_fused_backtrack_equals_the_reference_sequence(dev):
+    """The EFM step with the one-pass backtrack against the reference's reset +
+    one face march per cached velocity: identical steps."""
+    a, _ = make((32, 16, 16), fixed_iters=6, scheme="efm", reinit_every=4)
+    b, _ = make((32, 16, 16), fixed_iters=6, scheme="efm", reinit_every=4)
+    b.efm_fused = False
+    for _ in range(5):
+        a.substep()
+        b.substep()
+    for g, r in zip(a.velocity_numpy(), b.velocity_numpy()):
+        assert np.array_equal(g, r)
+    assert np.array_equal(a.psi_x.to_numpy(), b.psi_x.to_numpy())
+
+
 def test_particle_step_fixed_iterations(dev):
     """Two steps of the particle case (drag, viscous force, P2G, force pull-back,
     projection with a force buffer) against the oracle's composition."""
