@@ -36,8 +36,8 @@ FP32_PEAK_TFLOPS = 157.3
 MARCH_FLOP_PER_CELL = 22250.0      # fused EDGE march: 5 points x (RK4 with Jacobian + Hermite compose)
 # ED8: 8 points x (RK4 of the point + full-Hermite value); GE: 1 point (RK4 with Jacobian + Hermite gradient)
 SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 16550.0, "ge": 4430.0}
-# the face march (RK4 of a face point and its column, k_advance_faces): EFM's dominant kernel
-FACE_MARCH_FLOP_PER_FACE, FACE_MARCH_BYTES_PER_FACE = 2860.0, 52.0
+# the face march (RK4 of a face point and its column): per face and velocity in EFM's backtrack
+FACE_MARCH_FLOP_PER_FACE = 2860.0
 SCHEME_BYTES_PER_CELL = {"edge": 180.0, "ed8": 204.0, "ge": 324.0}
 MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0}  # one CG iteration
 MARCH_BYTES_PER_CELL = 180.0       # u 12 + old map 84 + new map 84 (compulsory FP32)
@@ -317,7 +317,8 @@ def main():
         nface = 3 * ncell + sim.res_y * sim.res_z + sim.res_x * sim.res_z + sim.res_x * sim.res_y
         marches = statistics.mean((i % sim.reinit_every) + 1 for i in range(i_first, i_first + a.steps))
         flop_launch = FACE_MARCH_FLOP_PER_FACE * nface * marches
-        bytes_launch = FACE_MARCH_BYTES_PER_FACE * nface * marches
+        # compulsory: the face map written once (24 B per face) and one velocity read per march
+        bytes_launch = (24.0 + 4.0 * marches) * nface
     else:
         flop_launch = SCHEME_FLOP_PER_CELL[a.scheme] * ncell
         bytes_launch = SCHEME_BYTES_PER_CELL[a.scheme] * ncell
