@@ -324,32 +324,59 @@ __global__ void __launch_bounds__(256) k_fill(float* __restrict__ p, float v, in
 }
 
 // sim/init_conditions.py:13-36 add_vortex_ring_and_smoke (pf = cell centres).
-__global__ void __launch_bounds__(256) k_vortex_ring(Grid g, float w, float rad, float delta, float3 c, float3 ex,
-                                                     float3 ey, const float* __restrict__ pf, float* __restrict__ vf,
-                                                     float* __restrict__ smokef, float3 color, int num_samples) {
+// The sample points and their vorticity vectors do not depend on the cell: a
+// block computes them once per chunk of RING_CHUNK samples into LDS (one
+// sample per thread, the same expressions), then every thread sums the chunk
+// for its cell in sample order.
+constexpr int RING_CHUNK = 256;
+__global__ void __launch_bounds__(RING_CHUNK) k_vortex_ring(Grid g, float w, float rad, float delta, float3 c,
+                                                            float3 ex, float3 ey, const float* __restrict__ pf,
+                                                            float* __restrict__ vf, float* __restrict__ smokef,
+                                                            float3 color, int num_samples) {
+  __shared__ float sp[6][RING_CHUNK];   // sample position (3) and vorticity vector (3)
   const int64_t N = ncell(g);
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= N) return;
+  const bool live = q < N;
   const float pi = 3.14159265358979323846f;
-  float curve_length = (2.0f * pi * rad) / (float)num_samples;
-  float P[3] = {pf[q], pf[N + q], pf[2 * N + q]};
-  float acc[3] = {0.f, 0.f, 0.f}, sm = 0.f;
-  for (int l = 0; l < num_samples; ++l) {
-    float theta = (float)l / (float)num_samples * 2.0f * pi;
-    float ct = cosf(theta), st = sinf(theta);
-    float ps[3] = {rad * (ct * ex.x + st * ey.x) + c.x, rad * (ct * ex.y + st * ey.y) + c.y,
-                   rad * (ct * ex.z + st * ey.z) + c.z};
-    float pd[3] = {P[0] - ps[0], P[1] - ps[1], P[2] - ps[2]};
-    float wv[3] = {w * (-st * ex.x + ct * ey.x), w * (-st * ex.y + ct * ey.y), w * (-st * ex.z + ct * ey.z)};
-    float r = sqrtf(pd[0] * pd[0] + pd[1] * pd[1] + pd[2] * pd[2]);
-    float rd = r / delta;
-    float e = expf(-(rd * rd * rd));
-    float fac = curve_length * (-1.0f / (4.0f * pi * (r * r * r)) * (1.0f - e));
-    acc[0] += fac * (pd[1] * wv[2] - pd[2] * wv[1]);
-    acc[1] += fac * (pd[2] * wv[0] - pd[0] * wv[2]);
-    acc[2] += fac * (pd[0] * wv[1] - pd[1] * wv[0]);
-    sm += curve_length * e;
+  const float curve_length = (2.0f * pi * rad) / (float)num_samples;
+  float P[3] = {0.f, 0.f, 0.f};
+  if (live) {
+    P[0] = pf[q];
+    P[1] = pf[N + q];
+    P[2] = pf[2 * N + q];
   }
+  float acc[3] = {0.f, 0.f, 0.f}, sm = 0.f;
+  for (int l0 = 0; l0 < num_samples; l0 += RING_CHUNK) {
+    const int l = l0 + (int)threadIdx.x;
+    if (l < num_samples) {
+      const float theta = (float)l / (float)num_samples * 2.0f * pi;
+      const float ct = cosf(theta), st = sinf(theta);
+      sp[0][threadIdx.x] = rad * (ct * ex.x + st * ey.x) + c.x;
+      sp[1][threadIdx.x] = rad * (ct * ex.y + st * ey.y) + c.y;
+      sp[2][threadIdx.x] = rad * (ct * ex.z + st * ey.z) + c.z;
+      sp[3][threadIdx.x] = w * (-st * ex.x + ct * ey.x);
+      sp[4][threadIdx.x] = w * (-st * ex.y + ct * ey.y);
+      sp[5][threadIdx.x] = w * (-st * ex.z + ct * ey.z);
+    }
+    __syncthreads();
+    const int m = min(RING_CHUNK, num_samples - l0);
+    if (live) {
+      for (int s = 0; s < m; ++s) {
+        const float pd[3] = {P[0] - sp[0][s], P[1] - sp[1][s], P[2] - sp[2][s]};
+        const float wv[3] = {sp[3][s], sp[4][s], sp[5][s]};
+        const float r = sqrtf(pd[0] * pd[0] + pd[1] * pd[1] + pd[2] * pd[2]);
+        const float rd = r / delta;
+        const float e = expf(-(rd * rd * rd));
+        const float fac = curve_length * (-1.0f / (4.0f * pi * (r * r * r)) * (1.0f - e));
+        acc[0] += fac * (pd[1] * wv[2] - pd[2] * wv[1]);
+        acc[1] += fac * (pd[2] * wv[0] - pd[0] * wv[2]);
+        acc[2] += fac * (pd[0] * wv[1] - pd[1] * wv[0]);
+        sm += curve_length * e;
+      }
+    }
+    __syncthreads();
+  }
+  if (!live) return;
   vf[q] += acc[0];
   vf[N + q] += acc[1];
   vf[2 * N + q] += acc[2];
@@ -652,8 +679,8 @@ int edge_vortex_ring(float w, float rad, float delta, const float* c, const floa
   EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_vortex_ring: bad dims");
   EDGE_REQUIRE(c && ex && ey && pf && vf && smokef && color && num_samples > 0, "edge_vortex_ring: bad arguments");
   Grid g = make_grid(nx, ny, nz, 1.0f);
-  hipLaunchKernelGGL(k_vortex_ring, dim3(blocks_for(ncell(g), 256)), dim3(256), 0, (hipStream_t)stream, g, w, rad,
-                     delta, make_float3(c[0], c[1], c[2]), make_float3(ex[0], ex[1], ex[2]),
+  hipLaunchKernelGGL(k_vortex_ring, dim3(blocks_for(ncell(g), RING_CHUNK)), dim3(RING_CHUNK), 0, (hipStream_t)stream,
+                     g, w, rad, delta, make_float3(c[0], c[1], c[2]), make_float3(ex[0], ex[1], ex[2]),
                      make_float3(ey[0], ey[1], ey[2]), pf, vf, smokef, make_float3(color[0], color[1], color[2]),
                      num_samples);
   EDGE_CHECK_LAUNCH();
