@@ -50,8 +50,11 @@ inline Grid make_grid(int nx, int ny, int nz, float dx) {
 inline bool dims_ok(int nx, int ny, int nz) {
   // the 3-tap B-spline and the Hermite corner pair need at least 4 cells per axis
   // index decodes use 32-bit arithmetic: every face grid together below 2^31
+  // buffer-resource gathers take a 32-bit byte range: one component's Hermite
+  // table (32 B per cell) below 4 GiB, i.e. fewer than 2^27 cells
   const int64_t nfaces = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
-  return nx >= 4 && ny >= 4 && nz >= 4 && nfaces < (int64_t)1 << 31;
+  const int64_t ncells = (int64_t)nx * ny * nz;
+  return nx >= 4 && ny >= 4 && nz >= 4 && nfaces < (int64_t)1 << 31 && ncells < (int64_t)1 << 27;
 }
 
 inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
