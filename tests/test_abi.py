@@ -73,3 +73,8 @@ def test_bad_arguments_are_rejected_without_gpu_work():
         _lib.call("edge_march_edge", *([None] * 12), 2, 2, 2, 0.1, 0.1, 0.01, None)
     with pytest.raises(RuntimeError, match="null"):
         _lib.call("edge_map_u_psi", *([None] * 7), 8, 8, 8, 0.1, None)
+    # the gathers' 32-bit buffer ranges: fewer than 2^27 cells (1024 x 512 x 256 is 2^27)
+    with pytest.raises(RuntimeError, match="bad dims"):
+        _lib.call("edge_split_psi_hermite", *([None] * 7), 1024, 512, 256, 0.1, None)
+    with pytest.raises(RuntimeError, match="bad dims"):
+        _lib.call("edge_backtrack_faces", None, None, None, None, 0, *([None] * 6), 2, 8, 8, 0.1, None)
