@@ -129,7 +129,7 @@ def test_edge_march_parity(orc32, orc64, dev):
     ops.march_edge([T_(c, dev) for c in u], tab, T_(X, dev), gp, gT, gdT, dx, dt, eps)
     assert scale_err(N_(gp), r32[0]) < 1e-6
     assert np.max(np.abs(N_(gT) - r32[1])) < 2e-5
-    # dT is an eps-difference: judge against FP64 with the FP32 oracle's own error as the yardstick
+    # dT is an eps-difference: compare against FP64 with the FP32 oracle's own error as the yardstick
     e_orc = np.max(np.abs(r32[2] - r64[2]))
     e_gpu = np.max(np.abs(N_(gdT) - r64[2]))
     assert e_gpu <= 4 * e_orc + 1e-3 * np.max(np.abs(r64[2])), (e_gpu, e_orc)
@@ -147,7 +147,7 @@ def test_driver_maps_parity(orc32, orc64, dev):
     ref64 = orc64.map_u_psi(u, psi, T, dT, dx)
     for a in range(3):
         # the FP32 oracle itself is ~1e-5 of scale from FP64 here (sum of
-        # d psi_c/dx_a u_c with cancellation): judge both against FP64
+        # d psi_c/dx_a u_c with cancellation): compare both against FP64
         e32 = scale_err(ref[a], ref64[a])
         assert scale_err(N_(out[a]), ref64[a]) <= max(2.0 * e32, 1e-5), (a, e32)
         assert scale_err(N_(out[a]), ref[a]) < 5e-5

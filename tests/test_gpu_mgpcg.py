@@ -3,8 +3,8 @@
 Tolerances (FP32): operator application and one V-cycle agree with the FP32
 oracle to 1e-5 of scale; a fixed number of CG iterations to 1e-4 (different
 FP64 reduction order changes alpha/beta in the last bits); converged solves are
-judged against the FP64 oracle through the true residual ||b - A p|| <= 1e-6
-||b|| (the north-star stop rule) and the pressure difference.
+compared with the FP64 oracle through the true residual ||b - A p|| <= 1e-6
+||b|| (the relative stop rule) and the pressure difference.
 """
 import ctypes
 

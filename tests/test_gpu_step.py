@@ -5,7 +5,7 @@ the oracle's composition of the same step (oracle/step.py).
   smoke and map within 1e-4 of scale (FP32, different summation orders,
   compounded over ~60 kernels);
 * a 10-step trajectory with converged projections (1e-6 relative on both
-  sides): velocity within 1e-3 of scale (the north-star trajectory bound).
+  sides): velocity within 1e-3 of scale (the trajectory bound).
 """
 import numpy as np
 import pytest
@@ -56,7 +56,7 @@ def test_one_step_fixed_iterations(dev, clamp):
 def test_ed8_ge_steps_fixed_iterations(dev, scheme):
     """Two steps of the run_ed8.py / run_ge.py loops.  Their maps carry
     eps-differences (ED8: up to 1/(8 eps^3)) or finite differences whose FP32
-    rounding differs between any two implementations, so the GPU is judged
+    rounding differs between any two implementations, so the GPU is compared
     against the FP64 oracle with the FP32 oracle's own error as the yardstick."""
     from oracle.step import OracleStep
     sim, ref32 = make((32, 16, 16), fixed_iters=6, scheme=scheme)

@@ -190,5 +190,5 @@ def test_warm_start_keeps_the_solution(dev):
     # the same projected velocity up to the solver tolerance, whichever start was kept
     for a, b_ in zip(fc, fw):
         assert max_scale(b_.to_numpy(), a.to_numpy()) < 1e-4
-    # x0 was judged: used iff ||b - A x0||^2 <= ||b||^2
+    # x0 was checked: used iff ||b - A x0||^2 <= ||b||^2
     assert warm.stats["x0_used"] == (1.0 if warm.stats["r02"] <= warm.stats["b2"] else 0.0)
