@@ -314,3 +314,32 @@ def test_clamp_u_and_max_speed(orc64):
     assert ms == pytest.approx(max(1e-3, float(np.sqrt(cu ** 2 + cv ** 2 + cw ** 2).max())))
     zero = [np.zeros_like(x) for x in uu]
     assert orc64.max_speed(zero) == pytest.approx(1e-3)
+
+
+def test_vortex_ring_biot_savart_on_the_axis(orc64):
+    """sim/init_conditions.py:13-36 against the closed form of a circular vortex
+    ring (circulation w, radius R) on its axis: u = w R^2 / (2 (R^2 + z^2)^(3/2))
+    along ex x ey.  Far from the core the regularisation 1 - exp(-(r/delta)^3)
+    is 1 to machine precision, and the 2000-sample trapezoid rule on the circle
+    is exact to rounding for this periodic integrand; the smoke there stays 0,
+    while a point on the core line is dyed with the ring's colour.  (A point
+    exactly on a sample gives 0/0, in the reference as here.)"""
+    w, R, delta = 1.3, 0.2, 0.03
+    c = np.array([0.5, 0.4, 0.6])
+    ex, ey = np.array([0.0, 0.0, -1.0]), np.array([0.0, 1.0, 0.0])
+    n_hat = np.cross(ex, ey)
+    z = np.linspace(-0.3, 0.3, 13)
+    pts = c[:, None] + n_hat[:, None] * z[None, :]
+    th = np.pi / 2000                      # on the core line, between samples 0 and 1
+    core = c + R * (np.cos(th) * ex + np.sin(th) * ey)
+    X = np.concatenate([pts, core[:, None]], axis=1).reshape(3, -1, 1, 1)
+    vf = np.zeros(X.shape, np.float64)
+    smoke = np.zeros((4,) + X.shape[1:], np.float64)
+    orc64.vortex_ring(w, R, delta, c, ex, ey, X, vf, smoke, (0.25, 0.5, 1.0), 2000)
+    v = vf.reshape(3, -1)
+    want = w * R ** 2 / (2 * (R ** 2 + z ** 2) ** 1.5)
+    assert np.max(np.abs(v[:, :13] - n_hat[:, None] * want[None, :])) < 1e-12 * want.max()
+    s = smoke.reshape(4, -1)
+    assert np.all(s[:, :13] == 0)
+    assert np.allclose(s[:, 13], [0.25, 0.5, 1.0, 1.0])
+    assert np.all(np.isfinite(v[:, 13]))
