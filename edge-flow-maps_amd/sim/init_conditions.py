@@ -108,27 +108,36 @@ DELTA_SPEED = 0.6
 DELTA_ATTACK_DEG = 20.0
 
 
-def voxelize_triangle(vertices, dx, subsamples=4):
-    """Cells (i, j) of a pitch-dx grid covered by a triangle in the z = 0 plane.
-
-    The reference voxelizes its plate with trimesh (`mesh.voxelized(dx)`, not
-    available here); this marks every cell of the triangle's bounding box with
-    at least one of subsamples^2 sample points inside the triangle.  Returns
-    the (nx, ny) bool array and the grid origin (x0, y0) of cell (0, 0)."""
-    v = np.asarray(vertices, np.float64)[:, :2]
-    lo = np.floor(v.min(0) / dx) * dx
-    hi = np.ceil(v.max(0) / dx) * dx
-    n = np.maximum(np.round((hi - lo) / dx).astype(int), 1)
-    s = (np.arange(subsamples) + 0.5) / subsamples
-    gi, gj, si, sj = np.meshgrid(np.arange(n[0]), np.arange(n[1]), s, s, indexing="ij")
-    px = lo[0] + (gi + si) * dx
-    py = lo[1] + (gj + sj) * dx
-
-    def side(a, b):
-        return (b[0] - a[0]) * (py - a[1]) - (b[1] - a[1]) * (px - a[0])
-    d0, d1, d2 = side(v[0], v[1]), side(v[1], v[2]), side(v[2], v[0])
-    inside = ((d0 >= 0) & (d1 >= 0) & (d2 >= 0)) | ((d0 <= 0) & (d1 <= 0) & (d2 <= 0))
-    return inside.any(axis=(2, 3)), lo
+def voxelize_triangle(vertices, pitch, max_iter=10):
+    """The reference's `trimesh.Trimesh(vertices, [[0, 1, 2]]).voxelized(pitch).matrix`
+    (sim/init_conditions.py:139-150), computed here without trimesh: trimesh's
+    "subdivide" voxelizer splits every triangle with an edge longer than pitch/2
+    into four at its edge midpoints ((a + b) / 2, up to max_iter times), keeps
+    the corners of the triangles that are small enough, and marks the voxel
+    round(corner / pitch) of each (round half to even).  Triangles are carried
+    as coordinate triples; edge lengths are sqrt((dx^2 + dy^2) + dz^2) like
+    trimesh's, so the same triangles stop at the same pass.  Returns the dense
+    bool matrix over the occupied index range and its origin index."""
+    tri = np.asarray(vertices, np.float64)[None]          # (t, 3 corners, 3)
+    kept = []
+    max_edge = pitch / 2.0
+    for _ in range(max_iter + 1):
+        d = tri[:, [1, 2, 0]] - tri                       # the three edge vectors
+        long_ = ((d ** 2).sum(axis=2) ** 0.5 > max_edge).any(axis=1)
+        kept.append(tri[~long_].reshape(-1, 3))
+        if not long_.any():
+            break
+        t = tri[long_]
+        m01 = (t[:, 0] + t[:, 1]) / 2.0
+        m12 = (t[:, 1] + t[:, 2]) / 2.0
+        m20 = (t[:, 2] + t[:, 0]) / 2.0
+        tri = np.concatenate([np.stack([t[:, 0], m01, m20], 1), np.stack([m01, t[:, 1], m12], 1),
+                              np.stack([m20, m12, t[:, 2]], 1), np.stack([m01, m12, m20], 1)])
+    hit = np.unique(np.round(np.concatenate(kept) / pitch).astype(np.int64), axis=0)
+    lo = hit.min(axis=0)
+    m = np.zeros(tuple(hit.max(axis=0) - lo + 1), bool)
+    m[tuple((hit - lo).T)] = True
+    return m, lo
 
 
 def set_boundary_mask_delta(X, boundary_mask, boundary_vel, _t, dx):
@@ -137,17 +146,17 @@ def set_boundary_mask_delta(X, boundary_mask, boundary_vel, _t, dx):
     the box's outer cell layer as solid; the outer layer moves with the
     inflow (0.6 cos 20, 0, 0.6 sin 20), the plate is at rest."""
     shape = boundary_mask.shape
-    ang = np.radians(15.0)
-    vert = np.array([[0.0, 0.0, 0.0], [np.cos(ang), np.sin(ang), 0.0], [np.cos(ang), -np.sin(ang), 0.0]])
-    plate, _ = voxelize_triangle(vert, dx)
+    ang = np.radians(15)
+    x, y = np.cos(ang) * 1.0, np.sin(ang) * 1.0
+    vert = np.array([[0, 0, 0], [x, y, 0], [x, -y, 0]])
+    plate, _ = voxelize_triangle(vert, dx)   # (sx, sy, 1): the plate is flat
     size = plate.shape
-    # a flat triangle voxelizes one cell thick: z offset 0.2 // dx (- 1 // 2)
-    off = np.array([int(0.1 // dx), int(0.5 // dx) - size[1] // 2, int(0.2 // dx)], np.int64)
+    off = np.array([0.1 // dx, 0.5 // dx - size[1] // 2, 0.2 // dx - size[2] // 2], np.int64)
     mask = np.zeros(shape, np.int32)
     sx = min(size[0], shape[0] - off[0])
     sy = min(size[1], shape[1] - off[1])
-    for t in range(4):
-        mask[off[0]:off[0] + sx, off[1]:off[1] + sy, off[2] + t] += plate[:sx, :sy]
+    for t in range(4):   # extruded four cells thick
+        mask[off[0]:off[0] + sx, off[1]:off[1] + sy, off[2] + t] += plate[:sx, :sy, 0]
     mask[mask > 0] = 1
     vel = np.zeros(shape + (3,), np.float32)
     a = np.radians(DELTA_ATTACK_DEG)

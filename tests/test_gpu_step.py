@@ -227,6 +227,13 @@ def test_delta_case_solid_projection(dev, orc64):
     u = [t.cpu().numpy().astype(np.float64) for t in sim.rhs_snapshots[-1]]   # before the step's last projection
     bm = sim.boundary_mask.to_numpy()
     bvel = np.moveaxis(sim.boundary_vel.to_numpy(), -1, 0)
+    # the mask: the restated trimesh plate plus the box's outer layer
+    from oracle import voxel_ref
+    want = voxel_ref.delta_plate_mask(bm.shape, sim.dx)
+    want[[0, -1], :, :] = 1
+    want[:, [0, -1], :] = 1
+    want[:, :, [0, -1]] = 1
+    assert np.array_equal(bm, want)
     assert bm[1:-1, 1:-1, 1:-1].sum() > 50   # the plate is there
     _, rp, rst = orc64.poisson(u, levels=sim.solver.n_mg_levels, bm=bm, bvel=bvel, max_iters=200, stop_mode=1,
                                rel_tol=1e-9)

@@ -343,3 +343,58 @@ def test_vortex_ring_biot_savart_on_the_axis(orc64):
     assert np.all(s[:, :13] == 0)
     assert np.allclose(s[:, 13], [0.25, 0.5, 1.0, 1.0])
     assert np.all(np.isfinite(v[:, 13]))
+
+
+def _plate():
+    ang = np.radians(15)
+    x, y = np.cos(ang) * 1.0, np.sin(ang) * 1.0
+    return np.array([[0, 0, 0], [x, y, 0], [x, -y, 0]], np.float64)
+
+
+@pytest.mark.parametrize("res", [32, 128])
+def test_voxelizer_restatement_is_pinned_by_geometry(res):
+    """The restated trimesh voxelizer (the reference's delta plate): every voxel
+    whose cube lies inside the triangle is marked, and every marked voxel's
+    cube touches the triangle (a half-voxel shift or a floor instead of a round
+    breaks one of the two); mirror symmetric in y like the triangle; and no
+    kept vertex lies within 1e-9 of a rounding boundary, so the bit-level order
+    of the midpoint sums cannot change the result."""
+    from oracle import voxel_ref as V
+    dx = 1.0 / res
+    tri = _plate()
+    m, lo = V.voxelize_subdivide(tri, np.array([[0, 1, 2]]), dx)
+    assert m.shape[2] == 1
+    m2 = m[:, :, 0]
+    assert np.array_equal(m2, m2[:, ::-1])
+    ii, jj = np.meshgrid(np.arange(m2.shape[0]) + lo[0], np.arange(m2.shape[1]) + lo[1], indexing="ij")
+    a, b = np.radians(15), 1.0
+    x1 = np.cos(a) * b
+
+    def inside(px, py):   # the closed triangle (0,0), (x1, +-y1)
+        return (px >= 0) & (px <= x1) & (np.abs(py) <= px * np.tan(a))
+    corners = [(sx, sy) for sx in (-0.5, 0.5) for sy in (-0.5, 0.5)]
+    full = np.ones_like(m2)
+    for sx, sy in corners:
+        full &= inside((ii + sx) * dx, (jj + sy) * dx)
+    assert np.all(m2[full])
+    # a marked voxel's square meets the triangle: a point of a 41 x 41 sample of it is inside
+    s = (np.arange(41) / 40.0) - 0.5
+    touch = np.zeros_like(m2)
+    for sx in s:
+        for sy in s:
+            touch |= inside((ii + sx) * dx, (jj + sy) * dx)
+    assert not np.any(m2 & ~touch)
+    v = V.subdivide_to_size(tri, np.array([[0, 1, 2]]), dx / 2)
+    assert np.min(np.abs(np.abs(v[:, :2] / dx - np.round(v[:, :2] / dx)) - 0.5)) > 1e-9
+
+
+@pytest.mark.parametrize("res", [16, 48, 128, 256])
+def test_delta_plate_voxelizer_equals_the_restatement(res):
+    """sim/init_conditions.voxelize_triangle (coordinate triangles) against the
+    restatement of trimesh's indexed subdivision: the same voxels."""
+    from oracle import voxel_ref as V
+    from sim.init_conditions import voxelize_triangle
+    dx = 1.0 / res
+    a, lo_a = voxelize_triangle(_plate(), dx)
+    b, lo_b = V.voxelize_subdivide(_plate(), np.array([[0, 1, 2]]), dx)
+    assert np.array_equal(a, b) and np.array_equal(lo_a, lo_b)
