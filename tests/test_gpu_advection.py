@@ -301,3 +301,26 @@ def face_coords_np(a, n, dx):
     i, j, k = np.meshgrid(*[np.arange(m) for m in d], indexing="ij")
     o = [0.0 if a == b else 0.5 for b in range(3)]
     return np.stack([(i + o[0]) * dx, (j + o[1]) * dx, (k + o[2]) * dx]).astype(np.float32)
+
+
+def test_primitives_against_golden_vectors(dev):
+    """The HIP B-spline, MAC velocity and full-Hermite evaluations against the
+    brute-force golden vectors (tests/golden/make_golden.py), FP32 tolerances."""
+    import os
+    from edge_hip import ops
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "interp_golden.npz")
+    with np.load(path, allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    dx, pts = float(g["dx"]), g["pts"]
+    n = g["psi"].shape[1:]
+    for name, BL in (("own_x", (0.0, 0.5, 0.5)), ("cell", (0.5, 0.5, 0.5))):
+        v, gr = ops.eval_bspline(T_(g["f"], dev), BL, T_(pts, dev), dx, grad=True)
+        assert scale_err(N_(v), g[f"bspline_{name}_val"]) < 1e-5
+        assert scale_err(N_(gr), g[f"bspline_{name}_grad"]) < 1e-5
+    v, J = ops.eval_umac([T_(g[f"u{a}"], dev) for a in range(3)], T_(pts, dev), dx, grad=True)
+    assert scale_err(N_(v), g["umac_val"]) < 1e-5
+    assert scale_err(N_(J), g["umac_jac"]) < 1e-5
+    tab = ops.pack_table("full", T_(g["psi"], dev), T_(g["T"], dev), T_(g["dT"], dev), T_(g["d2T"], dev), dx)
+    v, G = ops.eval_map(tab, n, T_(pts, dev), dx, grad=True)
+    assert scale_err(N_(v), g["hermite_val"]) < 1e-5
+    assert scale_err(N_(G), g["hermite_grad"]) < 1e-5
