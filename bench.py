@@ -6,9 +6,10 @@ CFL 0.5, BFECC clamp, projection to ||r|| <= 1e-6 ||b|| (FP64 norms).  A step
 is one iteration of the reference's loop (run_edge.py:588-711): semi-Lagrangian
 half step + projection, EDGE march, face march, BFECC pull-back, projection.
 
-Prints ONE JSON line: ms/step of the mixed-precision path (value), the plain
-FP32 path beside it, Poisson ms and iterations of both on the same right-hand
-sides, per-phase times, the roofline of the dominant kernel (HIP events on the
+Prints ONE JSON line: ms/step with the chosen projection solver (value; by
+default "dense": FP32 multigrid with an FP64 dense coarsest solve), the plain
+FP32 reference solver beside it, Poisson ms and iterations of plain, mixed
+(BF16 multigrid) and dense on the same right-hand sides, per-phase times, the roofline of the dominant kernel (HIP events on the
 launch stream) and the CPU baseline (the oracle restatement, timed on this
 host's cores on one 64x32x32 step and scaled by the cell count).
 
@@ -39,19 +40,25 @@ SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 16550.0, "ge": 4430.
 # the face march (RK4 of a face point and its column): per face and velocity in EFM's backtrack
 FACE_MARCH_FLOP_PER_FACE = 2860.0
 SCHEME_BYTES_PER_CELL = {"edge": 180.0, "ed8": 204.0, "ge": 324.0}
-MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0}  # one CG iteration
+MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0, "dense": 145.0}  # one CG iteration
 MARCH_BYTES_PER_CELL = 180.0       # u 12 + old map 84 + new map 84 (compulsory FP32)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
-def workload_name(res, case, scheme="edge"):
-    if scheme != "edge":
-        return f"{scheme.upper()} step {res[0]}x{res[1]}x{res[2]} four rings, mixed projection"
-    if case == "cfg3":
-        return f"EDGE step {res[0]}x{res[1]}x{res[2]} delta-wing plate, oblique inflow, solid mixed projection"
-    if case == "cfg5":
-        return f"EDGE step {res[0]}x{res[1]}x{res[2]} one ring + laden particles, mixed projection"
-    return f"EDGE step {res[0]}x{res[1]}x{res[2]} four rings, mixed projection"
+def march_workload(res, case, scheme="edge"):
+    """The map march's workload (what the march traffic counters depend on)."""
+    what = {"cfg3": "delta-wing plate, oblique inflow", "cfg5": "one ring + laden particles"}.get(case, "four rings")
+    return f"{scheme.upper()} step {res[0]}x{res[1]}x{res[2]} {what}"
+
+
+def workload_name(res, case, scheme="edge", solver="dense"):
+    solid = "solid " if case == "cfg3" else ""
+    return f"{march_workload(res, case, scheme)}, {solid}{solver} projection"
+
+
+SOLVER_NOTE = {"plain": "plain (FP32 multigrid, the reference V-cycle with 20 bottom pairs, FP64 dots)",
+               "mixed": "mixed (BF16 multigrid, FP64 dots, dense FP64 coarsest)",
+               "dense": "dense (FP32 multigrid, FP64 dots, dense FP64 coarsest)"}
 
 
 def read_pmc(d, counter):
@@ -97,6 +104,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--poisson-reps", type=int, default=3, help="0 skips the same-RHS solver comparison")
     ap.add_argument("--no-io", action="store_true", help="skip timing one frame output and one checkpoint")
+    ap.add_argument("--solver", default="dense", choices=["plain", "mixed", "dense"],
+                    help="projection solver of the timed steps")
     ap.add_argument("--case", default="cfg2", choices=["cfg2", "cfg3", "cfg5"],
                     help="cfg2: the headline EDGE step; cfg3: delta-wing plate with the solid projection; "
                          "cfg5: 512x256x256 with 16M laden particles")
@@ -151,9 +160,9 @@ def make_solver(sim, mode, warm_start=None):
 
 
 def poisson_compare(sim, reps, torch):
-    """Plain vs mixed (vs learned warm start + mixed, when the trained network is
-    present) projection of the same velocities (the inputs of one step's two
-    projections), to ||r|| <= 1e-6 ||b||."""
+    """Plain vs mixed vs dense (vs learned warm start + mixed, when the trained
+    network is present) projection of the same velocities (the inputs of one
+    step's two projections), to ||r|| <= 1e-6 ||b||."""
     import math
     sim.rhs_snapshots = []
     sim.substep()
@@ -162,7 +171,7 @@ def poisson_compare(sim, reps, torch):
     res = {}
     from sim.learned import load_default
     net = load_default()
-    modes = ("plain", "mixed") + (("learned",) if net is not None else ())
+    modes = ("plain", "mixed", "dense") + (("learned",) if net is not None else ())
     for mode in modes:
         solver = make_solver(sim, mode, warm_start=net if mode == "learned" else None)
         per = []
@@ -190,6 +199,7 @@ def poisson_compare(sim, reps, torch):
                          iters=sum(p["iters"] for p in per))
         del solver
     res["speedup_mixed_vs_plain"] = round(res["plain"]["ms"] / res["mixed"]["ms"], 3)
+    res["speedup_dense_vs_plain"] = round(res["plain"]["ms"] / res["dense"]["ms"], 3)
     if "learned" in res:
         res["speedup_learned_vs_plain"] = round(res["plain"]["ms"] / res["learned"]["ms"], 3)
         res["learned_note"] = "mixed solver started from the U-Net's p0 (inference included in ms)"
@@ -207,11 +217,13 @@ def io_timing(sim, torch):
     d = tempfile.mkdtemp(prefix="edge_io_")
     try:
         wn, sm, _, _ = sim._frame_buffers()
+        args = (sim.u_x.ptr, sim.u_y.ptr, sim.u_z.ptr, sim.smoke.ptr, 4, wn.data_ptr(), sm.data_ptr(), sim.res_x,
+                sim.res_y, sim.res_z, float(sim.dx), stream_handle())
+        _lib.call("edge_vtk_frame", *args)   # first launch (code object load) untimed
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()   # the frame kernel alone (|curl u| and smoke in VTK order), on torch's stream
-        _lib.call("edge_vtk_frame", sim.u_x.ptr, sim.u_y.ptr, sim.u_z.ptr, sim.smoke.ptr, 4, wn.data_ptr(),
-                  sm.data_ptr(), sim.res_x, sim.res_y, sim.res_z, float(sim.dx), stream_handle())
+        _lib.call("edge_vtk_frame", *args)
         e1.record()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -257,7 +269,7 @@ def main():
     a = parse()
     if a.summarize_pmc:
         rec = summarize_pmc(*a.summarize_pmc, probe_mib=a.copy_probe or 1024)
-        rec["k_march_edge"]["workload"] = workload_name(a.res, a.case)
+        rec["k_march_edge"]["workload"] = march_workload(a.res, a.case)
         with open(TRAFFIC_FILE, "w") as fh:
             json.dump(rec, fh, indent=1)
         print(json.dumps(rec))
@@ -270,7 +282,7 @@ def main():
     from run_edge import Simulation
 
     res = tuple(a.res) if (a.case != "cfg5" or a.res != [256, 128, 128]) else (512, 256, 256)
-    sim = Simulation(a.case, solver_mode="mixed", stop="relative", rel_tol=1e-6, res=res, check_every=0,
+    sim = Simulation(a.case, solver_mode=a.solver, stop="relative", rel_tol=1e-6, res=res, check_every=0,
                      num_particles=a.particles, scheme=a.scheme)
     sim.initialize()
     for _ in range(a.warmup):
@@ -291,23 +303,23 @@ def main():
     ms_step = total_ms / a.steps
     ph = {k: statistics.median(p.get(k, 0.0) for p in phases) for k in phases[0]}
     pstats = sim.poisson_stats[-2 * a.steps:]
-    mixed_iters = [s["iters"] for s in pstats]
+    step_iters = [s["iters"] for s in pstats]
 
     plain = None
     if not a.no_plain:
-        mixed_solver = sim.solver
+        step_solver = sim.solver
         sim.solver = make_solver(sim, "plain")
         run_steps(sim, 1, torch)  # warm the plain solver
         pw, pph = run_steps(sim, a.steps, torch)
         plain = dict(ms_per_step=round(statistics.mean(pw), 3), step_ms=[round(w, 3) for w in pw],
                      phases_ms={k: round(statistics.median(p.get(k, 0.0) for p in pph), 3) for k in pph[0]},
                      poisson_iters=[s["iters"] for s in sim.poisson_stats[-2 * a.steps:]])
-        sim.solver = mixed_solver
+        sim.solver = step_solver
 
     pcmp = poisson_compare(sim, a.poisson_reps, torch) if a.poisson_reps > 0 else None
 
     ncell = sim.res_x * sim.res_y * sim.res_z
-    wl = workload_name(res, a.case, a.scheme)
+    wl = workload_name(res, a.case, a.scheme, a.solver)
     # the dominant kernel: the fused EDGE march, one launch per step, timed per launch with
     # HIP events on its stream; FP32 vector-ALU bound (no matrix op on this path)
     march_ms = statistics.mean(march_launch_ms)
@@ -326,7 +338,7 @@ def main():
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and a.scheme == "edge":
         rec = json.load(open(TRAFFIC_FILE)).get("k_march_edge", {})
-        if rec.get("workload") == wl:
+        if rec.get("workload") == march_workload(res, a.case, a.scheme):
             traffic = rec.get("hbm_bytes_per_launch")
     roof_march = dict(bound="mfma", pipe="FP32 vector ALU; 157.3 TFLOP/s is MI355X's FP32 peak",
                       achieved=round(march_tflops, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
@@ -335,12 +347,12 @@ def main():
                       avg_launch_ms=round(march_ms, 4), launches=len(march_launch_ms))
     # the projection's CG iteration (many kernels), HBM-bound: bytes per iteration / ms per iteration
     if pcmp is not None:
-        it_ms = pcmp["mixed"]["ms"] / max(pcmp["mixed"]["iters"], 1)   # includes per-solve setup
+        it_ms = pcmp[a.solver]["ms"] / max(pcmp[a.solver]["iters"], 1)   # includes per-solve setup
     else:
-        it_ms = (ph.get("poisson1", 0.0) + ph.get("poisson2", 0.0)) / max(sum(mixed_iters[-2:]), 1)
-    mg_gbs = MG_BYTES_PER_CELL["mixed"] * ncell / (it_ms * 1e-3) / 1e9
+        it_ms = (ph.get("poisson1", 0.0) + ph.get("poisson2", 0.0)) / max(sum(step_iters[-2:]), 1)
+    mg_gbs = MG_BYTES_PER_CELL[a.solver] * ncell / (it_ms * 1e-3) / 1e9
     roof_mg = dict(bound="hbm", achieved=round(mg_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                   frac=round(mg_gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="mixed CG iteration (22 launches)",
+                   frac=round(mg_gbs / HBM_PEAK_GBS, 4), traffic=None, kernel=f"{a.solver} CG iteration (22 launches)",
                    time_ms=round(it_ms, 4))
 
     if a.copy_probe:
@@ -374,11 +386,11 @@ def main():
                  "cfg5": "synthetic: one vortex ring, particles uniform in its torus (numpy default_rng(0))"}[a.case],
         "config": {"workload": wl, "cells": ncell, "reinit_every": sim.reinit_every,
                    "poisson_stop": "||r|| <= 1e-6 ||b||",
-                   "solver": "mixed (BF16 multigrid, FP64 dots, dense FP64 coarsest)",
+                   "solver": SOLVER_NOTE[a.solver],
                    **({"particles": sim.laden_particle_num} if sim.lp is not None else {})},
         "phases_ms": {k: round(v, 3) for k, v in ph.items()},
         "step_ms": [round(w, 3) for w in walls],
-        "poisson_iters_mixed": mixed_iters,
+        "poisson_iters": step_iters,
         "plain": plain,
         "poisson_same_rhs": pcmp,
         "roofline": roof_march,

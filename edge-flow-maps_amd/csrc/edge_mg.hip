@@ -1,14 +1,17 @@
 // Multigrid-preconditioned CG pressure projection (sim/mgpcg.py,
 // sim/mgpcg_solid.py) on gfx950.
 //
-// Two storage modes share one algorithm skeleton:
+// Three modes share one algorithm skeleton:
 //  * EDGE_MG_PLAIN (0): the reference V-cycle (2 pre/post red-black
 //    Gauss-Seidel half-sweep pairs, 20 bottom pairs, restriction r[l+1] +=
 //    res/2, injection prolongation; sim/mgpcg.py:147-195) in FP32.
 //  * EDGE_MG_MIXED (1): the same smoother and transfers with the multigrid
 //    levels stored in BF16 (FP32 arithmetic) and the 20 bottom pairs replaced
 //    by a dense FP64 solve of the coarsest level.
-// Both keep x, p, Ap and the CG residual in FP32, do every inner product in
+//  * EDGE_MG_DENSE (2): FP32 levels with that dense coarsest solve (the
+//    level-0 passes are latency-bound, so BF16's halved bytes buy ~7% per
+//    iteration while its rounding costs about one CG iteration in twelve).
+// All keep x, p, Ap and the CG residual in FP32, do every inner product in
 // FP64 with a deterministic two-level reduction (per-block partials, the last
 // block to finish sums them in a fixed order and updates the CG scalars on the
 // device), and never read a scalar back except for the convergence flag.
@@ -1667,7 +1670,7 @@ struct Solver {
 // dedicated bottom kernel is faster than the tail's generic sweeps).
 int tail_level(const Solver& s) {
   if (!s.tail_from) return 0;
-  if (!(s.mode == 1 && s.dense_ok) || s.tail_from == s.L - 1) return 0;
+  if (!(s.mode >= 1 && s.dense_ok) || s.tail_from == s.L - 1) return 0;
   return s.tail_from;
 }
 
@@ -1812,7 +1815,7 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
       hipLaunchKernelGGL(k_restrict<S>, dim3(blocks_for(s.nl[l + 1], BS)), dim3(BS), 0, st, s.lv[l], s.lv[l + 1], s.bt,
                          rl, zl, (S*)s.rl[l + 1], (S*)s.zl[l + 1], sc, ig);
   }
-  const bool dense = s.mode == 1 && s.dense_ok;
+  const bool dense = s.mode >= 1 && s.dense_ok;
   if (lt) {
     int lds = 0;
     Tail T = tail_plan(s, sizeof(S), &lds);
@@ -1998,7 +2001,7 @@ int prepare_masks(Solver& s, hipStream_t st) {
     hipLaunchKernelGGL(k_nbmask, dim3(blocks_for(s.nl[l], BS)), dim3(BS), 0, st, s.lv[l], s.nbm[l]);
     EDGE_CHECK_LAUNCH();
   }
-  if (s.mode == 1) {
+  if (s.mode >= 1) {
     // the dense coarse inverse depends on the coarse mask
     std::vector<int8_t> cm(s.nl[s.L - 1]);
     EDGE_CHECK_HIP(hipMemcpyAsync(cm.data(), s.bm[s.L - 1], cm.size(), hipMemcpyDeviceToHost, st));
@@ -2174,7 +2177,8 @@ int start_solve(Solver& s, hipStream_t st, const float* x0) {
 extern "C" {
 
 // Create a solver for an nx x ny x nz all-cell Poisson problem.
-// mode 0 plain (FP32, reference V-cycle), 1 mixed (BF16 levels, dense FP64 bottom).
+// mode 0 plain (FP32, reference V-cycle), 1 mixed (BF16 levels, dense FP64 bottom),
+// 2 dense (FP32 levels, dense FP64 bottom).
 // bt: host int[6] boundary_types (row-major [axis][side], 2 = wall), or NULL for all walls.
 int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, void** handle) {
   clear_error();
@@ -2182,7 +2186,7 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
   *handle = nullptr;
   EDGE_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2, "edge_mg_create: bad dims");
   EDGE_REQUIRE(levels >= 2 && levels <= MAXL, "edge_mg_create: levels must be in 2..%d", MAXL);
-  EDGE_REQUIRE(mode == 0 || mode == 1, "edge_mg_create: mode must be 0 (plain) or 1 (mixed)");
+  EDGE_REQUIRE(mode >= 0 && mode <= 2, "edge_mg_create: mode must be 0 (plain), 1 (mixed) or 2 (dense)");
   for (int l = 0; l < levels; ++l) {
     EDGE_REQUIRE(((nx >> l) << l) == nx && ((ny >> l) << l) == ny && ((nz >> l) << l) == nz,
                  "edge_mg_create: dims must be divisible by 2^(levels-1)");
@@ -2224,7 +2228,7 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
   if (hipMalloc(&s->p, s->N * 4) != hipSuccess) return fail("hipMalloc p");
   if (hipMalloc(&s->Ap, s->N * 4) != hipSuccess) return fail("hipMalloc Ap");
   for (int l = 0; l < levels; ++l) {
-    if (l == 0 && mode == 0) {
+    if (l == 0 && mode != 1) {
       s->rl[0] = s->r;
     } else if (hipMalloc(&s->rl[l], s->nl[l] * s->elem) != hipSuccess) {
       return fail("hipMalloc level r");
@@ -2241,7 +2245,7 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
   if (s->nl[levels - 1] * 3 * 4 > 160 * 1024 && mode == 0) {
     // bottom falls back to per-sweep launches; fine
   }
-  if (mode == 1) {
+  if (mode >= 1) {
     std::vector<int8_t> cm(s->nl[levels - 1], 0);
     s->dense_ok = build_dense(*s, cm);
   }
@@ -2478,7 +2482,7 @@ int edge_mg_set_tiles(void* handle, int enable) {
 // 1 when the mixed solver's dense coarsest solve is active (0: red-black bottom).
 int edge_mg_dense_bottom(void* handle) {
   if (!handle) return 0;
-  return ((Solver*)handle)->dense_ok && ((Solver*)handle)->mode == 1 ? 1 : 0;
+  return ((Solver*)handle)->dense_ok && ((Solver*)handle)->mode >= 1 ? 1 : 0;
 }
 
 }  // extern "C"

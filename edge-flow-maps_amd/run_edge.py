@@ -606,7 +606,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("-c", "--case", default="cfg1")
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--solver", default="plain", choices=["plain", "mixed"])
+    ap.add_argument("--solver", default="plain", choices=["plain", "mixed", "dense"])
     ap.add_argument("--res", type=int, nargs=3, default=None)
     ap.add_argument("--particles", type=int, default=None, help="particle count of a particle case")
     ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge", "efm"])

@@ -3,15 +3,16 @@
 MGPCG_3 keeps the reference's constructor and Poisson() interface
 (sim/mgpcg.py:364-491); the solver runs in libedge_hip.so.  Extensions, all
 keyword-only with defaults that reproduce the reference:
-  mode        "plain" (FP32, the reference V-cycle) or "mixed" (BF16 multigrid
+  mode        "plain" (FP32, the reference V-cycle), "mixed" (BF16 multigrid
+              storage, FP64 dense coarsest solve) or "dense" (FP32 multigrid
               storage, FP64 dense coarsest solve)
   stop        "reference" (rTr < max(abs_tol, rel_tol * z0'r0), sim/mgpcg.py:228)
               or "relative" (||r|| <= rel_tol ||b||, FP64 norms)
   check_every CG iterations between host convergence checks (the reference
               reads a scalar every iteration: 1; at most 8), or 0 for batches
-              sized from the last two solves (default for "mixed")
+              sized from the last two solves (default for "mixed", "dense")
   graph       replay each batch of CG iterations as captured HIP graphs
-              (default: on for "mixed", off for "plain")
+              (default: on for "mixed" and "dense", off for "plain")
   x0          warm start for Poisson(), used only if ||b - A x0|| <= ||b||
   warm_start  a learned initial guess (sim.learned.UNet, or True for the
               committed network): Poisson() computes b, runs the network
@@ -40,6 +41,9 @@ def _bt_array(boundary_types):
     return (ctypes.c_int * 6)(*bt.tolist())
 
 
+_MODES = {"plain": 0, "mixed": 1, "dense": 2}
+
+
 class _MGBase:
     def __init__(self, boundary_types, N, base_level=3, levels=None, mode="plain", stop="reference",
                  check_every=None, abs_tol=1e-12, rel_tol=1e-12, graph=None, warm_start=None):
@@ -54,9 +58,11 @@ class _MGBase:
         self.boundary_types = boundary_types
         self._bt = _bt_array(boundary_types)
         h = ctypes.c_void_p()
-        _lib.call("edge_mg_create", *self.N, self.n_mg_levels, 0 if mode == "plain" else 1, self._bt, ctypes.byref(h))
+        if mode not in _MODES:
+            raise ValueError(f"mode must be one of {sorted(_MODES)}, got {mode!r}")
+        _lib.call("edge_mg_create", *self.N, self.n_mg_levels, _MODES[mode], self._bt, ctypes.byref(h))
         self._h = h
-        self.graph = (mode == "mixed") if graph is None else bool(graph)
+        self.graph = (mode != "plain") if graph is None else bool(graph)
         _lib.call("edge_mg_set_graph", self._h, int(self.graph))
         self.p = Field(None, float, self.N)
         self.stats = {}
@@ -136,7 +142,7 @@ class _MGBase:
 
     def get_memory_usage(self):
         n = math.prod(self.N)
-        elem = 4 if self.mode == "plain" else 2
+        elem = 2 if self.mode == "mixed" else 4
         levels = sum(n // 8 ** l for l in range(self.n_mg_levels))
         return n * 4 * 5 + levels * elem * 2
 

@@ -167,7 +167,8 @@ int edge_split_central_vector(const float* vc, float* ux, float* uy, float* uz, 
 /* Solver for an nx*ny*nz cell grid, `levels` multigrid levels (dims divisible
  * by 2^(levels-1), nz even on every level).  mode 0 plain: FP32 storage and
  * the reference V-cycle (sim/mgpcg.py:23-57,147-195); mode 1 mixed: BF16
- * level storage, FP32 arithmetic, dense FP64 coarsest solve.  bt: host
+ * level storage, FP32 arithmetic, dense FP64 coarsest solve; mode 2 dense:
+ * FP32 level storage with the dense FP64 coarsest solve.  bt: host
  * int[6] boundary_types ([axis][side], 2 = wall) or NULL for all walls.
  * Replaces MGPCG_3.__init__ (sim/mgpcg.py:366-371) and MGPCG_Solid_3.__init__
  * (sim/mgpcg_solid.py:315-324).  The handle owns its hierarchy. */

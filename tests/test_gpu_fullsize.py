@@ -45,14 +45,14 @@ def test_full_size_step_against_the_oracle(dev):
 
 
 def test_full_size_mixed_steps_are_deterministic_and_converged(dev):
-    """Two mixed-projection steps twice from the same initial state: identical
+    """Two steps with the measured solver ("dense") twice from the same initial state: identical
     bits (FP64 reductions in a fixed order, no atomics in this case), every
     projection at ||b - A p|| <= 1e-6 ||b|| (FP64 on the device), the map inside
     the domain, and the kinetic energy within 5%."""
     from run_edge import Simulation
     out = []
     for _ in range(2):
-        sim = Simulation("cfg2", solver_mode="mixed", res=RES, num_samples=400, check_every=0)
+        sim = Simulation("cfg2", solver_mode="dense", res=RES, num_samples=400, check_every=0)
         sim.initialize()
         e0 = sum(float((f.data.double() ** 2).sum()) for f in (sim.u_x, sim.u_y, sim.u_z))
         for _ in range(2):
