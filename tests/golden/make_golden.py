@@ -16,6 +16,10 @@ Writes tests/golden/interp_golden.npz (a few hundred KB).  Each output follows:
   * hermite_f/df/g/dg sim/hermite.py:4-41
   * interp_hermite / interp_grad_hermite
                       sim/hermite.py:459-607        8 corners, T*dx, dT*dx^2, d2T*dx^3
+  * interp_(grad_)hermite_garm_fd_unroll
+                      sim/hermite.py:298-456        d2T from central differences of dT
+  * interp_(grad_)hermite_fd_unroll
+                      sim/hermite.py:64-295         dT and d2T from central differences of T
 with eps = 1e-6 (util/taichi_utils.py:3).
 """
 import math
@@ -100,8 +104,40 @@ def hermite_dg(x):
     return 3 * a ** 2 - 4 * a + 1 if a < 1.0 else 0.0
 
 
-def interp_grad_hermite(psi_c, T_c, dT_c, d2T_c, p, dx):
-    """Component c: psi_c (n), T_c (n, 3), dT_c (n, 3), d2T_c (n); BL = 0.5."""
+def coef_full(T_c, dT_c, d2T_c, ii, jj, kk, dx):
+    """sim/hermite.py:459-512: the stored T, dT, d2T of corner (ii, jj, kk)."""
+    v1 = sample(T_c, ii, jj, kk) * dx
+    v2 = sample(dT_c, ii, jj, kk) * dx * dx
+    return v1, v2, sample(d2T_c, ii, jj, kk) * dx * dx * dx
+
+
+def coef_garm(T_c, dT_c, d2T_c, ii, jj, kk, dx):
+    """sim/hermite.py:298-355: T, dT stored; d2T from central differences of dT."""
+    g = lambda a, b, c, m: sample(dT_c, a, b, c)[m]   # noqa: E731
+    v3 = ((g(ii + 1, jj, kk, 0) - g(ii - 1, jj, kk, 0)) + (g(ii, jj + 1, kk, 1) - g(ii, jj - 1, kk, 1)) +
+          (g(ii, jj, kk + 1, 2) - g(ii, jj, kk - 1, 2))) / 6 * dx * dx
+    return sample(T_c, ii, jj, kk) * dx, sample(dT_c, ii, jj, kk) * dx * dx, v3
+
+
+def coef_fd(T_c, dT_c, d2T_c, ii, jj, kk, dx):
+    """sim/hermite.py:64-158: only T stored; dT and d2T from central differences of T."""
+    g = lambda a, b, c, m: sample(T_c, a, b, c)[m]   # noqa: E731
+    v2x = ((g(ii, jj, kk + 1, 1) - g(ii, jj, kk - 1, 1)) + (g(ii, jj + 1, kk, 2) - g(ii, jj - 1, kk, 2))) / 4 * dx
+    v2y = ((g(ii, jj, kk + 1, 0) - g(ii, jj, kk - 1, 0)) + (g(ii + 1, jj, kk, 2) - g(ii - 1, jj, kk, 2))) / 4 * dx
+    v2z = ((g(ii, jj + 1, kk, 0) - g(ii, jj - 1, kk, 0)) + (g(ii + 1, jj, kk, 1) - g(ii - 1, jj, kk, 1))) / 4 * dx
+    v30 = ((g(ii + 1, jj, kk + 1, 1) - g(ii + 1, jj, kk - 1, 1)) + (g(ii + 1, jj + 1, kk, 2) - g(ii + 1, jj - 1, kk, 2))) / 4
+    v31 = ((g(ii - 1, jj, kk + 1, 1) - g(ii - 1, jj, kk - 1, 1)) + (g(ii - 1, jj + 1, kk, 2) - g(ii - 1, jj - 1, kk, 2))) / 4
+    v32 = ((g(ii, jj + 1, kk + 1, 0) - g(ii, jj + 1, kk - 1, 0)) + (g(ii + 1, jj + 1, kk, 2) - g(ii - 1, jj + 1, kk, 2))) / 4
+    v33 = ((g(ii, jj - 1, kk + 1, 0) - g(ii, jj - 1, kk - 1, 0)) + (g(ii + 1, jj - 1, kk, 2) - g(ii - 1, jj - 1, kk, 2))) / 4
+    v34 = ((g(ii, jj + 1, kk + 1, 0) - g(ii, jj - 1, kk + 1, 0)) + (g(ii + 1, jj, kk + 1, 1) - g(ii - 1, jj, kk + 1, 1))) / 4
+    v35 = ((g(ii, jj + 1, kk - 1, 0) - g(ii, jj - 1, kk - 1, 0)) + (g(ii + 1, jj, kk - 1, 1) - g(ii - 1, jj, kk - 1, 1))) / 4
+    v3 = (v30 - v31 + v32 - v33 + v34 - v35) / 6 * dx
+    return sample(T_c, ii, jj, kk) * dx, np.array([v2x, v2y, v2z]), v3
+
+
+def interp_grad_hermite(psi_c, T_c, dT_c, d2T_c, p, dx, coef=coef_full):
+    """Component c: psi_c (n), T_c (n, 3), dT_c (n, 3), d2T_c (n); BL = 0.5.
+    `coef` gives the corner's T*dx, dT*dx^2, d2T*dx^3 (stored or derived)."""
     s, t, l = _clamped(psi_c, p, dx, (0.5, 0.5, 0.5))
     iu, iv, iw = math.floor(s), math.floor(t), math.floor(l)
     val = px = py = pz = 0.0
@@ -115,11 +151,10 @@ def interp_grad_hermite(psi_c, T_c, dT_c, d2T_c, p, dx):
                 ii, jj, kk = iu + i, iv + j, iw + k
                 # (coefficient, basis index per axis: 0 = f, 1 = g)
                 terms = [(sample(psi_c, ii, jj, kk), (0, 0, 0))]
-                v1 = sample(T_c, ii, jj, kk) * dx
+                v1, v2, v3 = coef(T_c, dT_c, d2T_c, ii, jj, kk, dx)
                 terms += [(v1[0], (1, 0, 0)), (v1[1], (0, 1, 0)), (v1[2], (0, 0, 1))]
-                v2 = sample(dT_c, ii, jj, kk) * dx * dx
                 terms += [(v2[0], (0, 1, 1)), (v2[1], (1, 0, 1)), (v2[2], (1, 1, 0))]
-                terms += [(sample(d2T_c, ii, jj, kk) * dx * dx * dx, (1, 1, 1))]
+                terms += [(v3, (1, 1, 1))]
                 for cf, (a, b, c) in terms:
                     val += cf * hx[a] * hy[b] * hz[c]
                     px += cf * hx[2 + a] * hy[b] * hz[c]
@@ -162,16 +197,18 @@ def main():
     dT = rng.normal(size=(3, 3) + n)
     d2T = rng.normal(size=(3,) + n)
     out.update(psi=psi, T=T, dT=dT, d2T=d2T)
-    hv = np.zeros((len(pts), 3))
-    hg = np.zeros((len(pts), 3, 3))
-    for c in range(3):
-        Tc = np.moveaxis(T[c], 0, -1)
-        dTc = np.moveaxis(dT[c], 0, -1)
-        for q, p in enumerate(pts):
-            v, g = interp_grad_hermite(psi[c], Tc, dTc, d2T[c], p, dx)
-            hv[q, c] = v
-            hg[q, c] = g
-    out["hermite_val"], out["hermite_grad"] = hv, hg
+    for kind, coef in (("full", coef_full), ("garm", coef_garm), ("fd", coef_fd)):
+        hv = np.zeros((len(pts), 3))
+        hg = np.zeros((len(pts), 3, 3))
+        for c in range(3):
+            Tc = np.moveaxis(T[c], 0, -1)
+            dTc = np.moveaxis(dT[c], 0, -1)
+            for q, p in enumerate(pts):
+                v, g = interp_grad_hermite(psi[c], Tc, dTc, d2T[c], p, dx, coef)
+                hv[q, c] = v
+                hg[q, c] = g
+        key = "hermite" if kind == "full" else f"hermite_{kind}"
+        out[f"{key}_val"], out[f"{key}_grad"] = hv, hg
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "interp_golden.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path}")

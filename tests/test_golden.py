@@ -1,8 +1,8 @@
 """The oracle against the golden vectors of tests/golden/ (a literal FP64
 brute force of the reference's interpolation loops, written by
 tests/golden/make_golden.py): quadratic B-spline value and gradient at both
-staggerings, the MAC velocity and its Jacobian, and the full tricubic Hermite
-with its gradient, at 60 points including grid nodes, corners and points
+staggerings, the MAC velocity and its Jacobian, and the tricubic Hermite (full,
+garm and fd variants) with its gradient, at 60 points including grid nodes, corners and points
 beyond the clamp."""
 import os
 
@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import np_ref
-from oracle.oracle import HK_FULL
+from oracle.oracle import HK_FD, HK_FULL, HK_GARM_FD
 
 G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "interp_golden.npz")
 
@@ -48,3 +48,12 @@ def test_hermite_golden(orc64, g):
     nv, ng = np_ref.hermite_full(g["psi"], g["T"], g["dT"], g["d2T"], g["pts"], float(g["dx"]), grad=True)
     assert rel(nv, g["hermite_val"]) < 1e-12
     assert rel(ng, g["hermite_grad"]) < 1e-12
+
+
+@pytest.mark.parametrize("key,hk", [("hermite_garm", HK_GARM_FD), ("hermite_fd", HK_FD)])
+def test_hermite_derived_variants_golden(orc64, g, key, hk):
+    """The garm (d2T from differences of dT) and fd (dT, d2T from differences of
+    T) variants the EDGE and ED4 marches use."""
+    v, Gr = orc64.eval_hermite(hk, g["psi"], g["T"], g["dT"], g["d2T"], g["pts"], float(g["dx"]), grad=True)
+    assert rel(v, g[f"{key}_val"]) < 1e-13
+    assert rel(Gr, g[f"{key}_grad"]) < 1e-13

@@ -320,7 +320,8 @@ def test_primitives_against_golden_vectors(dev):
     v, J = ops.eval_umac([T_(g[f"u{a}"], dev) for a in range(3)], T_(pts, dev), dx, grad=True)
     assert scale_err(N_(v), g["umac_val"]) < 1e-5
     assert scale_err(N_(J), g["umac_jac"]) < 1e-5
-    tab = ops.pack_table("full", T_(g["psi"], dev), T_(g["T"], dev), T_(g["dT"], dev), T_(g["d2T"], dev), dx)
-    v, G = ops.eval_map(tab, n, T_(pts, dev), dx, grad=True)
-    assert scale_err(N_(v), g["hermite_val"]) < 1e-5
-    assert scale_err(N_(G), g["hermite_grad"]) < 1e-5
+    for kind, key in (("full", "hermite"), ("garm", "hermite_garm"), ("fd", "hermite_fd")):
+        tab = ops.pack_table(kind, T_(g["psi"], dev), T_(g["T"], dev), T_(g["dT"], dev), T_(g["d2T"], dev), dx)
+        v, G = ops.eval_map(tab, n, T_(pts, dev), dx, grad=True)
+        assert scale_err(N_(v), g[f"{key}_val"]) < 1e-5, kind
+        assert scale_err(N_(G), g[f"{key}_grad"]) < 1e-5, kind
