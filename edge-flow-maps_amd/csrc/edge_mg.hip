@@ -986,9 +986,12 @@ __global__ void __launch_bounds__(BS) k_p_Ap8(Lv L, int bt, const S* __restrict_
 
 // p = z + beta p, x += alpha p, r -= alpha Ap, r.r and sum r (as k_update_xr).
 template <class S>
-__global__ void __launch_bounds__(BS) k_update_xr8(Lv L, const S* __restrict__ z, float* __restrict__ x,
+// zn (may equal z, hence no __restrict__ on either): when not null, also the next
+// V-cycle's fill(0) + first red half-sweep from the updated residual (dense mode:
+// FP32 levels whose level-0 r is the CG residual itself, no recentering).
+__global__ void __launch_bounds__(BS) k_update_xr8(Lv L, int bt, const S* z, float* __restrict__ x,
                                                   float* __restrict__ r, float* __restrict__ p,
-                                                  const float* __restrict__ Ap, Red R) {
+                                                  const float* __restrict__ Ap, S* zn, Red R) {
   EXIT_IF_DONE(R.sc);
   const float alpha = (float)R.sc[S_ALPHA];
   const float beta = (float)R.sc[S_BETA];
@@ -1017,6 +1020,18 @@ __global__ void __launch_bounds__(BS) k_update_xr8(Lv L, const S* __restrict__ z
     st8(p + q0, pv);
     st8(x + q0, xv);
     st8(r + q0, rv);
+    if (zn) {
+      int i, j, k0;
+      chunk_ijk(L, t_, i, j, k0);
+      float zo[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int k = k0 + c;
+        const uint32_t mb = nbc(M, c);
+        zo[c] = (((i + j + k) & 1) == 0 && !(mb & NB_SELF)) ? rv[c] / diag8(L, bt, i, j, k, mb) : 0.f;
+      }
+      st8(zn + q0, zo);
+    }
   }
   block_partial(R, s2, s1);
 }
@@ -1714,7 +1729,8 @@ void finish(Solver& s, hipStream_t st, int nb, Fin fin, int guard) {
 
 template <class S>
 struct Ops {
-  static void vcycle(Solver& s, hipStream_t st, int ignore_done);
+  // z_ready: level 0's z already holds the fill(0) + first red half-sweep
+  static void vcycle(Solver& s, hipStream_t st, int ignore_done, bool z_ready = false);
 };
 
 int bottom_lds_bytes(const Solver& s) { return (int)(s.nl[s.L - 1] * sizeof(float)); }
@@ -1781,12 +1797,14 @@ void last_dot(Solver& s, hipStream_t st, Fin fin, int ig, int guard) {
 }
 
 template <class S>
-void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig) {
+void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig, bool z_ready) {
   const int L = s.L;
   const double* sc = s.sc;
   S* r0 = (S*)s.rl[0];
   S* z0 = (S*)s.zl[0];
-  if (chunked(s.lv[0]))
+  if (z_ready) {
+    // k_update_xr8 of the previous iteration wrote z (dense mode)
+  } else if (chunked(s.lv[0]))
     hipLaunchKernelGGL(k_first08<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig);
   else
     hipLaunchKernelGGL(k_first0<S>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig);
@@ -2016,6 +2034,12 @@ int prepare_masks(Solver& s, hipStream_t st) {
 
 // The update half of a CG iteration: Ap, alpha, x, r, r.r and the stop test
 // (sim/mgpcg.py:234-259).
+// Dense mode on a chunked level 0: r needs no recentering (the dense coarsest
+// solve handles the constant), and rl[0] is r itself, so the update of r can
+// also write the next V-cycle's fill(0) + first red half-sweep, which saves
+// k_first08's pass over r and z.
+inline bool z_ready(const Solver& s) { return s.mode == 2 && chunked(s.lv[0]); }
+
 template <class S>
 void cg_update(Solver& s, hipStream_t st) {
   Red R = red(s);
@@ -2023,8 +2047,10 @@ void cg_update(Solver& s, hipStream_t st) {
   if (chunked(L0)) {
     hipLaunchKernelGGL(k_p_Ap8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
     finish(s, st, chunk_blocks(s.N), F_ALPHA, 1);
-    hipLaunchKernelGGL(k_update_xr8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, (const S*)s.zl[0], s.x, s.r, s.p,
-                       s.Ap, R);
+    // dense mode: the next V-cycle's first red half-sweep here (see z_ready)
+    S* zn = z_ready(s) ? (S*)s.zl[0] : nullptr;
+    hipLaunchKernelGGL(k_update_xr8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.x,
+                       s.r, s.p, s.Ap, zn, R);
     finish(s, st, chunk_blocks(s.N), F_XR, 1);
   } else {
     hipLaunchKernelGGL(k_p_Ap<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
@@ -2040,7 +2066,7 @@ void cg_update(Solver& s, hipStream_t st) {
 // batch that ends at convergence then launches nothing after the test.
 template <class S>
 void cg_iteration(Solver& s, hipStream_t st) {
-  Ops<S>::vcycle(s, st, 0);
+  Ops<S>::vcycle(s, st, 0, z_ready(s));
   last_dot<S>(s, st, F_ZR, 0, 1);
   cg_update<S>(s, st);
 }
