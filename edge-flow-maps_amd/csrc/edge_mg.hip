@@ -30,6 +30,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "edge_common.h"
@@ -2283,10 +2284,20 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
   if (s->tail_from) {
     int bytes = 0;
     for (int l = s->tail_from; l < levels; ++l) bytes += 2 * (int)s->nl[l] * (int)s->elem;
-    hipError_t e = mode == 1 ? hipFuncSetAttribute((const void*)k_vtail<__hip_bfloat16>,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, bytes)
-                             : hipFuncSetAttribute((const void*)k_vtail<float>,
-                                                   hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    // The attribute belongs to the kernel, not the solver: only ever raise it, so
+    // that a solver created later with a smaller tail cannot lower it below what
+    // an earlier solver launches with.
+    static std::mutex mu;
+    static int tail_attr[2] = {0, 0};
+    std::lock_guard<std::mutex> lock(mu);
+    const int ty = mode == 1 ? 1 : 0;
+    hipError_t e = hipSuccess;
+    if (bytes > tail_attr[ty]) {
+      e = ty ? hipFuncSetAttribute((const void*)k_vtail<__hip_bfloat16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   bytes)
+             : hipFuncSetAttribute((const void*)k_vtail<float>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+      if (e == hipSuccess) tail_attr[ty] = bytes;
+    }
     if (e != hipSuccess) s->tail_from = 0;  // per-level kernels instead
   }
   {

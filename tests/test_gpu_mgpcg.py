@@ -355,3 +355,15 @@ def test_dense_coarsest_solve_is_the_fp64_inverse(dev, mode, tol):
     nrm = np.linalg.norm(mf(want))
     assert np.linalg.norm(mf(z20) - mf(want)) > 0.2 * nrm
     assert np.linalg.norm(mf(z) - mf(want)) <= tol * nrm   # BF16 levels (mode 1) or FP32 (mode 2)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_a_smaller_solver_created_later_leaves_the_first_working(dev, mode):
+    # both hierarchies end in the one-workgroup tail; the second one's tail needs
+    # less dynamic LDS, and must not lower the kernel's limit below the first one's
+    u = smooth_velocity(*N3, DX, seed=21)
+    A = Solver(N3, 3, mode)
+    B = Solver((16, 16, 8), 3, mode)
+    p, st, code = A.poisson([T_(c, dev) for c in u], stop_mode=1, rel_tol=1e-6)
+    assert code == 0 and st[5] == 1.0
+    del B
