@@ -10,8 +10,8 @@ Prints ONE JSON line: ms/step with the chosen projection solver (value; by
 default "dense": FP32 multigrid with an FP64 dense coarsest solve), the plain
 FP32 reference solver beside it, Poisson ms and iterations of plain, mixed
 (BF16 multigrid) and dense on the same right-hand sides, per-phase times, the roofline of the dominant kernel (HIP events on the
-launch stream) and the CPU baseline (the oracle restatement, timed on this
-host's cores on one 64x32x32 step and scaled by the cell count).
+launch stream) and the CPU baseline (the oracle restatement running one step
+of the same workload on this host's cores).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--res NX NY NZ]
 """
@@ -245,24 +245,25 @@ def io_timing(sim, torch):
         shutil.rmtree(d, ignore_errors=True)
 
 
-def cpu_baseline(torch):
-    """The oracle (CPU restatement) of one full step at 64x32x32, scaled x(cells) to 256x128x128."""
-    from run_edge import Simulation
-    from oracle.step import OracleStep
+def cpu_baseline(sim):
+    """The oracle (the CPU restatement, OpenMP) running one full step of the
+    measured workload itself, from the simulation's current state: about 20 s of
+    CPU work on 16 threads, no scaling."""
     import numpy as np
-    small = Simulation("cfg2", solver_mode="plain", res=(64, 32, 32), num_samples=500)
-    small.initialize()
-    u = small.velocity_numpy()
-    smoke = np.moveaxis(small.smoke.to_numpy(), -1, 0)
+    from oracle.step import OracleStep
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    st = OracleStep(u, smoke, small.dx, 12, 0.5, small.ed_eps, levels=3, BFECC_clamp=True)
+    u = sim.velocity_numpy()
+    smoke = np.moveaxis(sim.smoke.to_numpy(), -1, 0)
+    st = OracleStep(u, smoke, sim.dx, sim.reinit_every, sim.CFL, sim.ed_eps, levels=sim.solver.n_mg_levels,
+                    BFECC_clamp=sim.BFECC_clamp, stop_mode=1, rel_tol=1e-6)
     t0 = time.perf_counter()
     st.substep()
     dt = time.perf_counter() - t0
-    scale = (256 * 128 * 128) / (64 * 32 * 32)
-    return dict(value=round(dt * 1e3 * scale, 1), unit="ms/step", cores=threads, kind="port",
-                sample=f"one full oracle step (advection + 2 projections, FP32, OpenMP) at 64x32x32 = "
-                       f"{dt:.2f} s, scaled x{scale:.0f} by cell count to 256x128x128")
+    n = f"{sim.res_x}x{sim.res_y}x{sim.res_z}"
+    return dict(value=round(dt * 1e3, 1), unit="ms/step", cores=threads, kind="port",
+                sample=f"one full oracle step (reinitialisation, advection, EDGE march, pull-backs, two "
+                       f"projections to 1e-6; FP32, OpenMP) of the measured {n} workload from the timed "
+                       f"run's state, {dt:.1f} s, unscaled")
 
 
 def main():
@@ -367,7 +368,7 @@ def main():
         del src, dst
 
     io_rec = None if a.no_io else io_timing(sim, torch)
-    cpu = None if a.no_cpu else cpu_baseline(torch)
+    cpu = None if (a.no_cpu or a.case != "cfg2" or a.scheme != "edge") else cpu_baseline(sim)
 
     line = {
         "metric": "ms/step @256x128x128; Poisson ms+iters to 1e-6 rel, learned/mixed vs plain",
