@@ -294,8 +294,8 @@ __global__ void __launch_bounds__(256) k_max_speed(Grid g, Mac U, unsigned int* 
 __global__ void k_set_u32(unsigned int* p, unsigned int v) { *p = v; }
 
 // util/taichi_utils.py:12-15 add_fields (dest = f1 + m f2) and 6-9 scale_field.
-__global__ void __launch_bounds__(256) k_axpy(const float* __restrict__ f1, const float* __restrict__ f2,
-                                              float* __restrict__ dest, float m, int64_t n) {
+// The step calls them in place (dest == f1, r == a): no __restrict__ here.
+__global__ void __launch_bounds__(256) k_axpy(const float* f1, const float* f2, float* dest, float m, int64_t n) {
   int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (t + 3 < n) {
     float4 a = *(const float4*)(f1 + t), b = *(const float4*)(f2 + t);
@@ -306,8 +306,7 @@ __global__ void __launch_bounds__(256) k_axpy(const float* __restrict__ f1, cons
   }
 }
 
-__global__ void __launch_bounds__(256) k_scale(const float* __restrict__ a, float alpha, float* __restrict__ r,
-                                               int64_t n) {
+__global__ void __launch_bounds__(256) k_scale(const float* a, float alpha, float* r, int64_t n) {
   int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (t + 3 < n) {
     float4 v = *(const float4*)(a + t);
@@ -315,6 +314,28 @@ __global__ void __launch_bounds__(256) k_scale(const float* __restrict__ a, floa
     *(float4*)(r + t) = o;
   } else {
     for (; t < n; ++t) r[t] = alpha * a[t];
+  }
+}
+
+// add_fields then scale_field in one pass: dest = s (f1 + m f2), rounded as the
+// two calls round (f1 + m f2 to FP32, then the product).
+__global__ void __launch_bounds__(256) k_axpy_scale(const float* f1, const float* f2, float* dest, float m, float sc,
+                                                    int64_t n) {
+  int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  if (t + 3 < n) {
+    float4 a = *(const float4*)(f1 + t), b = *(const float4*)(f2 + t);
+    float4 r;
+    r.x = a.x + m * b.x;
+    r.y = a.y + m * b.y;
+    r.z = a.z + m * b.z;
+    r.w = a.w + m * b.w;
+    r = {sc * r.x, sc * r.y, sc * r.z, sc * r.w};
+    *(float4*)(dest + t) = r;
+  } else {
+    for (; t < n; ++t) {
+      const float v = f1[t] + m * f2[t];
+      dest[t] = sc * v;
+    }
   }
 }
 
@@ -640,6 +661,18 @@ int edge_axpy(const float* f1, const float* f2, float* dest, float m, long n, vo
   if (n == 0) return EDGE_OK;
   hipLaunchKernelGGL(k_axpy, dim3(blocks_for((n + 3) / 4, 256)), dim3(256), 0, (hipStream_t)stream, f1, f2, dest, m,
                      (int64_t)n);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_axpy_scale(const float* f1, const float* f2, float* dest, float m, float s, long n, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(f1 && f2 && dest && n >= 0, "edge_axpy_scale: bad arguments");
+  EDGE_REQUIRE((((uintptr_t)f1 | (uintptr_t)f2 | (uintptr_t)dest) & 15) == 0,
+               "edge_axpy_scale: pointers not 16-byte aligned");
+  if (n == 0) return EDGE_OK;
+  hipLaunchKernelGGL(k_axpy_scale, dim3(blocks_for((n + 3) / 4, 256)), dim3(256), 0, (hipStream_t)stream, f1, f2,
+                     dest, m, s, (int64_t)n);
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }

@@ -48,6 +48,7 @@ SIGNATURES = {
     "edge_max_speed": "ppppiiip",
     "edge_axpy": "pppflp",
     "edge_scale": "pfplp",
+    "edge_axpy_scale": "pppfflp",
     "edge_copy": "pplp",
     "edge_fill": "pflp",
     "edge_vortex_ring": "fffpppppppiiiip",

@@ -43,7 +43,7 @@ from sim.init_conditions import (init_delta, init_four_vorts, init_rings, init_v
 from sim.mgpcg import MGPCG_3  # noqa: E402
 from sim.mgpcg_solid import MGPCG_Solid_3  # noqa: E402
 from util import io_utils  # noqa: E402
-from util.taichi_utils import (add_fields, center_coords_func, curl, get_central_vector, scale_field,  # noqa: E402
+from util.taichi_utils import (add_fields, add_scale_fields, center_coords_func, curl, get_central_vector,  # noqa: E402
                                split_central_vector, x_coords_func, y_coords_func, z_coords_func)
 
 CASES = {
@@ -367,10 +367,9 @@ class Simulation:
         pull_s(self.init_smoke, self.smoke)
         map_u(*U, *EU, self.F_x, self.F_y, self.F_z, self.phi_x, self.phi_y, self.phi_z, dx)
         map_smoke(self.smoke, self.err_smoke, self.phi_x, self.phi_y, self.phi_z, dx)
+        # run_edge.py:675-681: err = (err - init) * 0.5, one pass per field
         for e, i0 in zip(EU + (self.err_smoke,), IU + (self.init_smoke,)):
-            add_fields(e, i0, e, -1.)
-        for e in EU + (self.err_smoke,):
-            scale_field(e, 0.5, e)
+            add_scale_fields(e, i0, e, -1., 0.5)
         pull_u(EU, TU)
         pull_s(self.err_smoke, self.tmp_smoke)
         for u_, t_, e_ in zip(U, TU, EU):

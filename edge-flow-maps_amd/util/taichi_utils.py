@@ -25,6 +25,15 @@ def add_fields(f1, f2, dest, multiplier):
     touch(dest)
 
 
+def add_scale_fields(f1, f2, dest, multiplier, scale):
+    """add_fields(f1, f2, dest, multiplier) then scale_field(dest, scale, dest)
+    (run_edge.py:675-681) in one pass: the same values."""
+    assert f1.data.shape == f2.data.shape == dest.data.shape
+    _lib.call("edge_axpy_scale", as_ptr(f1), as_ptr(f2), as_ptr(dest), float(multiplier), float(scale), dest.numel,
+              stream_handle())
+    touch(dest)
+
+
 def _coords(mode, pf, dx, cell_shape):
     nx, ny, nz = cell_shape
     _lib.call("edge_coords", mode, as_ptr(pf), nx, ny, nz, float(dx), stream_handle())

@@ -139,6 +139,9 @@ int edge_max_speed(const float* ux, const float* uy, const float* uz, float* out
                    void* stream);
 /* add_fields (util/taichi_utils.py:12-15): dest = f1 + m f2. */
 int edge_axpy(const float* f1, const float* f2, float* dest, float m, long n, void* stream);
+/* No counterpart in the reference: add_fields then scale_field (run_edge.py:675-681,
+ * err = (err - init) * 0.5) in one pass, dest = s * (f1 + m * f2), the same values. */
+int edge_axpy_scale(const float* f1, const float* f2, float* dest, float m, float s, long n, void* stream);
 /* scale_field (util/taichi_utils.py:6-9). */
 int edge_scale(const float* a, float alpha, float* r, long n, void* stream);
 /* Taichi Field.copy_from / fill (run_edge.py:621-624, 644). */

@@ -325,3 +325,23 @@ def test_primitives_against_golden_vectors(dev):
         v, G = ops.eval_map(tab, n, T_(pts, dev), dx, grad=True)
         assert scale_err(N_(v), g[f"{key}_val"]) < 1e-5, kind
         assert scale_err(N_(G), g[f"{key}_grad"]) < 1e-5, kind
+
+
+def test_add_scale_fields_equals_add_then_scale(dev):
+    """run_edge.py:675-681's two passes (add_fields then scale_field, in place)
+    and the fused one give the same bits, including a length that is not a
+    multiple of 4."""
+    from edge_hip import ti
+    from util.taichi_utils import add_fields, add_scale_fields, scale_field
+    rng = np.random.default_rng(17)
+    for shape in [(33, 16, 16), (7, 5, 3)]:
+        a, b = rng.normal(size=shape).astype(np.float32), rng.normal(size=shape).astype(np.float32)
+        f1, f2, g1 = ti.field(float, shape=shape), ti.field(float, shape=shape), ti.field(float, shape=shape)
+        f1.from_numpy(a)
+        g1.from_numpy(a)
+        f2.from_numpy(b)
+        add_fields(f1, f2, f1, -1.)
+        scale_field(f1, 0.5, f1)
+        add_scale_fields(g1, f2, g1, -1., 0.5)
+        assert np.array_equal(f1.to_numpy(), g1.to_numpy())
+        assert np.array_equal(g1.to_numpy(), ((a - b) * np.float32(0.5)).astype(np.float32))
