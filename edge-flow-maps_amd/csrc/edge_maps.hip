@@ -219,6 +219,11 @@ __global__ void __launch_bounds__(256) k_map_smoke_psi(Grid g, int mode, const f
 
 // run_edge.py:508-519 clamp_u over all three face grids: u2 = clamp(u, min, max
 // of the six neighbours in u1).
+// SUB: the clamped value is U1 - U2 on the same face (the BFECC correction
+// u - tmp_u, computed exactly as add_fields(u, tmp, err, -1) rounds it) and the
+// result overwrites U2 in place; each thread reads and writes only its own face
+// of U2, so the in-place update has no race.
+template <bool SUB>
 __global__ void __launch_bounds__(256) k_clamp_u(Grid g, CFaces U, CFaces U1, Faces U2) {
   int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   int a, i, j, k;
@@ -237,7 +242,8 @@ __global__ void __launch_bounds__(256) k_clamp_u(Grid g, CFaces U, CFaces U1, Fa
         a_ = at(i, j, k - 1), c_ = at(i, j, k + 1);
   float mx = fmaxf(fmaxf(fmaxf(l_, r_), fmaxf(b_, t_)), fmaxf(a_, c_));
   float mn = fminf(fminf(fminf(l_, r_), fminf(b_, t_)), fminf(a_, c_));
-  U2.f[a][l] = fminf(fmaxf(U.f[a][l], mn), mx);
+  float v = SUB ? u1[l] - U2.f[a][l] : U.f[a][l];
+  U2.f[a][l] = fminf(fmaxf(v, mn), mx);
 }
 
 // sim/hermite.py:1578-1621 advect_u_semi_lagrangian: RK4 backtrace from the face
@@ -618,8 +624,21 @@ int edge_clamp_u(const float* ux, const float* uy, const float* uz, const float*
   EDGE_REQUIRE(ux && uy && uz && u1x && u1y && u1z && u2x && u2y && u2z, "edge_clamp_u: null pointer");
   Grid g = make_grid(nx, ny, nz, 1.0f);
   int64_t nf = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
-  hipLaunchKernelGGL(k_clamp_u, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g,
+  hipLaunchKernelGGL(k_clamp_u<false>, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g,
                      CFaces{{ux, uy, uz}}, CFaces{{u1x, u1y, u1z}}, Faces{{u2x, u2y, u2z}});
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_sub_clamp_u(const float* ux, const float* uy, const float* uz, float* tx, float* ty, float* tz, int nx,
+                     int ny, int nz, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_sub_clamp_u: bad dims");
+  EDGE_REQUIRE(ux && uy && uz && tx && ty && tz, "edge_sub_clamp_u: null pointer");
+  Grid g = make_grid(nx, ny, nz, 1.0f);
+  int64_t nf = (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1);
+  hipLaunchKernelGGL(k_clamp_u<true>, dim3(blocks_for(nf, 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     CFaces{{ux, uy, uz}}, CFaces{{ux, uy, uz}}, Faces{{tx, ty, tz}});
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }

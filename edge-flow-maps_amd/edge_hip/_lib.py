@@ -44,6 +44,7 @@ SIGNATURES = {
     "edge_map_smoke": "ippipppiiifp",
     "edge_map_smoke_psi": "ippipiiifp",
     "edge_clamp_u": "pppppppppiiip",
+    "edge_sub_clamp_u": "ppppppiiip",
     "edge_advect_u_sl": "ppppppiiiffp",
     "edge_max_speed": "ppppiiip",
     "edge_axpy": "pppflp",

@@ -32,7 +32,7 @@ if _HERE not in sys.path:
 
 from edge_hip import _lib  # noqa: E402
 from edge_hip.field import Field, stream_handle, ti  # noqa: E402
-from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smoke_psi, clamp_u3,  # noqa: E402
+from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smoke_psi, sub_clamp_u3,  # noqa: E402
                              map_smoke, map_smoke_psi, map_u, map_u_psi, march_phi_grid, reset_to_identity)
 from sim import lp as lp_kernels  # noqa: E402
 from sim.edge_driver import backtrack_faces, clamp_smoke, split_psi_hermite  # noqa: E402
@@ -372,13 +372,14 @@ class Simulation:
             add_scale_fields(e, i0, e, -1., 0.5)
         pull_u(EU, TU)
         pull_s(self.err_smoke, self.tmp_smoke)
-        for u_, t_, e_ in zip(U, TU, EU):
-            add_fields(u_, t_, e_, -1.)
+        if not self.BFECC_clamp:
+            for u_, t_, e_ in zip(U, TU, EU):
+                add_fields(u_, t_, e_, -1.)
         add_fields(self.smoke, self.tmp_smoke, self.smoke, -1.)
         pull_s(self.init_smoke, self.smoke, clamp=True)
         if self.BFECC_clamp:
-            clamp_u3(self.err_u_x, self.err_u_y, self.err_u_z, self.u_x, self.u_y, self.u_z, self.tmp_u_x,
-                     self.tmp_u_y, self.tmp_u_z)
+            # err = u - tmp, then tmp = clamp(err) in one pass (err_u stays unused)
+            sub_clamp_u3(self.u_x, self.u_y, self.u_z, self.tmp_u_x, self.tmp_u_y, self.tmp_u_z)
             self.u_x.swap_storage(self.tmp_u_x)
             self.u_y.swap_storage(self.tmp_u_y)
             self.u_z.swap_storage(self.tmp_u_z)

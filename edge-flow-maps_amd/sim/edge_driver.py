@@ -192,6 +192,15 @@ def clamp_u3(u_x, u_y, u_z, u1_x, u1_y, u1_z, u2_x, u2_y, u2_z):
     touch(u2_x, u2_y, u2_z)
 
 
+def sub_clamp_u3(u_x, u_y, u_z, t_x, t_y, t_z):
+    """run_edge.py:685-693 add_fields(u, t, err, -1) then clamp_u(err, u, t), one launch over the
+    three face grids, in place on t; err is not written."""
+    nx, ny, nz = _cells_of_mac(u_x)
+    _lib.call("edge_sub_clamp_u", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), as_ptr(t_x), as_ptr(t_y), as_ptr(t_z),
+              nx, ny, nz, stream_handle())
+    touch(t_x, t_y, t_z)
+
+
 def split_psi_hermite(psi_x, psi_y, psi_z, T_x, T_y, T_z, full_map, dx):
     """run_ed8.py:190-223 / run_ge.py:184-217; full_map = (psi, T_x/y/z, dT_x/y/z, d2T_x/y/z)."""
     psi, Tx, Ty, Tz, dTx, dTy, dTz, d2x, d2y, d2z = full_map

@@ -131,6 +131,11 @@ int edge_map_smoke_psi(int mode, const float* s0, float* s1, int nc, const float
 /* clamp_u (run_edge.py:508-519) on the three face grids. */
 int edge_clamp_u(const float* ux, const float* uy, const float* uz, const float* u1x, const float* u1y,
                  const float* u1z, float* u2x, float* u2y, float* u2z, int nx, int ny, int nz, void* stream);
+/* No single counterpart: add_fields(u, t, err, -1) then clamp_u(err, u, t) (run_edge.py:685-693,
+   508-519) in one pass, in place on t: t = clamp(u - t, min/max of u's six face neighbours).
+   Same values as the two calls; err is not written. */
+int edge_sub_clamp_u(const float* ux, const float* uy, const float* uz, float* tx, float* ty, float* tz, int nx,
+                     int ny, int nz, void* stream);
 /* advect_u_semi_lagrangian (sim/hermite.py:1578-1621); dt is the full argument. */
 int edge_advect_u_sl(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
                      int nx, int ny, int nz, float dx, float dt, void* stream);

@@ -179,6 +179,21 @@ def test_driver_maps_parity(orc32, orc64, dev):
     gc = ops.clamp_u(gu, [T_(c, dev) for c in u1])
     for a in range(3):
         assert np.array_equal(N_(gc[a]).astype(np.float32), orc32.clamp_u(u[a], u1[a]))
+    # the fused BFECC tail: t = clamp(u1 - t) in place, the same bits as add_fields + clamp_u
+    from edge_hip import _lib
+    from edge_hip.field import stream_handle
+    _p = ops._p
+    gu1 = [T_(c, dev) for c in u1]
+    gt = [T_(c, dev) for c in u]
+    err = [(a_ - b_) for a_, b_ in zip(gu1, gt)]
+    two = ops.clamp_u(err, gu1)
+    _lib.call("edge_sub_clamp_u", *[_p(c) for c in gu1], *[_p(c) for c in gt], *n, stream_handle())
+    torch.cuda.synchronize()
+    for a in range(3):
+        assert torch.equal(gt[a], two[a])
+        u1f, uf = u1[a].astype(np.float32), u[a].astype(np.float32)
+        ref = orc32.clamp_u(u1f + np.float32(-1.) * uf, u1f)
+        assert np.array_equal(N_(gt[a]).astype(np.float32), ref)
     sl = ops.advect_u_sl(gu, dx, 0.25 * dx)
     rsl = orc32.advect_u_sl(u, dx, 0.25 * dx)
     for a in range(3):
