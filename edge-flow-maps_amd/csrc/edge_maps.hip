@@ -26,8 +26,12 @@ __device__ __forceinline__ int64_t nfaces(const Grid& g) {
   return (int64_t)(g.nx + 1) * g.ny * g.nz + (int64_t)g.nx * (g.ny + 1) * g.nz + (int64_t)g.nx * g.ny * (g.nz + 1);
 }
 
-// run_edge.py:360-373 map_u: u1_a = F_a . interp_u_MAC(u0, phi_a).
-__global__ void __launch_bounds__(256) k_map_u(Grid g, Mac U0, Faces U1, CFaces F, CFaces PHI) {
+// run_edge.py:360-373 map_u: u1_a = F_a . interp_u_MAC(u0, phi_a).  With I
+// given (I.f[0] non-null) the stored value is sc * (u1_a + m * I_a), rounded as
+// add_fields(u1, I, u1, m) then scale_field(u1, sc, u1) round it (the BFECC
+// error (map - init) / 2 without the extra pass over u1).
+__global__ void __launch_bounds__(256) k_map_u(Grid g, Mac U0, Faces U1, CFaces F, CFaces PHI, CFaces I, float m,
+                                               float sc) {
   int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   int a, i, j, k;
   int64_t l, nf;
@@ -37,7 +41,12 @@ __global__ void __launch_bounds__(256) k_map_u(Grid g, Mac U0, Faces U1, CFaces 
   float p[3] = {ph[l], ph[nf + l], ph[2 * nf + l]};
   float v[3];
   interp_u_mac<false>(g, U0, p, v, nullptr);
-  U1.f[a][l] = Fa[l] * v[0] + Fa[nf + l] * v[1] + Fa[2 * nf + l] * v[2];
+  float r = Fa[l] * v[0] + Fa[nf + l] * v[1] + Fa[2 * nf + l] * v[2];
+  if (I.f[0]) {
+    r = r + m * I.f[a][l];
+    r = sc * r;
+  }
+  U1.f[a][l] = r;
 }
 
 // run_edge.py:376-417 map_u_psi: u1_a(X_a) = sum_c d psi_c/d x_a (X_a) u0_c(psi(X_a)).
@@ -156,9 +165,12 @@ __device__ __forceinline__ void linear_taps(const Grid& g, const float* __restri
 
 // run_edge.py:420-427 map_smoke (mode 0) and 469-477 clamp_smoke (mode 1):
 // psi_c = mean of the six face points of phi.
+// With s2 given, mode 0 stores sc * (map + m * s2) and mode 1 clamps s1 + m * s2,
+// rounded as add_fields then scale_field (mode 0) or add_fields (mode 1) would
+// round them in a pass of their own.
 template <int NC>
 __global__ void __launch_bounds__(256) k_map_smoke(Grid g, int mode, const float* __restrict__ s0, float* __restrict__ s1,
-                                                   CFaces PHI) {
+                                                   CFaces PHI, const float* __restrict__ s2, float m, float sc) {
   const int64_t N = ncell(g);
   const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (q >= N) return;
@@ -180,8 +192,18 @@ __global__ void __launch_bounds__(256) k_map_smoke(Grid g, int mode, const float
   linear_taps<NC>(g, s0, p, mode, out, mn, mx);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    if (mode == 0) s1[c * N + q] = out[c];
-    else s1[c * N + q] = fminf(fmaxf(s1[c * N + q], mn[c]), mx[c]);
+    if (mode == 0) {
+      float r = out[c];
+      if (s2) {
+        r = r + m * s2[c * N + q];
+        r = sc * r;
+      }
+      s1[c * N + q] = r;
+    } else {
+      float v = s1[c * N + q];
+      if (s2) v = v + m * s2[c * N + q];
+      s1[c * N + q] = fminf(fmaxf(v, mn[c]), mx[c]);
+    }
   }
 }
 
@@ -548,7 +570,27 @@ int edge_map_u(const float* u0x, const float* u0y, const float* u0z, float* u1x,
                                   (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1),
                                   256)),
                      dim3(256), 0, (hipStream_t)stream, g, Mac{u0x, u0y, u0z}, Faces{{u1x, u1y, u1z}},
-                     CFaces{{Fx, Fy, Fz}}, CFaces{{phix, phiy, phiz}});
+                     CFaces{{Fx, Fy, Fz}}, CFaces{{phix, phiy, phiz}}, CFaces{{nullptr, nullptr, nullptr}}, 0.f,
+                     0.f);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_map_u_add_scale(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+                         const float* Fx, const float* Fy, const float* Fz, const float* phix, const float* phiy,
+                         const float* phiz, const float* ix, const float* iy, const float* iz, float m, float s,
+                         int nx, int ny, int nz, float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_map_u_add_scale: bad dims");
+  EDGE_REQUIRE(u0x && u0y && u0z && u1x && u1y && u1z && Fx && Fy && Fz && phix && phiy && phiz && ix && iy && iz,
+               "edge_map_u_add_scale: null pointer");
+  EDGE_REQUIRE(ix != u1x && iy != u1y && iz != u1z, "edge_map_u_add_scale: the added field must not be the output");
+  Grid g = make_grid(nx, ny, nz, dx);
+  hipLaunchKernelGGL(k_map_u, dim3(blocks_for(
+                                  (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1),
+                                  256)),
+                     dim3(256), 0, (hipStream_t)stream, g, Mac{u0x, u0y, u0z}, Faces{{u1x, u1y, u1z}},
+                     CFaces{{Fx, Fy, Fz}}, CFaces{{phix, phiy, phiz}}, CFaces{{ix, iy, iz}}, m, s);
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }
@@ -579,23 +621,40 @@ int edge_split_psi_hermite(const float* tab, float* phix, float* phiy, float* ph
   return EDGE_OK;
 }
 
-// mode 0: map_smoke, 1: clamp_smoke (smoke1 updated in place).  nc in 1..4.
-int edge_map_smoke(int mode, const float* s0, float* s1, int nc, const float* phix, const float* phiy,
-                   const float* phiz, int nx, int ny, int nz, float dx, void* stream) {
-  clear_error();
-  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_map_smoke: bad dims");
-  EDGE_REQUIRE(s0 && s1 && phix && phiy && phiz && (mode == 0 || mode == 1), "edge_map_smoke: bad arguments");
-  EDGE_REQUIRE(nc == 4 || nc == 1, "edge_map_smoke: nc must be 1 or 4 (got %d)", nc);
+static int map_smoke_launch(const char* who, int mode, const float* s0, float* s1, const float* s2, float m,
+                            float sc, int nc, const float* phix, const float* phiy, const float* phiz, int nx, int ny,
+                            int nz, float dx, void* stream) {
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "%s: bad dims", who);
+  EDGE_REQUIRE(s0 && s1 && phix && phiy && phiz && (mode == 0 || mode == 1), "%s: bad arguments", who);
+  EDGE_REQUIRE(nc == 4 || nc == 1, "%s: nc must be 1 or 4 (got %d)", who, nc);
+  EDGE_REQUIRE(s0 != s1 && s2 != s1 && s2 != s0, "%s: the smoke fields must be distinct", who);
   Grid g = make_grid(nx, ny, nz, dx);
   dim3 grid(blocks_for(ncell(g), 256));
   if (nc == 4)
     hipLaunchKernelGGL(k_map_smoke<4>, grid, dim3(256), 0, (hipStream_t)stream, g, mode, s0, s1,
-                       CFaces{{phix, phiy, phiz}});
+                       CFaces{{phix, phiy, phiz}}, s2, m, sc);
   else
     hipLaunchKernelGGL(k_map_smoke<1>, grid, dim3(256), 0, (hipStream_t)stream, g, mode, s0, s1,
-                       CFaces{{phix, phiy, phiz}});
+                       CFaces{{phix, phiy, phiz}}, s2, m, sc);
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
+}
+
+// mode 0: map_smoke, 1: clamp_smoke (smoke1 updated in place).  nc in 1..4.
+int edge_map_smoke(int mode, const float* s0, float* s1, int nc, const float* phix, const float* phiy,
+                   const float* phiz, int nx, int ny, int nz, float dx, void* stream) {
+  clear_error();
+  return map_smoke_launch("edge_map_smoke", mode, s0, s1, nullptr, 0.f, 0.f, nc, phix, phiy, phiz, nx, ny, nz, dx,
+                          stream);
+}
+
+int edge_map_smoke_add_scale(int mode, const float* s0, float* s1, const float* s2, float m, float s, int nc,
+                             const float* phix, const float* phiy, const float* phiz, int nx, int ny, int nz, float dx,
+                             void* stream) {
+  clear_error();
+  EDGE_REQUIRE(s2, "edge_map_smoke_add_scale: null pointer");
+  return map_smoke_launch("edge_map_smoke_add_scale", mode, s0, s1, s2, m, s, nc, phix, phiy, phiz, nx, ny, nz, dx,
+                          stream);
 }
 
 // mode 0: map_smoke_psi, 1: clamp_smoke_psi.

@@ -43,7 +43,7 @@ from sim.init_conditions import (init_delta, init_four_vorts, init_rings, init_v
 from sim.mgpcg import MGPCG_3  # noqa: E402
 from sim.mgpcg_solid import MGPCG_Solid_3  # noqa: E402
 from util import io_utils  # noqa: E402
-from util.taichi_utils import (add_fields, add_scale_fields, center_coords_func, curl, get_central_vector,  # noqa: E402
+from util.taichi_utils import (add_fields, center_coords_func, curl, get_central_vector,  # noqa: E402
                                split_central_vector, x_coords_func, y_coords_func, z_coords_func)
 
 CASES = {
@@ -352,31 +352,37 @@ class Simulation:
             def pull_u(u0, u1):
                 map_u_psi(*u0, *u1, dx, gm)
 
-            def pull_s(s0, s1, clamp=False):
-                (clamp_smoke_psi if clamp else map_smoke_psi)(s0, s1, dx, gm)
+            def pull_s(s0, s1, clamp=False, sub=None):
+                if clamp:
+                    clamp_smoke_psi(s0, s1, dx, gm, sub=sub)
+                else:
+                    map_smoke_psi(s0, s1, dx, gm)
         else:
             # run_ed8.py:409-433: through the face values of the full map
             def pull_u(u0, u1):
                 map_u(*u0, *u1, self.T_x, self.T_y, self.T_z, self.psi_x, self.psi_y, self.psi_z, dx)
 
-            def pull_s(s0, s1, clamp=False):
-                (clamp_smoke if clamp else map_smoke)(s0, s1, self.psi_x, self.psi_y, self.psi_z, dx)
+            def pull_s(s0, s1, clamp=False, sub=None):
+                if clamp:
+                    clamp_smoke(s0, s1, self.psi_x, self.psi_y, self.psi_z, dx, sub=sub)
+                else:
+                    map_smoke(s0, s1, self.psi_x, self.psi_y, self.psi_z, dx)
         U, IU = (self.u_x, self.u_y, self.u_z), (self.init_u_x, self.init_u_y, self.init_u_z)
         EU, TU = (self.err_u_x, self.err_u_y, self.err_u_z), (self.tmp_u_x, self.tmp_u_y, self.tmp_u_z)
         pull_u(IU, U)
         pull_s(self.init_smoke, self.smoke)
-        map_u(*U, *EU, self.F_x, self.F_y, self.F_z, self.phi_x, self.phi_y, self.phi_z, dx)
-        map_smoke(self.smoke, self.err_smoke, self.phi_x, self.phi_y, self.phi_z, dx)
-        # run_edge.py:675-681: err = (err - init) * 0.5, one pass per field
-        for e, i0 in zip(EU + (self.err_smoke,), IU + (self.init_smoke,)):
-            add_scale_fields(e, i0, e, -1., 0.5)
+        # run_edge.py:669-682: err = (map(u) - init) * 0.5 in the map's own pass
+        map_u(*U, *EU, self.F_x, self.F_y, self.F_z, self.phi_x, self.phi_y, self.phi_z, dx,
+              add_scale=(*IU, -1., 0.5))
+        map_smoke(self.smoke, self.err_smoke, self.phi_x, self.phi_y, self.phi_z, dx,
+                  add_scale=(self.init_smoke, -1., 0.5))
         pull_u(EU, TU)
         pull_s(self.err_smoke, self.tmp_smoke)
         if not self.BFECC_clamp:
             for u_, t_, e_ in zip(U, TU, EU):
                 add_fields(u_, t_, e_, -1.)
-        add_fields(self.smoke, self.tmp_smoke, self.smoke, -1.)
-        pull_s(self.init_smoke, self.smoke, clamp=True)
+        # run_edge.py:688-689: smoke = clamp(smoke - tmp_smoke), one pass
+        pull_s(self.init_smoke, self.smoke, clamp=True, sub=self.tmp_smoke)
         if self.BFECC_clamp:
             # err = u - tmp, then tmp = clamp(err) in one pass (err_u stays unused)
             sub_clamp_u3(self.u_x, self.u_y, self.u_z, self.tmp_u_x, self.tmp_u_y, self.tmp_u_z)

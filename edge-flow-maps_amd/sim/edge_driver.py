@@ -80,12 +80,22 @@ def backtrack_faces(psi_x, psi_y, psi_z, T_x, T_y, T_z, velocities, dts, dx):
     touch(psi_x, psi_y, psi_z, T_x, T_y, T_z)
 
 
-def map_u(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, T_x, T_y, T_z, psi_x, psi_y, psi_z, dx):
-    """run_edge.py:360-373."""
+def map_u(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, T_x, T_y, T_z, psi_x, psi_y, psi_z, dx, add_scale=None):
+    """run_edge.py:360-373.
+
+    add_scale = (f_x, f_y, f_z, multiplier, scale) folds the BFECC arithmetic
+    into the same pass: u1 = scale * (map_u(u0) + multiplier * f), the values of
+    map_u, add_fields(u1, f, u1, multiplier), scale_field(u1, scale, u1)
+    (run_edge.py:671-681)."""
     nx, ny, nz = _cells_of_mac(u_x0)
-    _lib.call("edge_map_u", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
-              as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z), nx, ny, nz,
-              float(dx), stream_handle())
+    args = [as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1), as_ptr(T_x),
+            as_ptr(T_y), as_ptr(T_z), as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z)]
+    if add_scale is None:
+        _lib.call("edge_map_u", *args, nx, ny, nz, float(dx), stream_handle())
+    else:
+        f_x, f_y, f_z, m, sc = add_scale
+        _lib.call("edge_map_u_add_scale", *args, as_ptr(f_x), as_ptr(f_y), as_ptr(f_z), float(m), float(sc), nx, ny,
+                  nz, float(dx), stream_handle())
     touch(u_x1, u_y1, u_z1)
 
 
@@ -147,31 +157,41 @@ def map_u_psi(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, dx, garm_map):
     touch(u_x1, u_y1, u_z1)
 
 
-def _smoke(mode, smoke0, smoke1, psi_x, psi_y, psi_z, dx):
+def _smoke_ptrs(mode, smoke0, smoke1, phi_ptrs, dx, add):
     nx, ny, nz = smoke0.shape
-    _lib.call("edge_map_smoke", mode, as_ptr(smoke0), as_ptr(smoke1), smoke0.n or 1, as_ptr(psi_x), as_ptr(psi_y),
-              as_ptr(psi_z), nx, ny, nz, float(dx), stream_handle())
+    nc = smoke0.n or 1
+    if add is None:
+        _lib.call("edge_map_smoke", mode, as_ptr(smoke0), as_ptr(smoke1), nc, *phi_ptrs, nx, ny, nz, float(dx),
+                  stream_handle())
+    else:
+        s2, m, sc = add
+        _lib.call("edge_map_smoke_add_scale", mode, as_ptr(smoke0), as_ptr(smoke1), as_ptr(s2), float(m), float(sc),
+                  nc, *phi_ptrs, nx, ny, nz, float(dx), stream_handle())
     touch(smoke1)
 
 
-def map_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx):
-    """run_edge.py:420-427."""
-    _smoke(0, smoke0, smoke1, psi_x, psi_y, psi_z, dx)
+def _smoke(mode, smoke0, smoke1, psi_x, psi_y, psi_z, dx, add=None):
+    _smoke_ptrs(mode, smoke0, smoke1, (as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z)), dx, add)
 
 
-def clamp_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx):
-    """run_edge.py:469-477 (smoke1 clamped in place)."""
-    _smoke(1, smoke0, smoke1, psi_x, psi_y, psi_z, dx)
+def map_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx, add_scale=None):
+    """run_edge.py:420-427.  add_scale = (f, multiplier, scale) stores
+    scale * (map + multiplier * f), the values of map_smoke, add_fields and
+    scale_field (run_edge.py:672-682) in one pass."""
+    _smoke(0, smoke0, smoke1, psi_x, psi_y, psi_z, dx, add_scale)
 
 
-def _smoke_psi(mode, smoke0, smoke1, dx, garm_map):
+def clamp_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx, sub=None):
+    """run_edge.py:469-477 (smoke1 clamped in place).  With sub, smoke1 - sub is
+    clamped: add_fields(smoke1, sub, smoke1, -1) (run_edge.py:688) in the same pass."""
+    _smoke(1, smoke0, smoke1, psi_x, psi_y, psi_z, dx, None if sub is None else (sub, -1., 1.))
+
+
+def _smoke_psi(mode, smoke0, smoke1, dx, garm_map, add=None):
     # psi_c = 1/6 sum of psi at the cell's six faces (run_edge.py:448-465) is
     # map_smoke's average of a face map, here the cached psi at the faces
-    nx, ny, nz = smoke0.shape
     phi, _ = face_map_of(garm_map, dx)
-    _lib.call("edge_map_smoke", mode, as_ptr(smoke0), as_ptr(smoke1), smoke0.n or 1, *[t.data_ptr() for t in phi],
-              nx, ny, nz, float(dx), stream_handle())
-    touch(smoke1)
+    _smoke_ptrs(mode, smoke0, smoke1, [t.data_ptr() for t in phi], dx, add)
 
 
 def map_smoke_psi(smoke0, smoke1, dx, garm_map):
@@ -179,9 +199,10 @@ def map_smoke_psi(smoke0, smoke1, dx, garm_map):
     _smoke_psi(0, smoke0, smoke1, dx, garm_map)
 
 
-def clamp_smoke_psi(smoke0, smoke1, dx, garm_map):
-    """run_edge.py:480-505 (smoke1 clamped in place)."""
-    _smoke_psi(1, smoke0, smoke1, dx, garm_map)
+def clamp_smoke_psi(smoke0, smoke1, dx, garm_map, sub=None):
+    """run_edge.py:480-505 (smoke1 clamped in place).  With sub, smoke1 - sub is
+    clamped: add_fields(smoke1, sub, smoke1, -1) (run_edge.py:688) in the same pass."""
+    _smoke_psi(1, smoke0, smoke1, dx, garm_map, None if sub is None else (sub, -1., 1.))
 
 
 def clamp_u3(u_x, u_y, u_z, u1_x, u1_y, u1_z, u2_x, u2_y, u2_z):

@@ -116,6 +116,13 @@ int edge_coords(int mode, float* out, int nx, int ny, int nz, float dx, void* st
 int edge_map_u(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
                const float* Fx, const float* Fy, const float* Fz, const float* phix, const float* phiy,
                const float* phiz, int nx, int ny, int nz, float dx, void* stream);
+/* No single counterpart: map_u (run_edge.py:360-373) then add_fields(u1, i, u1, m) and
+   scale_field(u1, s, u1) (run_edge.py:671-681): u1 = s (map_u(u0) + m i), the same values
+   as the three calls in one pass.  i must not be u1. */
+int edge_map_u_add_scale(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+                         const float* Fx, const float* Fy, const float* Fz, const float* phix, const float* phiy,
+                         const float* phiz, const float* ix, const float* iy, const float* iz, float m, float s,
+                         int nx, int ny, int nz, float dx, void* stream);
 /* map_u_psi (run_edge.py:376-417); tab: garm table of the cell map. */
 int edge_map_u_psi(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
                    const float* tab, int nx, int ny, int nz, float dx, void* stream);
@@ -125,6 +132,13 @@ int edge_split_psi_hermite(const float* tab, float* phix, float* phiy, float* ph
 /* mode 0 map_smoke (run_edge.py:420-427), 1 clamp_smoke (469-477); nc 1 or 4. */
 int edge_map_smoke(int mode, const float* s0, float* s1, int nc, const float* phix, const float* phiy,
                    const float* phiz, int nx, int ny, int nz, float dx, void* stream);
+/* No single counterpart: edge_map_smoke with a third smoke field s2 folded in, the same
+   values as the separate calls: mode 0 is map_smoke then add_fields(s1, s2, s1, m) and
+   scale_field(s1, s, s1) (run_edge.py:672-682); mode 1 is add_fields(s1, s2, s1, m) then
+   clamp_smoke (run_edge.py:688-689).  s0, s1, s2 distinct. */
+int edge_map_smoke_add_scale(int mode, const float* s0, float* s1, const float* s2, float m, float s, int nc,
+                             const float* phix, const float* phiy, const float* phiz, int nx, int ny, int nz, float dx,
+                             void* stream);
 /* mode 0 map_smoke_psi (run_edge.py:443-466), 1 clamp_smoke_psi (480-505). */
 int edge_map_smoke_psi(int mode, const float* s0, float* s1, int nc, const float* tab, int nx, int ny, int nz,
                        float dx, void* stream);

@@ -174,15 +174,39 @@ def test_driver_maps_parity(orc32, orc64, dev):
         g = ops.map_smoke(mode, T_(s0, dev), T_(s1, dev) if mode else None, gphi, dx)
         r = orc32.map_smoke(mode, s0, s1 if mode else None, phi, dx)
         assert scale_err(N_(g), r) < 1e-5
+    # the BFECC arithmetic folded into the maps: the bits of the separate passes
+    from edge_hip import _lib
+    from edge_hip.field import stream_handle
+    _p = ops._p
+    gi = [T_(c, dev) for c in smooth_velocity(*n, dx, seed=16)]
+    fused = [torch.empty_like(c) for c in gi]
+    _lib.call("edge_map_u_add_scale", *[_p(c) for c in gu], *[_p(c) for c in fused], *[_p(c) for c in gF],
+              *[_p(c) for c in gphi], *[_p(c) for c in gi], -1., 0.5, *n, dx, stream_handle())
+    for a in range(3):
+        seq = torch.empty_like(out[a])
+        _lib.call("edge_axpy", _p(out[a]), _p(gi[a]), _p(seq), -1., seq.numel(), stream_handle())
+        _lib.call("edge_scale", _p(seq), 0.5, _p(seq), seq.numel(), stream_handle())
+        assert torch.equal(fused[a], seq), a
+    gs0, gs1, gs2 = T_(s0, dev), T_(s1, dev), T_(rng.uniform(-0.5, 1.5, size=(4,) + n), dev)
+    for mode in (0, 1):
+        sm = gs1.clone()
+        _lib.call("edge_map_smoke_add_scale", mode, _p(gs0), _p(sm), _p(gs2), -1., 0.5, 4,
+                  *[_p(c) for c in gphi], *n, dx, stream_handle())
+        if mode == 0:
+            seq = ops.map_smoke(0, gs0, None, gphi, dx)
+            _lib.call("edge_axpy", _p(seq), _p(gs2), _p(seq), -1., seq.numel(), stream_handle())
+            _lib.call("edge_scale", _p(seq), 0.5, _p(seq), seq.numel(), stream_handle())
+        else:
+            seq = gs1.clone()
+            _lib.call("edge_axpy", _p(seq), _p(gs2), _p(seq), -1., seq.numel(), stream_handle())
+            seq = ops.map_smoke(1, gs0, seq, gphi, dx)
+        assert torch.equal(sm, seq), mode
     # clamp_u, semi-Lagrangian, max speed
     u1 = smooth_velocity(*n, dx, seed=13)
     gc = ops.clamp_u(gu, [T_(c, dev) for c in u1])
     for a in range(3):
         assert np.array_equal(N_(gc[a]).astype(np.float32), orc32.clamp_u(u[a], u1[a]))
     # the fused BFECC tail: t = clamp(u1 - t) in place, the same bits as add_fields + clamp_u
-    from edge_hip import _lib
-    from edge_hip.field import stream_handle
-    _p = ops._p
     gu1 = [T_(c, dev) for c in u1]
     gt = [T_(c, dev) for c in u]
     err = [(a_ - b_) for a_, b_ in zip(gu1, gt)]
