@@ -18,6 +18,7 @@ instead (Field.swap_storage); the values are identical.
 Usage: python run_edge.py -c cfg1 [--steps N] [--solver plain|mixed]
 """
 import argparse
+import gc
 import math
 import os
 import sys
@@ -70,6 +71,16 @@ CASES = {
                  lp=dict(max_num=16777216, Re=1.0e4, St=5.0e-3, gravity=1.0, density_ratio=0.6,
                          epiral_radius_ratio=35.0, laden_radius=0.01, ring_R=0.25, tube=0.08)),
 }
+
+
+def freeze_setup_objects():
+    """Before a long step loop: move every object made so far (modules, fields,
+    solver state, all living until the end) out of the cyclic collector's
+    generations, so that a full collection during the loop scans only the
+    loop's own objects instead of pausing the host, and the GPU queue behind
+    it, for a pass over every object of the process."""
+    gc.collect()
+    gc.freeze()
 
 
 def cell_order(pos, dx, res, tile=8):
@@ -596,6 +607,7 @@ class Simulation:
             self.load_checkpoint(ckptdir, from_frame)
         self.write_frame(vtkdir, pdir, self.frame_idx)
         self.save_checkpoint(ckptdir, self.frame_idx, state=state)
+        freeze_setup_objects()
         while self.frame_idx < total_frames:
             t0 = time.perf_counter()
             dt = self.substep()
