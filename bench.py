@@ -39,6 +39,9 @@ MARCH_FLOP_PER_CELL = 22250.0      # fused EDGE march: 5 points x (RK4 with Jaco
 SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 16550.0, "ge": 4430.0}
 # the face march (RK4 of a face point and its column): per face and velocity in EFM's backtrack
 FACE_MARCH_FLOP_PER_FACE = 2860.0
+# the reference formulation's count (64 weight products per Hermite evaluation, 27 per
+# B-spline evaluation), the algorithmic work the roofline record is priced with
+SCHEME_FLOP_PER_CELL_ALG = {"edge": 33600.0, "ed8": 21400.0, "ge": 6800.0}
 SCHEME_BYTES_PER_CELL = {"edge": 180.0, "ed8": 204.0, "ge": 324.0}
 MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0, "dense": 145.0}  # one CG iteration
 MARCH_BYTES_PER_CELL = 180.0       # u 12 + old map 84 + new map 84 (compulsory FP32)
@@ -245,25 +248,60 @@ def io_timing(sim, torch):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def _rel(a, b):
+    """max |a - b| / max |b| over a list of arrays (FP64)."""
+    import numpy as np
+    num = max(float(np.max(np.abs(np.asarray(x, np.float64) - np.asarray(y, np.float64)))) for x, y in zip(a, b))
+    den = max(float(np.max(np.abs(np.asarray(y, np.float64)))) for y in b)
+    return num / max(den, 1e-300)
+
+
 def cpu_baseline(sim):
     """The oracle (the CPU restatement, OpenMP) running one full step of the
     measured workload itself, from the simulation's current state: about 20 s of
-    CPU work on 16 threads, no scaling."""
+    CPU work on 16 threads, no scaling.  The GPU takes the same step from the
+    same state (a reinitialisation step, as the oracle's first one is), and the
+    step's velocity and pressure are compared with the FP32 oracle and with the
+    oracle in FP64 (the FP32 oracle's own distance from FP64 is the yardstick)."""
     import numpy as np
+    import torch
     from oracle.step import OracleStep
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     u = sim.velocity_numpy()
     smoke = np.moveaxis(sim.smoke.to_numpy(), -1, 0)
-    st = OracleStep(u, smoke, sim.dx, sim.reinit_every, sim.CFL, sim.ed_eps, levels=sim.solver.n_mg_levels,
-                    BFECC_clamp=sim.BFECC_clamp, stop_mode=1, rel_tol=1e-6)
+    # the GPU step: make it the first step of a reinitialisation interval
+    sim.i = (sim.i // sim.reinit_every + 1) * sim.reinit_every - 1
+    dt = sim.substep()
+    torch.cuda.synchronize()
+    gu = sim.velocity_numpy()
+    gp = sim.solver.p.to_numpy().astype(np.float64)
+
+    def oracle(dtype):
+        return OracleStep(u, smoke, sim.dx, sim.reinit_every, sim.CFL, sim.ed_eps, levels=sim.solver.n_mg_levels,
+                          BFECC_clamp=sim.BFECC_clamp, stop_mode=1, rel_tol=1e-6, dtype=dtype)
+    st = oracle(np.float32)
     t0 = time.perf_counter()
-    st.substep()
-    dt = time.perf_counter() - t0
+    st.substep(dt=dt)
+    elapsed = time.perf_counter() - t0
+    st64 = oracle(np.float64)
+    st64.substep(dt=dt)
+    mf = lambda x: np.asarray(x, np.float64) - np.mean(x)
     n = f"{sim.res_x}x{sim.res_y}x{sim.res_z}"
-    return dict(value=round(dt * 1e3, 1), unit="ms/step", cores=threads, kind="port",
+    base = dict(value=round(elapsed * 1e3, 1), unit="ms/step", cores=threads, kind="port",
                 sample=f"one full oracle step (reinitialisation, advection, EDGE march, pull-backs, two "
                        f"projections to 1e-6; FP32, OpenMP) of the measured {n} workload from the timed "
-                       f"run's state, {dt:.1f} s, unscaled")
+                       f"run's state, {elapsed:.1f} s, unscaled")
+    err = dict(step="one GPU step from the timed run's state against the oracle's step from the same state "
+                    "(same dt; velocity: max |diff| / max |ref| over the three face grids; pressure of the "
+                    "second projection, mean removed)",
+               velocity_vs_fp32_oracle=float("%.3g" % _rel(gu, st.u)),
+               velocity_vs_fp64_oracle=float("%.3g" % _rel(gu, st64.u)),
+               fp32_oracle_vs_fp64_oracle=float("%.3g" % _rel(st.u, st64.u)),
+               pressure_vs_fp64_oracle=float("%.3g" % _rel([mf(gp)], [mf(st64.p)])),
+               fp32_oracle_pressure_vs_fp64=float("%.3g" % _rel([mf(st.p)], [mf(st64.p)])),
+               poisson_iters_gpu=[s["iters"] for s in sim.poisson_stats[-2:]],
+               poisson_iters_fp32_oracle=list(st.iters), poisson_iters_fp64_oracle=list(st64.iters))
+    return base, err
 
 
 def main():
@@ -337,15 +375,25 @@ def main():
         flop_launch = SCHEME_FLOP_PER_CELL[a.scheme] * ncell
         bytes_launch = SCHEME_BYTES_PER_CELL[a.scheme] * ncell
     march_tflops = flop_launch / (march_ms * 1e-3) / 1e12
+    # the same launches priced with the reference formulation's count (64 weight products per
+    # Hermite evaluation, 27 per B-spline evaluation): the algorithmic figure per cell
+    flop_launch_alg = SCHEME_FLOP_PER_CELL_ALG.get(a.scheme, 0.0) * ncell if a.scheme != "efm" else flop_launch
+    march_tflops_alg = flop_launch_alg / (march_ms * 1e-3) / 1e12
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and a.scheme == "edge":
         rec = json.load(open(TRAFFIC_FILE)).get("k_march_edge", {})
         if rec.get("workload") == march_workload(res, a.case, a.scheme):
             traffic = rec.get("hbm_bytes_per_launch")
-    roof_march = dict(bound="mfma", pipe="FP32 vector ALU; 157.3 TFLOP/s is MI355X's FP32 peak",
-                      achieved=round(march_tflops, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                      frac=round(march_tflops / FP32_PEAK_TFLOPS, 4), traffic=traffic, kernel=march_kernel,
-                      flop_per_launch=flop_launch, algorithmic_bytes_per_launch=bytes_launch,
+    roof_march = dict(bound="valu", pipe="FP32 vector ALU (no matrix op on this path); 157.3 TFLOP/s is "
+                                         "MI355X's FP32 vector peak",
+                      achieved=round(march_tflops_alg, 2), peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                      frac=round(march_tflops_alg / FP32_PEAK_TFLOPS, 4), traffic=traffic, kernel=march_kernel,
+                      flop_source="algorithmic: the reference formulation's flops per cell",
+                      flop_per_cell=SCHEME_FLOP_PER_CELL_ALG.get(a.scheme), flop_per_launch=flop_launch_alg,
+                      counter_flop_per_cell=SCHEME_FLOP_PER_CELL.get(a.scheme),
+                      counter_achieved=round(march_tflops, 2),
+                      counter_frac=round(march_tflops / FP32_PEAK_TFLOPS, 4),
+                      algorithmic_bytes_per_launch=bytes_launch,
                       avg_launch_ms=round(march_ms, 4), launches=len(march_launch_ms))
     # the projection's CG iteration (many kernels), HBM-bound: bytes per iteration / ms per iteration
     if pcmp is not None:
@@ -369,7 +417,7 @@ def main():
         del src, dst
 
     io_rec = None if a.no_io else io_timing(sim, torch)
-    cpu = None if (a.no_cpu or a.case != "cfg2" or a.scheme != "edge") else cpu_baseline(sim)
+    cpu, oracle_err = (None, None) if (a.no_cpu or a.case != "cfg2" or a.scheme != "edge") else cpu_baseline(sim)
 
     line = {
         "metric": "ms/step @256x128x128; Poisson ms+iters to 1e-6 rel, learned/mixed vs plain",
@@ -399,6 +447,7 @@ def main():
         "roofline_cg_iteration": roof_mg,
         "io": io_rec,
         "cpu_baseline": cpu,
+        "error_vs_oracle": oracle_err,
     }
     print(json.dumps(line), flush=True)
 

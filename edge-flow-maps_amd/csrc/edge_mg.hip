@@ -1667,7 +1667,8 @@ struct Solver {
   int* fidx = nullptr;
   int nf = 0;
   bool dense_ok = false;
-  std::vector<int8_t> dense_mask;  // coarse mask the inverse was built for
+  bool dense_tried = false;        // build_dense ran for dense_mask (it may have failed)
+  std::vector<int8_t> dense_mask;  // coarse mask the inverse was last built (or tried) for
   size_t elem;
   // CG iteration batches captured as a HIP graph (replayed on the caller's stream)
   int use_graph = 0;
@@ -2025,8 +2026,13 @@ int prepare_masks(Solver& s, hipStream_t st) {
     std::vector<int8_t> cm(s.nl[s.L - 1]);
     EDGE_CHECK_HIP(hipMemcpyAsync(cm.data(), s.bm[s.L - 1], cm.size(), hipMemcpyDeviceToHost, st));
     EDGE_CHECK_HIP(hipStreamSynchronize(st));
-    if (!s.dense_ok || cm != s.dense_mask) {
+    // a mask whose build failed (too many coarse cells, or B not positive
+    // definite) is remembered too, so an unchanged mask does not rerun the
+    // O(n^3) host factorisation on every solve
+    if (!s.dense_tried || cm != s.dense_mask) {
       s.dense_ok = build_dense(s, cm);
+      s.dense_tried = true;
+      s.dense_mask = cm;
       drop_graph(s);  // the bottom's matrix, size and the tail split may have changed
     }
   }
@@ -2038,8 +2044,10 @@ int prepare_masks(Solver& s, hipStream_t st) {
 // Dense mode on a chunked level 0: r needs no recentering (the dense coarsest
 // solve handles the constant), and rl[0] is r itself, so the update of r can
 // also write the next V-cycle's fill(0) + first red half-sweep, which saves
-// k_first08's pass over r and z.
-inline bool z_ready(const Solver& s) { return s.mode == 2 && chunked(s.lv[0]); }
+// k_first08's pass over r and z.  Only while the dense bottom exists: without
+// it (coarsest level above DENSE_MAX fluid cells, or a factorisation that
+// failed) the red/black bottom needs the recentred residual of k_first08.
+inline bool z_ready(const Solver& s) { return s.mode == 2 && s.dense_ok && chunked(s.lv[0]); }
 
 template <class S>
 void cg_update(Solver& s, hipStream_t st) {
@@ -2275,6 +2283,8 @@ int edge_mg_create(int nx, int ny, int nz, int levels, int mode, const int* bt, 
   if (mode >= 1) {
     std::vector<int8_t> cm(s->nl[levels - 1], 0);
     s->dense_ok = build_dense(*s, cm);
+    s->dense_tried = true;
+    s->dense_mask = cm;
   }
   for (int l = 1; l < levels; ++l)
     if (s->nl[l] <= TAIL_MAX_CELLS) {

@@ -78,7 +78,8 @@ def freeze_setup_objects():
     solver state, all living until the end) out of the cyclic collector's
     generations, so that a full collection during the loop scans only the
     loop's own objects instead of pausing the host, and the GPU queue behind
-    it, for a pass over every object of the process."""
+    it, for a pass over every object of the process.  Process-wide: a caller
+    that is not the program's own entry point pairs it with gc.unfreeze()."""
     gc.collect()
     gc.freeze()
 
@@ -608,16 +609,21 @@ class Simulation:
         self.write_frame(vtkdir, pdir, self.frame_idx)
         self.save_checkpoint(ckptdir, self.frame_idx, state=state)
         freeze_setup_objects()
-        while self.frame_idx < total_frames:
-            t0 = time.perf_counter()
-            dt = self.substep()
-            log(f"[Simulate] Done step {self.i} dt {dt:.5g} poisson iters {self.poisson_stats[-1]['iters']}")
-            if self.output_frame:
-                self.write_frame(vtkdir, pdir)
-                if self.frame_idx % ckpt_every == 0:
-                    self.save_checkpoint(ckptdir, state=state)
-                ti.sync()
-                log(f"[Simulate] Done Frame {self.frame_idx} time {time.perf_counter() - t0:.2f}s")
+        try:
+            while self.frame_idx < total_frames:
+                t0 = time.perf_counter()
+                dt = self.substep()
+                log(f"[Simulate] Done step {self.i} dt {dt:.5g} poisson iters {self.poisson_stats[-1]['iters']}")
+                if self.output_frame:
+                    self.write_frame(vtkdir, pdir)
+                    if self.frame_idx % ckpt_every == 0:
+                        self.save_checkpoint(ckptdir, state=state)
+                    ti.sync()
+                    log(f"[Simulate] Done Frame {self.frame_idx} time {time.perf_counter() - t0:.2f}s")
+        finally:
+            # hand the frozen objects back to the collector: the freeze is
+            # process-wide and must not outlive this loop
+            gc.unfreeze()
 
 
 def main(argv=None):

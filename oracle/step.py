@@ -140,6 +140,15 @@ class OracleStep:
             # run_edge.py:661-667
             tmp = o.map_u(self.vis, self.F, self.phi, dx)
             self.init_u = [(i0 + dt * t).astype(self.dt_) for i0, t in zip(self.init_u, tmp)]
+        self.bfecc()
+        self.poisson()
+        return dt
+
+    def bfecc(self):
+        """run_edge.py:669-700: pull init_u / init_smoke back through the
+        backward map, push the result forward through (phi, F), and correct by
+        half the round-trip error (then the clamps)."""
+        o, dx = self.o, self.dx
         if self.scheme == "edge":
             def pull_u(u0):
                 return o.map_u_psi(u0, self.psi, self.T, self.dT, dx)
@@ -167,5 +176,3 @@ class OracleStep:
             self.u = [o.clamp_u(e, u) for e, u in zip(err_u, self.u)]
         else:
             self.u = err_u
-        self.poisson()
-        return dt

@@ -358,6 +358,31 @@ def test_dense_coarsest_solve_is_the_fp64_inverse(dev, mode, tol):
 
 
 @pytest.mark.parametrize("mode", [1, 2])
+def test_dense_modes_without_the_dense_bottom(orc64, dev, mode):
+    # two levels at 64x32x32: the coarsest level has 32x16x16 = 8192 cells, more
+    # than the dense solve takes, so the V-cycle keeps the reference's red/black
+    # bottom on the all-walls (singular) system, which needs the residual
+    # recentred (sim/mgpcg.py:169-175) in every iteration, not only the first
+    from edge_hip import _lib
+    n = (64, 32, 32)
+    u = smooth_velocity(*n, 1.0 / 32, seed=7)
+    b = divergence_rhs(u)
+    S = Solver(n, 2, mode)
+    assert _lib.lib().edge_mg_dense_bottom(S.h) == 0
+    ru, rp, rst = orc64.poisson(u, levels=2, stop_mode=1, rel_tol=1e-8, max_iters=400)
+    gu = [T_(c, dev) for c in u]
+    p, st, code = S.poisson(gu, stop_mode=1, rel_tol=1e-6, check_every=4)
+    assert code == 0 and st[5] == 1.0
+    P = N_(p)
+    res = b - orc64.apply_A(P)
+    assert np.linalg.norm(res) <= 5e-6 * np.linalg.norm(b)
+    assert scale_err(P - P.mean(), rp - rp.mean()) < 1e-4
+    # about as many iterations as the FP64 oracle's reference V-cycle needs
+    _, _, rst6 = orc64.poisson(u, levels=2, stop_mode=1, rel_tol=1e-6, max_iters=400)
+    assert st[0] <= rst6[0] + 3
+
+
+@pytest.mark.parametrize("mode", [1, 2])
 def test_a_smaller_solver_created_later_leaves_the_first_working(dev, mode):
     # both hierarchies end in the one-workgroup tail; the second one's tail needs
     # less dynamic LDS, and must not lower the kernel's limit below the first one's
