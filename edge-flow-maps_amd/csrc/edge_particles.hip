@@ -7,7 +7,7 @@
 #include <cstring>
 #include <map>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "edge_common.h"
 
@@ -295,7 +295,7 @@ int p2g_workspace(P2GWork& w, int64_t num, uint32_t ntiles, int end_bit, hipStre
     w.tiles_cap = ntiles;
   }
   size_t need = 0;
-  EDGE_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, w.k0, w.k1, w.v0, w.v1, (int)num, 0, end_bit, st));
+  EDGE_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, need, w.k0, w.k1, w.v0, w.v1, (size_t)num, 0u, (unsigned)end_bit, st));
   if (need > w.temp_bytes) {
     if (w.temp) (void)hipFree(w.temp);
     w.temp = nullptr;
@@ -452,7 +452,7 @@ int edge_lp_p2g(float* dragx, float* dragy, float* dragz, const float* diff, con
   hipLaunchKernelGGL(k_p2g_keys, dim3(blocks64(m)), dim3(256), 0, st, g, P, m, Lx, Ly, Lz, T, w.k0, w.v0);
   EDGE_CHECK_LAUNCH();
   size_t temp = w.temp_bytes;
-  EDGE_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(w.temp, temp, w.k0, w.k1, w.v0, w.v1, (int)m, 0, end_bit, st));
+  EDGE_CHECK_HIP(rocprim::radix_sort_pairs(w.temp, temp, w.k0, w.k1, w.v0, w.v1, (size_t)m, 0u, (unsigned)end_bit, st));
   w.perm_n = m;
   hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(nkeys)), dim3(256), 0, st, (float*)w.start, (int64_t)nkeys);
   hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(nkeys)), dim3(256), 0, st, (float*)w.end, (int64_t)nkeys);

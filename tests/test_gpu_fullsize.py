@@ -39,8 +39,37 @@ def test_full_size_step_against_the_oracle(dev):
     pairs = [(g, r32, r64, 1e-4) for g, r32, r64 in zip(sim.velocity_numpy(), ref32.u, ref64.u)]
     pairs.append((np.moveaxis(sim.smoke.to_numpy(), -1, 0), ref32.smoke, ref64.smoke, 1e-4))
     pairs.append((np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref32.psi, ref64.psi, 1e-5))
-    for g, r32, r64, floor in pairs:
+    for name, (g, r32, r64, floor) in zip(("u_x", "u_y", "u_z", "smoke", "psi"), pairs):
         e_gpu, e_32 = scale_err(g, r64), scale_err(r32, r64)
+        print(f"EDGE {name}: GPU vs FP64 oracle {e_gpu:.3g}, FP32 oracle vs FP64 {e_32:.3g}")
+        assert e_gpu <= max(3.0 * e_32, floor), (e_gpu, e_32)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("scheme", ["ed8", "ge"])
+def test_full_size_ed8_ge_step_against_the_oracle(dev, scheme):
+    """One ED8 (run_ed8.py:364-482) or GE (run_ge.py:363-481) step at
+    256x128x128 with converged projections, against the FP64 oracle's step,
+    with the FP32 oracle's own distance from it as the yardstick (as for EDGE
+    above)."""
+    from test_gpu_step import make
+    from oracle.step import OracleStep
+    sim, ref32 = make(RES, case="cfg2", scheme=scheme)
+    ref64 = OracleStep([np.asarray(c, np.float64) for c in ref32.u], ref32.smoke.astype(np.float64), sim.dx,
+                       sim.reinit_every, sim.CFL, sim.ed_eps, levels=sim.solver.n_mg_levels,
+                       BFECC_clamp=sim.BFECC_clamp, dtype=np.float64, stop_mode=1, rel_tol=1e-6, scheme=scheme)
+    dt = sim.substep()
+    torch.cuda.synchronize()
+    print(f"full-size {scheme} step dt {dt:.4g}; FP32 oracle step running", flush=True)
+    ref32.substep(dt=dt)
+    print("FP64 oracle step running", flush=True)
+    ref64.substep(dt=dt)
+    pairs = [(g, r32, r64, 1e-4) for g, r32, r64 in zip(sim.velocity_numpy(), ref32.u, ref64.u)]
+    pairs.append((np.moveaxis(sim.smoke.to_numpy(), -1, 0), ref32.smoke, ref64.smoke, 1e-4))
+    pairs.append((np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref32.psi, ref64.psi, 1e-5))
+    for name, (g, r32, r64, floor) in zip(("u_x", "u_y", "u_z", "smoke", "psi"), pairs):
+        e_gpu, e_32 = scale_err(g, r64), scale_err(r32, r64)
+        print(f"{scheme} {name}: GPU vs FP64 oracle {e_gpu:.3g}, FP32 oracle vs FP64 {e_32:.3g}")
         assert e_gpu <= max(3.0 * e_32, floor), (e_gpu, e_32)
 
 

@@ -41,15 +41,23 @@ def make(res, fixed_iters=None, case="cfg1", clamp=None, scheme="edge", dtype=np
 
 @pytest.mark.parametrize("clamp", [False, True])
 def test_one_step_fixed_iterations(dev, clamp):
+    """Two steps with 6 CG iterations per projection on both sides (so the
+    solve's stopping point cannot differ): the velocity and the pressure of
+    every step against the FP32 oracle."""
     sim, ref = make((32, 16, 16), fixed_iters=6, clamp=clamp)
-    for _ in range(2):
+    for step in range(2):
         dt = sim.substep()
         ref.substep(dt=dt)
-    torch.cuda.synchronize()
-    for g, r in zip(sim.velocity_numpy(), ref.u):
-        assert scale_err(g, r) < 1e-4
-    assert scale_err(np.moveaxis(sim.smoke.to_numpy(), -1, 0), ref.smoke) < 1e-4
-    assert scale_err(np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref.psi) < 1e-5
+        torch.cuda.synchronize()
+        eu = max(scale_err(g, r) for g, r in zip(sim.velocity_numpy(), ref.u))
+        P = sim.solver.p.to_numpy().astype(np.float64)
+        ep = scale_err(P - P.mean(), ref.p - ref.p.mean())
+        es = scale_err(np.moveaxis(sim.smoke.to_numpy(), -1, 0), ref.smoke)
+        em = scale_err(np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref.psi)
+        print(f"clamp {clamp} step {step}: velocity {eu:.3g}, pressure {ep:.3g}, smoke {es:.3g}, map {em:.3g}")
+        assert eu < 1e-4 and ep < 1e-4
+        assert es < 1e-4
+        assert em < 1e-5
 
 
 @pytest.mark.parametrize("scheme", ["ed8", "ge"])
@@ -133,6 +141,27 @@ def test_particle_step_fixed_iterations(dev):
     assert scale_err(np.moveaxis(sim.laden_particles_vel.to_numpy(), -1, 0), ref.vel) < 1e-4
     assert scale_err(np.moveaxis(sim.laden_particles_pos.to_numpy(), -1, 0), ref.pos) < 1e-6
     assert np.abs(ref.vel).max() > 0
+
+
+def test_cfg1_fifty_steps_at_its_stated_size(dev):
+    """cfg1 as stated: 64x32x32, dx = 1/32, two coaxial rings, 4 multigrid
+    levels, reinit every 12, both projections converged to ||r|| <= 1e-6 ||b||
+    on both sides, 50 steps against the FP32 oracle.  The error series is
+    printed; after 10 steps it is within the trajectory bound 1e-3."""
+    sim, ref = make((64, 32, 32))
+    assert sim.solver.n_mg_levels == 4 and sim.reinit_every == 12
+    errs = []
+    for s in range(50):
+        dt = sim.substep()
+        ref.substep(dt=dt)
+        torch.cuda.synchronize()
+        errs.append(max(scale_err(g, r) for g, r in zip(sim.velocity_numpy(), ref.u)))
+        if s == 9:
+            assert errs[-1] < 1e-3, errs
+    print("cfg1 64x32x32 velocity error per step:", " ".join(f"{e:.2g}" for e in errs))
+    print("GPU iterations", [st["iters"] for st in sim.poisson_stats], "oracle", ref.iters)
+    assert all(st["converged"] for st in sim.poisson_stats)
+    assert max(errs) < 1e-2
 
 
 def test_ten_step_trajectory(dev):
