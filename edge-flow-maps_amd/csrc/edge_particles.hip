@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -311,6 +312,130 @@ __global__ void __launch_bounds__(256) k_lp_zero(float* __restrict__ f, int64_t 
   if (t < n) f[t] = 0.f;
 }
 
+int64_t blocks64(int64_t n) { return (n + 255) / 256; }
+
+// One scatter's validated arguments and derived sizes.
+struct P2GCall {
+  Grid g;
+  CP3 P, D;
+  P3 drag;
+  const float* rr;
+  float radius, Lx, Ly, Lz;
+  Tiles T;
+  uint32_t ntiles, nkeys;
+  int end_bit;
+  int64_t m;
+  hipStream_t st;
+};
+
+// The sort keys and values (double-buffered), the cell ranges and the sort's
+// temporary storage of one scatter.
+struct P2GBufs {
+  uint32_t *k0, *k1, *v0, *v1, *start, *end;
+  void* temp;
+  size_t temp_bytes;
+};
+
+// Argument checks (host only, before any launch) and the sizes the sort and
+// the tiles need.
+int p2g_prepare(const char* who, float* dragx, float* dragy, float* dragz, const float* diff, const float* pos,
+                const float* ref_radius, long n, long num, float dx, float radius, float Lx, float Ly, float Lz, int nx,
+                int ny, int nz, void* stream, P2GCall& c) {
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "%s: bad dims", who);
+  EDGE_REQUIRE(dragx && dragy && dragz && diff && pos && ref_radius && n >= 0 && num >= 0 && n < (1L << 31),
+               "%s: bad arguments", who);
+  EDGE_REQUIRE(radius != 0.0f && dx > 0.0f, "%s: radius and dx must be non-zero", who);
+  // the 3 dx wall margin is what keeps every tap in the grid: the domain must match the grid
+  EDGE_REQUIRE(Lx <= (float)nx * dx * 1.0001f && Ly <= (float)ny * dx * 1.0001f && Lz <= (float)nz * dx * 1.0001f,
+               "%s: domain (%g, %g, %g) larger than the grid (%d, %d, %d) x %g", who, Lx, Ly, Lz, nx, ny, nz, dx);
+  const Tiles T{(nx + TILE - 1) / TILE, (ny + TILE - 1) / TILE, (nz + TILE - 1) / TILE};
+  EDGE_REQUIRE((int64_t)T.tx * T.ty * T.tz * TCELLS < (1LL << 31), "%s: grid too large for 32-bit cell keys", who);
+  c.st = (hipStream_t)stream;
+  c.g = make_grid(nx, ny, nz, dx);
+  float* dr[3] = {dragx, dragy, dragz};
+  for (int a = 0; a < 3; ++a) {
+    c.P.v[a] = pos + a * n;
+    c.D.v[a] = diff + a * n;
+    c.drag.v[a] = dr[a];
+  }
+  c.rr = ref_radius;
+  c.radius = radius;
+  c.Lx = Lx;
+  c.Ly = Ly;
+  c.Lz = Lz;
+  c.T = T;
+  c.ntiles = (uint32_t)T.tx * T.ty * T.tz;
+  c.nkeys = c.ntiles * TCELLS;   // tile-major cell ids; nkeys = not scattered
+  c.end_bit = 1;
+  while ((1u << c.end_bit) <= c.nkeys) ++c.end_bit;
+  c.m = std::min<int64_t>(num, n);
+  return EDGE_OK;
+}
+
+// The drag grids zeroed (every call, also without live particles).
+int p2g_zero(const P2GCall& c) {
+  for (int a = 0; a < 3; ++a)
+    hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(c.g.nf[a])), dim3(256), 0, c.st, c.drag.v[a], (int64_t)c.g.nf[a]);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+// Keys, the stable sort by cell, the cells' runs and the tile scatter.
+int p2g_launch(const P2GCall& c, const P2GBufs& b) {
+  hipLaunchKernelGGL(k_p2g_keys, dim3(blocks64(c.m)), dim3(256), 0, c.st, c.g, c.P, c.m, c.Lx, c.Ly, c.Lz, c.T, b.k0,
+                     b.v0);
+  EDGE_CHECK_LAUNCH();
+  size_t temp = b.temp_bytes;
+  EDGE_CHECK_HIP(rocprim::radix_sort_pairs(b.temp, temp, b.k0, b.k1, b.v0, b.v1, (size_t)c.m, 0u,
+                                           (unsigned)c.end_bit, c.st));
+  hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(c.nkeys)), dim3(256), 0, c.st, (float*)b.start, (int64_t)c.nkeys);
+  hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(c.nkeys)), dim3(256), 0, c.st, (float*)b.end, (int64_t)c.nkeys);
+  hipLaunchKernelGGL(k_p2g_ranges, dim3(blocks64(c.m)), dim3(256), 0, c.st, b.k1, c.m, c.nkeys, b.start, b.end);
+  hipLaunchKernelGGL(k_p2g_cells, dim3(c.ntiles), dim3(TCELLS), 0, c.st, c.g, c.drag, c.D, c.P, c.rr, c.radius, c.T,
+                     b.v1, b.start, b.end);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free scatter's storage: one file-scope device array per GPU
+// (each GPU holds its own copy from the library's load until the process
+// ends).  Layout: sort keys and values, double-buffered (4 x
+// NOALLOC_MAX_PARTICLES words), the cells' run starts and ends (2 x
+// NOALLOC_MAX_CELLS words), then the sort's temporary storage.  Every call on
+// a GPU shares it: calls that overlap on one GPU (two streams) race.  The
+// kernels clear what must start at zero (the run ends) on every call.
+// ---------------------------------------------------------------------------
+constexpr int64_t NOALLOC_MAX_PARTICLES = (int64_t)1 << 20;   // 1,048,576
+constexpr int64_t NOALLOC_MAX_CELLS = (int64_t)1 << 22;       // 4,194,304 tile-padded cells (256x128x128)
+constexpr size_t NOALLOC_SORT_OFFSET = (size_t)(4 * NOALLOC_MAX_PARTICLES + 2 * NOALLOC_MAX_CELLS) * 4;
+constexpr size_t NOALLOC_SORT_BYTES = (size_t)8 << 20;
+constexpr size_t NOALLOC_BYTES = NOALLOC_SORT_OFFSET + NOALLOC_SORT_BYTES;   // 56 MiB
+static_assert(NOALLOC_SORT_OFFSET % 256 == 0, "sort storage must stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch;   // device -> the array's address on that device
+
+// The array's device address on the current device, looked up once per device
+// (with that device current) and kept: the array's name in host code is only a
+// host-side stand-in, never a device address.
+int noalloc_scratch(unsigned char** out) {
+  int dev = 0;
+  EDGE_CHECK_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  if (it == g_scratch.end()) {
+    void* p = nullptr;
+    EDGE_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    EDGE_REQUIRE(p != nullptr && ((uintptr_t)p & 255) == 0, "edge_lp_p2g_noalloc: scratch address %p unusable", p);
+    it = g_scratch.emplace(dev, (unsigned char*)p).first;
+  }
+  *out = it->second;
+  return EDGE_OK;
+}
+
 // sim/lp.py:158-181 update_particles_position: x += v dt, clamped to the domain.
 __global__ void __launch_bounds__(256) k_lp_pos(CP3 vel, P3 pos, int64_t num, float dt, float Lx, float Ly, float Lz) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -365,7 +490,6 @@ __global__ void __launch_bounds__(256) k_lp_max_speed(CP3 vel, int64_t n, unsign
 
 __global__ void k_lp_set_u32(unsigned int* p, unsigned int v) { *p = v; }
 
-int64_t blocks64(int64_t n) { return (n + 255) / 256; }
 
 }  // namespace
 
@@ -422,45 +546,50 @@ int edge_lp_p2g(float* dragx, float* dragy, float* dragz, const float* diff, con
                 long n, long num, float dx, float radius, float Lx, float Ly, float Lz, int nx, int ny, int nz,
                 void* stream) {
   clear_error();
-  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_lp_p2g: bad dims");
-  EDGE_REQUIRE(dragx && dragy && dragz && diff && pos && ref_radius && n >= 0 && num >= 0 && n < (1L << 31),
-               "edge_lp_p2g: bad arguments");
-  EDGE_REQUIRE(radius != 0.0f && dx > 0.0f, "edge_lp_p2g: radius and dx must be non-zero");
-  // the 3 dx wall margin is what keeps every tap in the grid: the domain must match the grid
-  EDGE_REQUIRE(Lx <= (float)nx * dx * 1.0001f && Ly <= (float)ny * dx * 1.0001f && Lz <= (float)nz * dx * 1.0001f,
-               "edge_lp_p2g: domain (%g, %g, %g) larger than the grid (%d, %d, %d) x %g", Lx, Ly, Lz, nx, ny, nz, dx);
-  hipStream_t st = (hipStream_t)stream;
-  Grid g = make_grid(nx, ny, nz, dx);
-  float* dr[3] = {dragx, dragy, dragz};
-  for (int a = 0; a < 3; ++a)
-    hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(g.nf[a])), dim3(256), 0, st, dr[a], (int64_t)g.nf[a]);
-  EDGE_CHECK_LAUNCH();
-  int64_t m = std::min(num, n);
-  if (m == 0) return EDGE_OK;
-  const Tiles T{(nx + TILE - 1) / TILE, (ny + TILE - 1) / TILE, (nz + TILE - 1) / TILE};
-  EDGE_REQUIRE((int64_t)T.tx * T.ty * T.tz * TCELLS < (1LL << 31), "edge_lp_p2g: grid too large for 32-bit cell keys");
-  const uint32_t ntiles = (uint32_t)T.tx * T.ty * T.tz;
-  const uint32_t nkeys = ntiles * TCELLS;   // tile-major cell ids; nkeys = not scattered
-  int end_bit = 1;
-  while ((1u << end_bit) <= nkeys) ++end_bit;
+  P2GCall c;
+  int rc = p2g_prepare("edge_lp_p2g", dragx, dragy, dragz, diff, pos, ref_radius, n, num, dx, radius, Lx, Ly, Lz, nx,
+                       ny, nz, stream, c);
+  if (!rc) rc = p2g_zero(c);
+  if (rc || c.m == 0) return rc;
   int dev = 0;
   EDGE_CHECK_HIP(hipGetDevice(&dev));
   P2GWork& w = g_p2g[dev];
-  int rc = p2g_workspace(w, m, nkeys, end_bit, st);
+  rc = p2g_workspace(w, c.m, c.nkeys, c.end_bit, c.st);
   if (rc) return rc;
-  const CP3 P{{pos, pos + n, pos + 2 * n}}, D{{diff, diff + n, diff + 2 * n}};
-  hipLaunchKernelGGL(k_p2g_keys, dim3(blocks64(m)), dim3(256), 0, st, g, P, m, Lx, Ly, Lz, T, w.k0, w.v0);
-  EDGE_CHECK_LAUNCH();
-  size_t temp = w.temp_bytes;
-  EDGE_CHECK_HIP(rocprim::radix_sort_pairs(w.temp, temp, w.k0, w.k1, w.v0, w.v1, (size_t)m, 0u, (unsigned)end_bit, st));
-  w.perm_n = m;
-  hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(nkeys)), dim3(256), 0, st, (float*)w.start, (int64_t)nkeys);
-  hipLaunchKernelGGL(k_lp_zero, dim3(blocks64(nkeys)), dim3(256), 0, st, (float*)w.end, (int64_t)nkeys);
-  hipLaunchKernelGGL(k_p2g_ranges, dim3(blocks64(m)), dim3(256), 0, st, w.k1, m, nkeys, w.start, w.end);
-  hipLaunchKernelGGL(k_p2g_cells, dim3(ntiles), dim3(TCELLS), 0, st, g, P3{{dragx, dragy, dragz}}, D, P, ref_radius,
-                     radius, T, w.v1, w.start, w.end);
-  EDGE_CHECK_LAUNCH();
+  rc = p2g_launch(c, P2GBufs{w.k0, w.k1, w.v0, w.v1, w.start, w.end, w.temp, w.temp_bytes});
+  if (rc) return rc;
+  w.perm_n = c.m;
   return EDGE_OK;
+}
+
+int edge_lp_p2g_noalloc(float* dragx, float* dragy, float* dragz, const float* diff, const float* pos,
+                        const float* ref_radius, long n, long num, float dx, float radius, float Lx, float Ly,
+                        float Lz, int nx, int ny, int nz, void* stream) {
+  clear_error();
+  P2GCall c;
+  int rc = p2g_prepare("edge_lp_p2g_noalloc", dragx, dragy, dragz, diff, pos, ref_radius, n, num, dx, radius, Lx, Ly,
+                       Lz, nx, ny, nz, stream, c);
+  if (rc) return rc;
+  EDGE_REQUIRE(c.m <= NOALLOC_MAX_PARTICLES,
+               "edge_lp_p2g_noalloc: %lld particles exceed its maximum of %lld (use edge_lp_p2g)", (long long)c.m,
+               (long long)NOALLOC_MAX_PARTICLES);
+  EDGE_REQUIRE((int64_t)c.nkeys <= NOALLOC_MAX_CELLS,
+               "edge_lp_p2g_noalloc: the grid's %u tile-padded cells exceed its maximum of %lld (use edge_lp_p2g)",
+               c.nkeys, (long long)NOALLOC_MAX_CELLS);
+  size_t need = 0;
+  EDGE_CHECK_HIP(rocprim::radix_sort_pairs(nullptr, need, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                           (uint32_t*)nullptr, (size_t)c.m, 0u, (unsigned)c.end_bit, c.st));
+  EDGE_REQUIRE(need <= NOALLOC_SORT_BYTES, "edge_lp_p2g_noalloc: the sort needs %zu bytes of its %zu", need,
+               (size_t)NOALLOC_SORT_BYTES);
+  unsigned char* base = nullptr;
+  rc = noalloc_scratch(&base);
+  if (!rc) rc = p2g_zero(c);
+  if (rc || c.m == 0) return rc;
+  uint32_t* u = (uint32_t*)base;
+  const P2GBufs b{u, u + NOALLOC_MAX_PARTICLES, u + 2 * NOALLOC_MAX_PARTICLES, u + 3 * NOALLOC_MAX_PARTICLES,
+                  u + 4 * NOALLOC_MAX_PARTICLES, u + 4 * NOALLOC_MAX_PARTICLES + NOALLOC_MAX_CELLS,
+                  (void*)(base + NOALLOC_SORT_OFFSET), (size_t)NOALLOC_SORT_BYTES};
+  return p2g_launch(c, b);
 }
 
 int edge_lp_pos(const float* vel, float* pos, long n, long num, float dt, float Lx, float Ly, float Lz, void* stream) {

@@ -90,6 +90,7 @@ SIGNATURES = {
     "edge_lp_vel": "pppllfffffp",
     "edge_lp_vel_diff": "pppiiipppllffffp",
     "edge_lp_p2g": "ppppppllfffffiiip",
+    "edge_lp_p2g_noalloc": "ppppppllfffffiiip",
     "edge_lp_pos": "ppllffffp",
     "edge_lp_viscous": "ppppppiiiffp",
     "edge_lp_max_speed": "plpp",

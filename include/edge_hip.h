@@ -291,6 +291,17 @@ int edge_lp_vel_diff(const float* ux, const float* uy, const float* uz, int nx, 
 int edge_lp_p2g(float* dragx, float* dragy, float* dragz, const float* diff, const float* pos, const float* ref_radius,
                 long n, long num, float dx, float radius, float Lx, float Ly, float Lz, int nx, int ny, int nz,
                 void* stream);
+/* No counterpart in the reference: edge_lp_p2g for callers that cannot
+ * allocate (code under HIP graph capture, tight stepping loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results; the sort keys, cell ranges
+ * and sort storage come from a file-scope device array (56 MiB per GPU, held
+ * until the process ends) instead of the library's grown workspace.  Limits:
+ * min(n, num) <= 1,048,576 particles and at most 4,194,304 tile-padded cells
+ * (ceil(n/8)*8 per axis, e.g. 256x128x128); larger calls return EDGE_EINVAL
+ * with a message.  Calls that overlap on one GPU (different streams) race. */
+int edge_lp_p2g_noalloc(float* dragx, float* dragy, float* dragz, const float* diff, const float* pos,
+                        const float* ref_radius, long n, long num, float dx, float radius, float Lx, float Ly,
+                        float Lz, int nx, int ny, int nz, void* stream);
 /* update_particles_position (sim/lp.py:158-181). */
 int edge_lp_pos(const float* vel, float* pos, long n, long num, float dt, float Lx, float Ly, float Lz, void* stream);
 /* calculate_viscous_force (sim/lp.py:183-218) on the three face grids. */
