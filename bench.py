@@ -50,12 +50,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 def march_workload(res, case, scheme="edge"):
     """The map march's workload (what the march traffic counters depend on)."""
-    what = {"cfg3": "delta-wing plate, oblique inflow", "cfg5": "one ring + laden particles"}.get(case, "four rings")
+    what = {"cfg3": "delta-wing plate, oblique inflow", "cfg3plate": "four rings + plate span 0.3, inflow 1 at 20 deg",
+            "cfg5": "one ring + passive dye particles", "cfg5laden": "one ring + laden particles"}.get(case, "four rings")
     return f"{scheme.upper()} step {res[0]}x{res[1]}x{res[2]} {what}"
 
 
 def workload_name(res, case, scheme="edge", solver="dense"):
-    solid = "solid " if case == "cfg3" else ""
+    solid = "solid " if case in ("cfg3", "cfg3plate") else ""
     return f"{march_workload(res, case, scheme)}, {solid}{solver} projection"
 
 
@@ -109,10 +110,12 @@ def parse():
     ap.add_argument("--no-io", action="store_true", help="skip timing one frame output and one checkpoint")
     ap.add_argument("--solver", default="dense", choices=["plain", "mixed", "dense"],
                     help="projection solver of the timed steps")
-    ap.add_argument("--case", default="cfg2", choices=["cfg2", "cfg3", "cfg5"],
-                    help="cfg2: the headline EDGE step; cfg3: delta-wing plate with the solid projection; "
-                         "cfg5: 512x256x256 with 16M laden particles")
-    ap.add_argument("--particles", type=int, default=None, help="particle count for --case cfg5")
+    ap.add_argument("--case", default="cfg2", choices=["cfg2", "cfg3", "cfg3plate", "cfg5", "cfg5laden"],
+                    help="cfg2: the headline EDGE step; cfg3: the reference's delta-wing plate with the solid "
+                         "projection; cfg3plate: the four rings plus a plate of span 0.3 in an inflow of speed 1 at "
+                         "20 degrees (solid projection); cfg5: 512x256x256 with 16M passive dye particles; "
+                         "cfg5laden: 512x256x256 with 16M laden particles")
+    ap.add_argument("--particles", type=int, default=None, help="particle count for --case cfg5 / cfg5laden")
     ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge", "efm"],
                     help="backward-map march: EDGE (run_edge.py), ED8 (run_ed8.py) or GE (run_ge.py)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -320,7 +323,8 @@ def main():
     import torch
     from run_edge import Simulation, freeze_setup_objects
 
-    res = tuple(a.res) if (a.case != "cfg5" or a.res != [256, 128, 128]) else (512, 256, 256)
+    big = a.case in ("cfg5", "cfg5laden")
+    res = tuple(a.res) if (not big or a.res != [256, 128, 128]) else (512, 256, 256)
     sim = Simulation(a.case, solver_mode=a.solver, stop="relative", rel_tol=1e-6, res=res, check_every=0,
                      num_particles=a.particles, scheme=a.scheme)
     sim.initialize()
@@ -433,11 +437,17 @@ def main():
         "dtype": "f32",
         "data": {"cfg2": "synthetic: four seeded vortex rings (Biot-Savart init on the GPU)",
                  "cfg3": "synthetic: uniform oblique inflow past a voxelised triangular plate",
-                 "cfg5": "synthetic: one vortex ring, particles uniform in its torus (numpy default_rng(0))"}[a.case],
+                 "cfg3plate": "synthetic: four seeded vortex rings in a uniform inflow (speed 1, 20 degrees) past a "
+                              "voxelised triangular plate of span 0.3",
+                 "cfg5": "synthetic: one vortex ring, passive dye particles uniform in its torus (numpy "
+                         "default_rng(0))",
+                 "cfg5laden": "synthetic: one vortex ring, laden particles uniform in its torus (numpy "
+                              "default_rng(0))"}[a.case],
         "config": {"workload": wl, "cells": ncell, "reinit_every": sim.reinit_every,
                    "poisson_stop": "||r|| <= 1e-6 ||b||",
                    "solver": SOLVER_NOTE[a.solver],
-                   **({"particles": sim.laden_particle_num} if sim.lp is not None else {})},
+                   **({"particles": sim.laden_particle_num} if sim.lp is not None else {}),
+                   **({"dye_particles": sim.dye_num} if sim.dye is not None else {})},
         "phases_ms": {k: round(v, 3) for k, v in ph.items()},
         "step_ms": [round(w, 3) for w in walls],
         "poisson_iters": step_iters,

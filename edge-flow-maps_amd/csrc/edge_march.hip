@@ -293,6 +293,23 @@ __global__ void __launch_bounds__(256) k_advance_points(Grid g, Mac U, float* __
   pts[2 * n + t] = p[2];
 }
 
+// Passive dye particles (no reference counterpart: the 512x256x256 torus
+// workload's 16M dye particles, which only follow the flow).  Each live
+// particle q < num moves like a point of the forward face map, the RK4 of
+// run_edge.py:336-357 with +dt and the B-spline MAC velocity at every stage
+// (G2P), and is then clamped into the domain [0, L] as sim/lp.py:158-181
+// clamps laden particles.  pos: 3 planes of n (the capacity).
+__global__ void __launch_bounds__(256) k_dye_advance(Grid g, Mac U, float* __restrict__ pos, int64_t n, int64_t num,
+                                                     float dt, float Lx, float Ly, float Lz) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= num) return;
+  float p[3] = {pos[q], pos[n + q], pos[2 * n + q]};
+  rk4_psi(g, U, p, dt);
+  const float L[3] = {Lx, Ly, Lz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) pos[a * n + q] = fminf(fmaxf(p[a], 0.f), L[a]);
+}
+
 // Compose n points (and their rows T, if given) with a map table
 // (sim/hermite.py:921-1046).
 __global__ void __launch_bounds__(256) k_compose_points(Grid g, const float4* __restrict__ tab,
@@ -642,6 +659,21 @@ int edge_advance_points(int mode, const float* ux, const float* uy, const float*
   Grid g = make_grid(nx, ny, nz, dx);
   hipLaunchKernelGGL(k_advance_points, dim3(blocks_for(n, 256)), dim3(256), 0, (hipStream_t)stream, g,
                      Mac{ux, uy, uz}, pts, Rows{{Tx, Ty, Tz}}, (int64_t)n, dt, mode);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_dye_advance(const float* ux, const float* uy, const float* uz, float* pos, long n, long num, int nx, int ny,
+                     int nz, float dx, float dt, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_dye_advance: bad dims");
+  EDGE_MAC_OK(ux, uy, uz);
+  EDGE_REQUIRE(pos && n >= 0 && num >= 0 && n < (1L << 31) && dx > 0.0f, "edge_dye_advance: bad arguments");
+  const int64_t m = std::min<int64_t>(num, n);
+  if (m == 0) return EDGE_OK;
+  Grid g = make_grid(nx, ny, nz, dx);
+  hipLaunchKernelGGL(k_dye_advance, dim3(blocks_for(m, 256)), dim3(256), 0, (hipStream_t)stream, g, Mac{ux, uy, uz},
+                     pos, (int64_t)n, m, dt, (float)nx * dx, (float)ny * dx, (float)nz * dx);
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }

@@ -29,6 +29,7 @@ SIGNATURES = {
     "edge_march_ed4": "pppppppppiiifffp",
     "edge_advance_map_faces": "pppppppppiiiffp",
     "edge_advance_points": "ipppppppliiiffp",
+    "edge_dye_advance": "pppplliiiffp",
     "edge_compose_points": "pppppliiifp",
     "edge_calc_ed4_garm": "ppppppppppppppplfp",
     "edge_calc_ed": "ippppppppppplfp",

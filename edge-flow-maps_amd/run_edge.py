@@ -9,9 +9,12 @@ Every kernel is a libedge_hip.so kernel; this file only sequences them.
 The case table keeps the reference's cases 0 and 1 (run_edge.py:25-60) and
 adds the benchmark configurations ("cfg1": 64x32x32 leapfrogging rings,
 "cfg2": 256x128x128 four seeded rings, "cfg3": the reference's delta-wing
-case 2 with the solid projection, "cfg5": 512x256x256 with 16M laden
-particles, whose steps add the particle forces of run_edge.py:627-641,661-667
-and the projection with a force buffer, run_edge.py:704-708).  Where the reference copies a field
+case 2 with the solid projection, "cfg3plate": the four rings plus a
+triangular plate of span 0.3 in an inflow of speed 1 at 20 degrees (solid
+projection), "cfg5": 512x256x256 with 16M passive dye particles that
+follow the flow, "cfg5laden": the same with 16M laden particles, whose steps
+add the particle forces of run_edge.py:627-641,661-667 and the projection
+with a force buffer, run_edge.py:704-708).  Where the reference copies a field
 whose source is dead afterwards (u <- tmp_u), the storages are exchanged
 instead (Field.swap_storage); the values are identical.
 
@@ -39,8 +42,9 @@ from sim import lp as lp_kernels  # noqa: E402
 from sim.edge_driver import backtrack_faces, clamp_smoke, split_psi_hermite  # noqa: E402
 from sim.hermite import (advect_ps_garm, advect_ps_hermite_ed8, advect_psi_garm_ed4_fusion_unroll,  # noqa: E402
                          advect_u_semi_lagrangian, reset_to_identity_2, reset_to_identity_3)
-from sim.init_conditions import (init_delta, init_four_vorts, init_rings, init_vorts_leapfrog,  # noqa: E402
-                                 seeded_rings, set_boundary_mask_delta)
+from sim.init_conditions import (add_inflow, init_delta, init_four_vorts, init_rings,  # noqa: E402
+                                 init_vorts_leapfrog, seeded_rings, set_boundary_mask_delta,
+                                 set_boundary_mask_plate)
 from sim.mgpcg import MGPCG_3  # noqa: E402
 from sim.mgpcg_solid import MGPCG_Solid_3  # noqa: E402
 from util import io_utils  # noqa: E402
@@ -63,10 +67,21 @@ CASES = {
     # reference case 2 (run_edge.py:61-79): the delta-wing plate in an oblique inflow, solid projection
     "cfg3": dict(init="delta", res=(256, 128, 128), visualize_dt=1e9, reinit_every=10, CFL=0.5, BFECC_clamp=True,
                  eps_frac=0.02, base_level=3, solid=True),
-    # 512x256x256, dx = 1/256, one ring (R 0.25, core 0.08) carrying 16,777,216 laden particles seeded
-    # uniformly in the torus of tube radius 0.08 (numpy default_rng(seed)); the particle constants are
-    # those of the reference's ink cases (run_edge.py:80-140) with Re and St set for this unit domain
-    "cfg5": dict(init="torus", res=(512, 256, 256), visualize_dt=1e9, reinit_every=12, CFL=0.25, BFECC_clamp=True,
+    # the 256x128x128 four-ring flow plus a triangular plate (span 0.3, flat, four cells thick) in a
+    # uniform inflow of speed 1 at a 20-degree angle of attack, solid projection
+    "cfg3plate": dict(init="plate", res=(256, 128, 128), visualize_dt=1e9, reinit_every=12, CFL=0.5,
+                      BFECC_clamp=True, eps_frac=0.025, base_level=3, solid=True),
+    # 512x256x256, dx = 1/256, one ring (R 0.25, core 0.08) carrying 16,777,216 passive dye particles
+    # seeded uniformly in the torus of tube radius 0.08 (numpy default_rng(seed)), moved every step
+    # like points of the forward face map (G2P of the velocity at each RK4 stage)
+    "cfg5": dict(init="torus_dye", res=(512, 256, 256), visualize_dt=1e9, reinit_every=12, CFL=0.5,
+                 BFECC_clamp=True, eps_frac=0.025, base_level=3,
+                 dye=dict(max_num=16777216, ring_R=0.25, tube=0.08)),
+    # the same ring carrying 16,777,216 laden (inertial, two-way coupled) particles instead: the
+    # particle constants are those of the reference's ink cases (run_edge.py:80-140) with Re and St
+    # set for this unit domain
+    "cfg5laden": dict(init="torus", res=(512, 256, 256), visualize_dt=1e9, reinit_every=12, CFL=0.25,
+                      BFECC_clamp=True,
                  eps_frac=0.025, base_level=3,
                  lp=dict(max_num=16777216, Re=1.0e4, St=5.0e-3, gravity=1.0, density_ratio=0.6,
                          epiral_radius_ratio=35.0, laden_radius=0.01, ring_R=0.25, tube=0.08)),
@@ -216,6 +231,13 @@ class Simulation:
             self.drag_x, self.drag_y, self.drag_z = F(float, fx), F(float, fy), F(float, fz)
             self.vis_x, self.vis_y, self.vis_z = F(float, fx), F(float, fy), F(float, fz)
             self.laden_particle_num = n
+        # passive dye particles (the 512x256x256 torus case): positions only
+        self.dye = dict(cfg["dye"]) if "dye" in cfg else None
+        if self.dye is not None:
+            if num_particles is not None:
+                self.dye["max_num"] = int(num_particles)
+            self.dye_pos = V(3, float, self.dye["max_num"])
+            self.dye_num = self.dye["max_num"]
         self.num_samples = num_samples
         self.seed = seed
         self.i = -1
@@ -253,6 +275,21 @@ class Simulation:
             # run_edge.py:545-548 (the per-step dye emission of run_edge.py:643-648 is not part of this build)
             init_delta(self.X, self.u, self.smoke, self.tmp_smoke)
             set_boundary_mask_delta(self.X, self.boundary_mask, self.boundary_vel, 0, self.dx)
+        elif init == "plate":
+            dom = (self.res_x * self.dx, self.res_y * self.dx, self.res_z * self.dx)
+            init_rings(self.X, self.u, self.smoke, self.tmp_smoke, seeded_rings(4, dom, seed=self.seed),
+                       num_samples=self.num_samples)
+            add_inflow(self.u)
+            set_boundary_mask_plate(self.boundary_mask, self.boundary_vel, self.dx)
+        elif init == "torus_dye":
+            dom = np.array([self.res_x, self.res_y, self.res_z]) * self.dx
+            c = dom / 2
+            R, tube = self.dye["ring_R"], self.dye["tube"]
+            ring = dict(w=1.0, rad=R, delta=tube, c=tuple(c), ex=(0.0, 0.0, -1.0), ey=(0.0, 1.0, 0.0))
+            init_rings(self.X, self.u, self.smoke, self.tmp_smoke, [ring], num_samples=self.num_samples)
+            pos = torus_particles(self.dye_num, c, R, tube, seed=self.seed, dx=self.dx,
+                                  res=(self.res_x, self.res_y, self.res_z))
+            self.dye_pos.from_numpy(pos.T)
         elif init == "torus":
             dom = np.array([self.res_x, self.res_y, self.res_z]) * self.dx
             c = dom / 2
@@ -349,6 +386,10 @@ class Simulation:
         self._mark("phi_march")
         march_phi_grid(self.phi_x, self.phi_y, self.phi_z, self.F_x, self.F_y, self.F_z, self.u_x, self.u_y,
                        self.u_z, dx, curr_dt)
+        if self.dye is not None:
+            # the dye moves with the same velocity and RK4 as the face map's points
+            self._mark("dye")
+            lp_kernels.advect_dye_particles(self.u_x, self.u_y, self.u_z, self.dye_pos, dx, curr_dt, self.dye_num)
 
         self._mark("maps")
         if self.lp is not None:
@@ -512,6 +553,9 @@ class Simulation:
         if self.lp is not None and particle_vtkdir is not None:
             pos = self.laden_particles_pos.to_numpy()[:self.laden_particle_num, :]
             io_utils.write_to_vtks_lp(pos, {}, {}, os.path.join(particle_vtkdir, "field_{:03d}".format(frame)))
+        if self.dye is not None and particle_vtkdir is not None:
+            pos = self.dye_pos.to_numpy()[:self.dye_num, :]
+            io_utils.write_to_vtks_lp(pos, {}, {}, os.path.join(particle_vtkdir, "field_{:03d}".format(frame)))
 
     def _state_fields(self):
         """Fields a step reads before it writes them, besides u and smoke: the two maps,
@@ -523,6 +567,8 @@ class Simulation:
             names += ["d2T_x", "d2T_y", "d2T_z"]
         if self.scheme == "efm":
             names += [f"u_{a}_cache_{s}" for s in range(self.reinit_every) for a in "xyz"]
+        if self.dye is not None:
+            names += ["dye_pos"]
         if self.lp is not None:
             names += ["laden_particles_pos", "laden_particles_vel", "laden_particles_acc",
                       "laden_particles_vel_diff", "laden_particle_ref_radius", "vis_x", "vis_y", "vis_z"]
@@ -600,7 +646,7 @@ class Simulation:
         steps until frame total_frames, writing vtks/field_<frame>.vti (and particle/ files)
         on every output frame and ckpts/ every ckpt_every frames."""
         vtkdir, ckptdir = os.path.join(logsdir, "vtks"), os.path.join(logsdir, "ckpts")
-        pdir = os.path.join(logsdir, "particle") if self.lp is not None else None
+        pdir = os.path.join(logsdir, "particle") if (self.lp is not None or self.dye is not None) else None
         for d in (vtkdir, ckptdir) + ((pdir,) if pdir else ()):
             os.makedirs(d, exist_ok=True)
         self.initialize()

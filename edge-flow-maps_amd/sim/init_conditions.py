@@ -178,3 +178,68 @@ def init_delta(X, u, smoke1, smoke2):
     v[...] = (DELTA_SPEED * np.cos(a), 0.0, DELTA_SPEED * np.sin(a))
     u.from_numpy(v)
     smoke1.fill(0.0)
+
+
+# ---------------------------------------------------------------------------
+# The triangular plate of the 256x128x128 solid workload: the reference's
+# delta shape (half-angle 15 degrees, flat, four cells thick, the outer cell
+# layer moving with the inflow) at span 0.3, in an inflow of speed 1 at a
+# 20-degree angle of attack, added to the four seeded rings
+# ---------------------------------------------------------------------------
+PLATE_SPAN = 0.3
+PLATE_SPEED = 1.0
+PLATE_ATTACK_DEG = 20.0
+PLATE_HALF_ANGLE_DEG = 15.0
+PLATE_APEX = (0.3, 0.5, 0.5)   # leading-edge apex; the plate spans y symmetrically, centred in z
+
+
+def plate_vertices(span=PLATE_SPAN, half_angle_deg=PLATE_HALF_ANGLE_DEG):
+    """Apex at the origin, trailing edge of width `span` downstream (+x)."""
+    a = np.radians(half_angle_deg)
+    length = span / (2.0 * np.sin(a))
+    x, y = np.cos(a) * length, np.sin(a) * length
+    return np.array([[0.0, 0.0, 0.0], [x, y, 0.0], [x, -y, 0.0]])
+
+
+def plate_mask(shape, dx, span=PLATE_SPAN, apex=PLATE_APEX, thickness=4):
+    """The voxelised plate (the trimesh rule of voxelize_triangle), apex cell at
+    apex // dx, centred in y and z, `thickness` cells thick in z; the box's
+    outer layer is not included."""
+    plate, _ = voxelize_triangle(plate_vertices(span), dx)
+    size = plate.shape
+    off = np.array([apex[0] // dx, apex[1] // dx - size[1] // 2, round(apex[2] / dx) - thickness // 2], np.int64)
+    if np.any(off < 1) or off[0] + size[0] > shape[0] - 1 or off[1] + size[1] > shape[1] - 1 or \
+            off[2] + thickness > shape[2] - 1:
+        raise ValueError(f"the plate (span {span}) does not fit the {shape} grid at dx {dx}")
+    mask = np.zeros(shape, np.int32)
+    for t in range(thickness):
+        mask[off[0]:off[0] + size[0], off[1]:off[1] + size[1], off[2] + t] |= plate[:, :, 0].astype(np.int32)
+    return mask
+
+
+def plate_inflow(speed=PLATE_SPEED, attack_deg=PLATE_ATTACK_DEG):
+    a = np.radians(attack_deg)
+    return np.array([speed * np.cos(a), 0.0, speed * np.sin(a)], np.float32)
+
+
+def set_boundary_mask_plate(boundary_mask, boundary_vel, dx):
+    """The plate at rest and the box's outer cell layer, which moves with the
+    inflow (the reference's delta-case boundary, sim/init_conditions.py:127-167)."""
+    shape = boundary_mask.shape
+    mask = plate_mask(shape, dx)
+    vel = np.zeros(shape + (3,), np.float32)
+    edge = np.zeros(shape, bool)
+    edge[[0, -1], :, :] = True
+    edge[:, [0, -1], :] = True
+    edge[:, :, [0, -1]] = True
+    mask[edge] = 1
+    vel[edge] = plate_inflow()
+    boundary_mask.from_numpy(mask)
+    boundary_vel.from_numpy(vel)
+
+
+def add_inflow(u):
+    """u += the uniform inflow (cell-centred field)."""
+    v = u.to_numpy()
+    v += plate_inflow()
+    u.from_numpy(v)

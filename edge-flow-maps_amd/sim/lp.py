@@ -72,3 +72,13 @@ def calculate_viscous_force(u_x, u_y, u_z, vis_x, vis_y, vis_z, dx, Re):
     _lib.call("edge_lp_viscous", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), as_ptr(vis_x), as_ptr(vis_y), as_ptr(vis_z),
               *_cells(u_x), float(dx), float(Re), stream_handle())
     touch(vis_x, vis_y, vis_z)
+
+
+def advect_dye_particles(u_x, u_y, u_z, dye_pos, dx, curr_dt, dye_num):
+    """Passive dye (no reference counterpart; the 512x256x256 torus case's dye):
+    each live particle q < dye_num takes the RK4 step of a forward face-map
+    point (run_edge.py:336-357, +dt, the B-spline MAC velocity at every stage)
+    and is clamped into the domain (as sim/lp.py:158-181 clamps positions)."""
+    _lib.call("edge_dye_advance", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), as_ptr(dye_pos), _n(dye_pos), int(dye_num),
+              *_cells(u_x), float(dx), float(curr_dt), stream_handle())
+    touch(dye_pos)

@@ -80,6 +80,13 @@ int edge_backtrack_faces(const float* const* ux, const float* const* uy, const f
  * advance_map_1_transpose (sim/hermite.py:863-918).  pts: 3 planes of n. */
 int edge_advance_points(int mode, const float* ux, const float* uy, const float* uz, float* pts, float* Tx,
                         float* Ty, float* Tz, long n, int nx, int ny, int nz, float dx, float dt, void* stream);
+/* No counterpart in the reference (its particles are the inertial ones of
+ * sim/lp.py): passive dye particles for the 512x256x256 torus workload.  Each
+ * live particle q < num (pos: 3 planes of n) takes the RK4 step of a forward
+ * face-map point (run_edge.py:336-357, +dt, B-spline MAC velocity at every
+ * stage) and is clamped into [0, n_a dx] per axis (sim/lp.py:158-181). */
+int edge_dye_advance(const float* ux, const float* uy, const float* uz, float* pos, long n, long num, int nx, int ny,
+                     int nz, float dx, float dt, void* stream);
 /* interp_psi_hermite(_garm)(_fd_unroll) (sim/hermite.py:921-1046): compose n
  * points (and their rows, if T* given) with a map table. */
 int edge_compose_points(const float* tab, float* pts, float* Tx, float* Ty, float* Tz, long n, int nx, int ny,

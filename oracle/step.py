@@ -16,6 +16,20 @@ def face_coords(a, nx, ny, nz, dx):
     return np.stack([(i + o[0]) * dx, (j + o[1]) * dx, (k + o[2]) * dx])
 
 
+def dye_advance(o, u, pos, num, dx, dt):
+    """Passive dye: each live particle (q < num of the (3, n) positions) takes
+    advance_map_0's RK4 step with +dt (the forward face map's point march,
+    run_edge.py:336-357) and is clamped into [0, n dx] per axis (the clamp of
+    sim/lp.py:158-181)."""
+    out = np.array(pos, copy=True)
+    if num > 0:
+        live = o.advance_map_0(np.ascontiguousarray(out[:, :num]), u, dx, dt)
+        nx, ny, nz = u[0].shape[0] - 1, u[0].shape[1], u[0].shape[2]
+        L = np.array([nx * dx, ny * dx, nz * dx], out.dtype)[:, None]
+        out[:, :num] = np.minimum(np.maximum(live, 0), L)
+    return out
+
+
 class OracleStep:
     """Fields of run_edge.py:174-254 as numpy arrays; `substep()` = one loop iteration."""
 
@@ -38,6 +52,7 @@ class OracleStep:
         self.scheme = scheme   # "edge" (run_edge.py), "ed8" (run_ed8.py), "ge" (run_ge.py), "efm" (run_efm.py)
         self.u_cache = [None] * reinit_every   # EFM: u after each step's first projection
         self.dts = [0.0] * reinit_every
+        self.dye = None
 
     def set_particles(self, pos, vel, ref_radius, num, params):
         """Laden particles (run_edge.py:235-254): pos/vel (3, n); params as the driver's lp dict."""
@@ -47,6 +62,11 @@ class OracleStep:
         self.rr = np.ascontiguousarray(ref_radius, self.dt_)
         self.num = int(num)
         self.domain = (self.nx * self.dx, self.ny * self.dx, self.nz * self.dx)
+
+    def set_dye(self, pos, num):
+        """Passive dye particles: positions (3, n), the first num live."""
+        self.dye = np.ascontiguousarray(pos, self.dt_)
+        self.dye_num = int(num)
 
     def march_full_map(self, dt):
         """ED8 (sim/hermite.py:1290-1338) or GE (1398-1424), unfused, then
@@ -136,6 +156,8 @@ class OracleStep:
             self.march_full_map(dt)
         for a in range(3):
             self.phi[a], self.F[a] = o.advance_map_col(self.phi[a], self.F[a], self.u, dx, dt)
+        if self.dye is not None:
+            self.dye = dye_advance(o, self.u, self.dye, self.dye_num, dx, dt)
         if self.lp is not None:
             # run_edge.py:661-667
             tmp = o.map_u(self.vis, self.F, self.phi, dx)

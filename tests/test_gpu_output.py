@@ -88,9 +88,9 @@ def test_vtk_frame_one_channel_and_no_smoke(dev):
     assert scale_err(got_w, np.linalg.norm(w, axis=0)) < 1e-6
 
 
-@pytest.mark.parametrize("case", ["cfg1", "cfg5", "efm"])
+@pytest.mark.parametrize("case", ["cfg1", "cfg5laden", "efm"])
 def test_resume_mid_reinit_continues_bit_for_bit(dev, tmp_path, case):
-    kw = {"cfg5": dict(case="cfg5", res=(32, 16, 16), num_particles=3000),
+    kw = {"cfg5laden": dict(case="cfg5laden", res=(32, 16, 16), num_particles=3000),
           "efm": dict(scheme="efm", reinit_every=4)}.get(case, {})
     a = _sim(**kw)
     a.initialize()
@@ -109,7 +109,7 @@ def test_resume_mid_reinit_continues_bit_for_bit(dev, tmp_path, case):
         b.substep()
     got = _state(b)
     for k in want:
-        if case != "cfg5":
+        if case != "cfg5laden":
             assert np.array_equal(got[k], want[k]), k
         else:
             # the drag scatter adds tile borders with FP32 atomics, in an order that

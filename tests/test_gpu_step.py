@@ -41,22 +41,38 @@ def make(res, fixed_iters=None, case="cfg1", clamp=None, scheme="edge", dtype=np
 
 @pytest.mark.parametrize("clamp", [False, True])
 def test_one_step_fixed_iterations(dev, clamp):
-    """Two steps with 6 CG iterations per projection on both sides (so the
-    solve's stopping point cannot differ): the velocity and the pressure of
-    every step against the FP32 oracle."""
+    """Steps with 6 CG iterations per projection on both sides (so the solve's
+    stopping point cannot differ).  The first step starts from the same state
+    on both sides: its velocity and pressure are within 1e-5 of the FP32
+    oracle's, or, where FP32 rounding alone moves a value further, within 1.5x
+    the FP32 oracle's own distance from the FP64 oracle's step (the measured
+    values are printed).  The second step carries the first one's differences:
+    a two-step trajectory, held to 1e-4."""
+    from oracle.step import OracleStep
     sim, ref = make((32, 16, 16), fixed_iters=6, clamp=clamp)
+    ref64 = OracleStep([np.asarray(c, np.float64) for c in ref.u], ref.smoke.astype(np.float64), sim.dx,
+                       sim.reinit_every, sim.CFL, sim.ed_eps, levels=sim.solver.n_mg_levels, BFECC_clamp=sim.BFECC_clamp,
+                       dtype=np.float64, stop_mode=1, rel_tol=0.0, max_iters=6)
+    mf = lambda x: np.asarray(x, np.float64) - np.mean(x)
     for step in range(2):
         dt = sim.substep()
         ref.substep(dt=dt)
+        ref64.substep(dt=dt)
         torch.cuda.synchronize()
-        eu = max(scale_err(g, r) for g, r in zip(sim.velocity_numpy(), ref.u))
-        P = sim.solver.p.to_numpy().astype(np.float64)
-        ep = scale_err(P - P.mean(), ref.p - ref.p.mean())
+        gu = sim.velocity_numpy()
+        eu = max(scale_err(g, r) for g, r in zip(gu, ref.u))
+        yu = max(scale_err(r, r64) for r, r64 in zip(ref.u, ref64.u))
+        P = sim.solver.p.to_numpy()
+        ep, yp = scale_err(mf(P), mf(ref.p)), scale_err(mf(ref.p), mf(ref64.p))
         es = scale_err(np.moveaxis(sim.smoke.to_numpy(), -1, 0), ref.smoke)
         em = scale_err(np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref.psi)
-        print(f"clamp {clamp} step {step}: velocity {eu:.3g}, pressure {ep:.3g}, smoke {es:.3g}, map {em:.3g}")
-        assert eu < 1e-4 and ep < 1e-4
-        assert es < 1e-4
+        print(f"clamp {clamp} step {step}: velocity {eu:.3g} (FP32 oracle vs FP64 {yu:.3g}), pressure {ep:.3g} "
+              f"(FP32 oracle vs FP64 {yp:.3g}), smoke {es:.3g}, map {em:.3g}")
+        if step == 0:
+            assert eu <= max(1e-5, 1.5 * yu) and ep <= max(1e-5, 1.5 * yp)
+            assert es < 1e-5
+        else:
+            assert eu < 1e-4 and ep < 1e-4 and es < 1e-4
         assert em < 1e-5
 
 
@@ -121,7 +137,7 @@ def test_particle_step_fixed_iterations(dev):
     projection with a force buffer) against the oracle's composition."""
     from run_edge import Simulation
     from oracle.step import OracleStep
-    sim = Simulation("cfg5", solver_mode="plain", res=(32, 16, 16), num_samples=400, num_particles=3000)
+    sim = Simulation("cfg5laden", solver_mode="plain", res=(32, 16, 16), num_samples=400, num_particles=3000)
     sim.initialize()
     u0 = sim.velocity_numpy()
     s0 = np.moveaxis(sim.smoke.to_numpy(), -1, 0)
@@ -161,7 +177,7 @@ def test_cfg1_fifty_steps_at_its_stated_size(dev):
     print("cfg1 64x32x32 velocity error per step:", " ".join(f"{e:.2g}" for e in errs))
     print("GPU iterations", [st["iters"] for st in sim.poisson_stats], "oracle", ref.iters)
     assert all(st["converged"] for st in sim.poisson_stats)
-    assert max(errs) < 1e-2
+    assert max(errs) < 1e-3
 
 
 def test_ten_step_trajectory(dev):
@@ -281,3 +297,86 @@ def test_delta_case_solid_projection(dev, orc64):
         assert s.stats["rTr"] <= 1e-12 * s.stats["b2"] * 1.0001
         P = s.p.to_numpy().astype(np.float64)
         assert scale_err((P - P[fl].mean())[fl], (rp - rp[fl].mean())[fl]) < 1e-3, mode
+
+
+def _fluid_div(u, bm):
+    ux, uy, uz = [np.asarray(c, np.float64) for c in u]
+    div = (ux[1:] - ux[:-1]) + (uy[:, 1:] - uy[:, :-1]) + (uz[:, :, 1:] - uz[:, :, :-1])
+    return np.abs(div[bm == 0]).max()
+
+
+def test_plate_case_solid_projection(dev, orc64):
+    """The triangular plate (span 0.3, inflow 1 at 20 degrees, four rings) at
+    64x32x32: the plain, mixed (BF16 levels) and dense projections of one step's
+    velocity meet ||r|| <= 1e-6 ||b||, and their pressure is compared with the
+    FP64 oracle's solve of the same masked system (to 1e-9); the errors are
+    printed, with the pressure's own 1e-6-residual spread as the yardstick."""
+    from run_edge import Simulation
+    from sim.mgpcg_solid import MGPCG_Solid_3
+    from edge_hip import Field
+    from oracle import voxel_ref
+    from sim.init_conditions import plate_vertices
+    sim = Simulation("cfg3plate", solver_mode="mixed", res=(64, 32, 32), num_samples=400)
+    sim.initialize()
+    sim.rhs_snapshots = []
+    sim.substep()
+    u = [t.cpu().numpy().astype(np.float64) for t in sim.rhs_snapshots[-1]]
+    bm = sim.boundary_mask.to_numpy()
+    bvel = np.moveaxis(sim.boundary_vel.to_numpy(), -1, 0)
+    obj, _ = voxel_ref.voxelize_subdivide(plate_vertices(), np.array([[0, 1, 2]]), sim.dx)
+    assert bm[1:-1, 1:-1, 1:-1].sum() == 4 * obj.sum()
+    _, rp, rst = orc64.poisson(u, levels=sim.solver.n_mg_levels, bm=bm, bvel=bvel, max_iters=400, stop_mode=1,
+                               rel_tol=1e-9)
+    _, rp6, _ = orc64.poisson(u, levels=sim.solver.n_mg_levels, bm=bm, bvel=bvel, max_iters=400, stop_mode=1,
+                              rel_tol=1e-6)
+    fl = bm == 0
+    mf = lambda x: (x - x[fl].mean())[fl]
+    yard = scale_err(mf(rp6), mf(rp))
+    for mode in ("plain", "mixed", "dense"):
+        fs = []
+        for c in u:
+            f = Field(None, float, c.shape)
+            f.from_numpy(c.astype(np.float32))
+            fs.append(f)
+        s = MGPCG_Solid_3(np.full((3, 2), 2), sim.boundary_mask, sim.boundary_vel, [64, 32, 32],
+                          levels=sim.solver.n_mg_levels, mode=mode, stop="relative", rel_tol=1e-6)
+        s.Poisson(*fs, max_iters=200)
+        assert s.stats["converged"] == 1.0
+        assert s.stats["rTr"] <= 1e-12 * s.stats["b2"] * 1.0001
+        P = s.p.to_numpy().astype(np.float64)
+        e = scale_err(mf(P), mf(rp))
+        div = _fluid_div([f.to_numpy() for f in fs], bm)
+        print(f"plate 64x32x32 {mode}: {s.stats['iters']} iterations, pressure vs FP64 oracle {e:.3g} "
+              f"(FP64 oracle at 1e-6 vs 1e-9: {yard:.3g}), fluid divergence {div:.3g}")
+        assert e < 1e-3, mode
+
+
+def test_dye_steps_fixed_iterations(dev):
+    """Two steps of the passive-dye case (the torus ring with dye particles,
+    here 32x16x16 and 3000 particles) against the oracle's composition: the
+    velocity as in the plain step, the dye positions within 1e-6 of the domain
+    size (each a face-map point's RK4 through the same velocity)."""
+    from run_edge import Simulation
+    from oracle.step import OracleStep
+    sim = Simulation("cfg5", solver_mode="plain", res=(32, 16, 16), num_samples=400, num_particles=3000)
+    sim.initialize()
+    u0 = sim.velocity_numpy()
+    s0 = np.moveaxis(sim.smoke.to_numpy(), -1, 0)
+    pos0 = np.moveaxis(sim.dye_pos.to_numpy(), -1, 0)
+    sim.poisson_max_iters = 6
+    sim.solver.rel_tol = 0.0
+    ref = OracleStep(u0, s0, sim.dx, sim.reinit_every, sim.CFL, sim.ed_eps, levels=sim.solver.n_mg_levels,
+                     BFECC_clamp=sim.BFECC_clamp, stop_mode=1, rel_tol=0.0, max_iters=6)
+    ref.set_dye(pos0, sim.dye_num)
+    for _ in range(2):
+        dt = sim.substep()
+        ref.substep(dt=dt)
+    torch.cuda.synchronize()
+    for g, r in zip(sim.velocity_numpy(), ref.u):
+        assert scale_err(g, r) < 1e-4
+    got = np.moveaxis(sim.dye_pos.to_numpy(), -1, 0)
+    L = 32 * sim.dx
+    err = float(np.abs(got - ref.dye).max()) / L
+    moved = float(np.abs(ref.dye - pos0).max()) / sim.dx
+    print(f"dye 32x16x16: position error {err:.3g} of the domain, largest move {moved:.3g} cells")
+    assert err < 1e-6 and moved > 0.1

@@ -136,3 +136,32 @@ def test_cell_order_is_a_tile_major_permutation():
     a = torus_particles(3000, (1.0, 0.7, 0.5), 0.25, 0.08, seed=1)
     b = torus_particles(3000, (1.0, 0.7, 0.5), 0.25, 0.08, seed=1, dx=dx, res=res)
     assert np.array_equal(np.sort(a, axis=1), np.sort(b, axis=1))
+
+
+def test_dye_advance_known_answers(orc64):
+    """Passive dye: a constant flow c moves every particle by c dt (the RK4 of a
+    constant field is exact; B-spline partition of unity), and a particle that
+    would leave the box stops on its wall; a linear flow u = A x moves interior
+    particles to P(A dt) x (the 4th-order Taylor polynomial of exp(A dt)), with
+    +dt (the forward map's direction, not the backward map's -dt).  Only the
+    first num particles move."""
+    from helpers import interior_points, linear_velocity, taylor_P
+    from oracle.step import dye_advance
+    n, dx, dt = (12, 10, 8), 0.1, 0.05
+    L = np.array(n) * dx
+    c = np.array([0.8, -0.5, 0.3])
+    u = linear_velocity(np.zeros((3, 3)), c, *n, dx)
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, 1, size=(3, 50)) * L[:, None]
+    pos[:, 0] = [L[0] - 0.01, 0.02, 0.5 * L[2]]           # pushed through +x and -y walls
+    out = dye_advance(orc64, u, pos, 40, dx, dt)
+    want = np.clip(pos[:, :40] + c[:, None] * dt, 0, L[:, None])
+    np.testing.assert_allclose(out[:, :40], want, atol=1e-13)
+    assert out[0, 0] == L[0] and out[1, 0] == 0.0
+    np.testing.assert_array_equal(out[:, 40:], pos[:, 40:])
+    A = np.array([[0.2, -0.8, 0.3], [0.6, -0.1, -0.4], [-0.3, 0.5, -0.1]])
+    u = linear_velocity(A, np.zeros(3), *n, dx)
+    p = interior_points(*n, dx, 30, margin=3.5).T.copy()
+    out = dye_advance(orc64, u, p, 30, dx, dt)
+    np.testing.assert_allclose(out, taylor_P(A * dt) @ p, atol=1e-12)
+    assert np.max(np.abs(out - taylor_P(-A * dt) @ p)) > 1e-3

@@ -398,3 +398,28 @@ def test_delta_plate_voxelizer_equals_the_restatement(res):
     a, lo_a = voxelize_triangle(_plate(), dx)
     b, lo_b = V.voxelize_subdivide(_plate(), np.array([[0, 1, 2]]), dx)
     assert np.array_equal(a, b) and np.array_equal(lo_a, lo_b)
+
+
+@pytest.mark.parametrize("res", [32, 128])
+def test_plate_mask_is_the_restated_voxelizer_placed(res):
+    """The 256x128x128 solid workload's plate (span 0.3, half-angle 15 degrees):
+    its mask equals the restated trimesh voxelizer's matrix of the same
+    triangle, placed at the apex cell and extruded four cells in z; its y
+    extent is the span to within a cell and it is mirror symmetric in y."""
+    from oracle import voxel_ref as V
+    from sim.init_conditions import PLATE_APEX, plate_mask, plate_vertices
+    dx = 1.0 / res
+    shape = (2 * res, res, res)
+    m = plate_mask(shape, dx)
+    obj, _ = V.voxelize_subdivide(plate_vertices(), np.array([[0, 1, 2]]), dx)
+    ii, jj, kk = np.nonzero(m)
+    i0, j0, k0 = ii.min(), jj.min(), kk.min()
+    assert i0 == int(PLATE_APEX[0] // dx) and set(np.unique(kk)) == set(range(k0, k0 + 4))
+    assert k0 + 2 == round(PLATE_APEX[2] / dx)
+    for t in range(4):
+        sl = m[i0:i0 + obj.shape[0], j0:j0 + obj.shape[1], k0 + t]
+        assert np.array_equal(sl.astype(bool), obj[:, :, 0])
+    assert m.sum() == 4 * obj.sum()
+    assert abs((jj.max() - jj.min() + 1) * dx - 0.3) <= 1.5 * dx
+    jc = (jj.max() + jj.min()) / 2
+    assert abs(jc + 0.5 - 0.5 / dx) <= 0.5
