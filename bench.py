@@ -79,23 +79,43 @@ def read_pmc(d, counter):
     return out
 
 
+# kernels whose HBM traffic the counter runs record: short name -> algorithmic bytes per
+# launch at 256x128x128 (4,194,304 cells, 12,664,832 faces) and what they are
+TRAFFIC_KERNELS = {
+    "k_march_edge": (180.0 * 4194304, "cell: u 12 + old map 84 + new map 84 B"),
+    "k_map_u": (32.0 * 12664832, "face: u0 12 (gathered) + phi 12 + F 4 + out 4 B"),
+    "k_advect_sl": (8.0 * 12664832, "face: u in 4 + out 4 B"),
+    "k_advance_faces": (52.0 * 12664832, "face: phi, F read and written 48 + u 4 B"),
+    "k_map_smoke": (68.0 * 4194304, "cell: faces of phi 36 + smoke in 16 + out 16 B"),
+    "k_split_psi": (96.0 * 4194304 + 24.0 * 12664832, "cell: the garm table 96 B read once; face: phi, F out 24 B"),
+    "k_smooth8": (12.0 * 4194304, "cell: r 4 + z read 4 + z written 4 B (one half-sweep)"),
+}
+
+
 def summarize_pmc(fetch_dir, write_dir, probe_mib):
-    """HBM bytes per launch of the march kernel from FETCH_SIZE / WRITE_SIZE (KiB per
-    dispatch), each scaled by what the counter reads for the copy probe of known size."""
+    """HBM bytes per launch of the step's main kernels from FETCH_SIZE / WRITE_SIZE
+    (KiB per dispatch), each scaled by what the counter reads for the copy probe of
+    known size; the algorithmic bytes per launch beside them."""
     import numpy as np
     f, w = read_pmc(fetch_dir, "FETCH_SIZE"), read_pmc(write_dir, "WRITE_SIZE")
     known_kib = probe_mib * 1024.0
     # the probe is the last k_scale dispatch (the step's own k_scale launches come first)
     f_probe, w_probe = f["k_scale"][-1], w["k_scale"][-1]
     cf, cw = known_kib / f_probe, known_kib / w_probe
-    fm, wm = float(np.median(f["k_march_edge"])), float(np.median(w["k_march_edge"]))
-    rec = {"k_march_edge": {
-        "fetch_kib_per_launch": round(fm, 1), "write_kib_per_launch": round(wm, 1),
-        "copy_probe_mib": probe_mib, "copy_fetch_kib": round(f_probe, 1), "copy_write_kib": round(w_probe, 1),
-        "scale_fetch": round(cf, 4), "scale_write": round(cw, 4),
-        "hbm_bytes_per_launch": round((fm * cf + wm * cw) * 1024.0),
-        "launches": len(f["k_march_edge"]),
-    }}
+    rec = {}
+    for name, (alg, what) in TRAFFIC_KERNELS.items():
+        if name not in f or name not in w:
+            continue
+        fm, wm = float(np.median(f[name])), float(np.median(w[name]))
+        hbm = (fm * cf + wm * cw) * 1024.0
+        rec[name] = {
+            "fetch_kib_per_launch": round(fm, 1), "write_kib_per_launch": round(wm, 1),
+            "copy_probe_mib": probe_mib, "copy_fetch_kib": round(f_probe, 1), "copy_write_kib": round(w_probe, 1),
+            "scale_fetch": round(cf, 4), "scale_write": round(cw, 4),
+            "hbm_bytes_per_launch": round(hbm), "algorithmic_bytes_per_launch": round(alg),
+            "traffic_over_algorithmic": round(hbm / alg, 2), "algorithmic": what,
+            "launches": len(f[name]),
+        }
     return rec
 
 def parse():
@@ -158,8 +178,8 @@ def make_solver(sim, mode, warm_start=None):
     from sim.mgpcg_solid import MGPCG_Solid_3
     bt = np.full((3, 2), 2, np.int32)
     N = [sim.res_x, sim.res_y, sim.res_z]
-    kw = dict(base_level=3, mode="mixed" if mode == "learned" else mode, stop="relative", rel_tol=1e-6,
-              warm_start=warm_start)
+    base = {"learned": "mixed", "learned_dense": "dense"}.get(mode, mode)
+    kw = dict(base_level=3, mode=base, stop="relative", rel_tol=1e-6, warm_start=warm_start)
     if sim.solid:
         return MGPCG_Solid_3(bt, sim.boundary_mask, sim.boundary_vel, N, **kw)
     return MGPCG_3(bt, N, **kw)
@@ -177,9 +197,9 @@ def poisson_compare(sim, reps, torch):
     res = {}
     from sim.learned import load_default
     net = load_default()
-    modes = ("plain", "mixed", "dense") + (("learned",) if net is not None else ())
+    modes = ("plain", "mixed", "dense") + (("learned", "learned_dense") if net is not None else ())
     for mode in modes:
-        solver = make_solver(sim, mode, warm_start=net if mode == "learned" else None)
+        solver = make_solver(sim, mode, warm_start=net if mode.startswith("learned") else None)
         per = []
         for which, base in enumerate(rhs):
             times, iters, conv = [], [], True
@@ -199,8 +219,10 @@ def poisson_compare(sim, reps, torch):
             st = solver.stats
             per.append(dict(ms=round(statistics.median(times), 3), iters=int(statistics.median(iters)),
                             converged=conv, rel_res=float("%.3g" % math.sqrt(max(st["rTr"], 0.0) / max(st["b2"], 1e-300))),
-                            **({"x0_used": bool(solver.stats["x0_used"])} if mode == "learned"
-                                               else {})))
+                            **({"x0_used": bool(solver.stats["x0_used"]),
+                                "guess_rel_res": float("%.3g" % math.sqrt(solver.stats["r02"] /
+                                                                          max(solver.stats["b2"], 1e-300)))}
+                               if mode.startswith("learned") else {})))
         res[mode] = dict(per_solve=per, ms=round(sum(p["ms"] for p in per), 3),
                          iters=sum(p["iters"] for p in per))
         del solver
@@ -208,7 +230,9 @@ def poisson_compare(sim, reps, torch):
     res["speedup_dense_vs_plain"] = round(res["plain"]["ms"] / res["dense"]["ms"], 3)
     if "learned" in res:
         res["speedup_learned_vs_plain"] = round(res["plain"]["ms"] / res["learned"]["ms"], 3)
-        res["learned_note"] = "mixed solver started from the U-Net's p0 (inference included in ms)"
+        res["speedup_learned_dense_vs_plain"] = round(res["plain"]["ms"] / res["learned_dense"]["ms"], 3)
+        res["learned_note"] = ("learned: the mixed solver started from the U-Net's p0, learned_dense: the dense "
+                               "solver from it (inference included in ms)")
     return res
 
 
@@ -311,7 +335,8 @@ def main():
     a = parse()
     if a.summarize_pmc:
         rec = summarize_pmc(*a.summarize_pmc, probe_mib=a.copy_probe or 1024)
-        rec["k_march_edge"]["workload"] = march_workload(a.res, a.case)
+        for r in rec.values():
+            r["workload"] = march_workload(a.res, a.case)
         with open(TRAFFIC_FILE, "w") as fh:
             json.dump(rec, fh, indent=1)
         print(json.dumps(rec))
@@ -344,6 +369,7 @@ def main():
     march_kernel = {"edge": "k_march_edge", "ed8": "k_march_ed8", "ge": "k_march_ge",
                     "efm": "k_backtrack_faces"}[a.scheme]
     march_launch_ms = timing.durations(march_kernel)
+    other_launch_ms = {k: timing.durations(k) for k in ("k_map_u", "k_advect_sl", "k_advance_faces")}
     ms_step = total_ms / a.steps
     ph = {k: statistics.median(p.get(k, 0.0) for p in phases) for k in phases[0]}
     pstats = sim.poisson_stats[-2 * a.steps:]
@@ -399,6 +425,29 @@ def main():
                       counter_frac=round(march_tflops / FP32_PEAK_TFLOPS, 4),
                       algorithmic_bytes_per_launch=bytes_launch,
                       avg_launch_ms=round(march_ms, 4), launches=len(march_launch_ms))
+    # the step's other main kernels, each against its own bound (algorithmic work per unit
+    # from the reference formulation; face counts of the MAC grids)
+    nface = 3 * ncell + sim.res_y * sim.res_z + sim.res_x * sim.res_z + sim.res_x * sim.res_y
+    tr_all = json.load(open(TRAFFIC_FILE)) if os.path.exists(TRAFFIC_FILE) else {}
+    kernels = {}
+    for name, bound, per_unit, unit_note in (
+            ("k_map_u", "hbm", 32.0, "32 B per face (u0 gathered 12, phi 12, F 4, out 4)"),
+            ("k_advect_sl", "valu", 1980.0, "1,980 flop per face (RK4 + foot lookup, 5 B-spline MAC evaluations)"),
+            ("k_advance_faces", "valu", 3400.0, "3,400 flop per face (RK4 of the face point and its column)")):
+        ms = other_launch_ms.get(name) or []
+        if not ms:
+            continue
+        t = statistics.mean(ms)
+        work = per_unit * nface
+        if bound == "hbm":
+            ach, peak, unit = work / (t * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s"
+        else:
+            ach, peak, unit = work / (t * 1e-3) / 1e12, FP32_PEAK_TFLOPS, "TFLOP/s"
+        tr = tr_all.get(name, {})
+        kernels[name] = dict(bound=bound, achieved=round(ach, 2), peak=peak, unit=unit, frac=round(ach / peak, 4),
+                             avg_launch_ms=round(t, 4), launches=len(ms), per_face=unit_note,
+                             traffic=tr.get("hbm_bytes_per_launch") if tr.get("workload") == march_workload(
+                                 res, a.case, a.scheme) else None)
     # the projection's CG iteration (many kernels), HBM-bound: bytes per iteration / ms per iteration
     if pcmp is not None:
         it_ms = pcmp[a.solver]["ms"] / max(pcmp[a.solver]["iters"], 1)   # includes per-solve setup
@@ -455,6 +504,7 @@ def main():
         "poisson_same_rhs": pcmp,
         "roofline": roof_march,
         "roofline_cg_iteration": roof_mg,
+        "roofline_other_kernels": kernels,
         "io": io_rec,
         "cpu_baseline": cpu,
         "error_vs_oracle": oracle_err,

@@ -69,6 +69,7 @@ enum Slot {
   S_NCELL,
   S_ZTR0,
   S_NEU,         // all walls: recentre r (sim/mgpcg.py:211-217)
+  S_EPS,         // the guard added to the alpha and beta denominators (see set_scalars)
   S_COUNT
 };
 
@@ -227,17 +228,20 @@ __device__ void finalize(double* sc, int fin, double s0, double s1) {
       }
       break;
     case F_ZR: {
-      // sim/mgpcg.py:270-276
+      // sim/mgpcg.py:270-276 (S_EPS: the reference's 1e-12, or 0 with a zero guard)
       double nz = s0;
-      sc[S_BETA] = nz / (sc[S_ZTR] + 1e-12);
+      double den = sc[S_ZTR] + sc[S_EPS];
+      sc[S_BETA] = den != 0.0 ? nz / den : 0.0;
       sc[S_ZTR] = nz;
       break;
     }
-    case F_ALPHA:
+    case F_ALPHA: {
       // sim/mgpcg.py:236-239
       sc[S_PAP] = s0;
-      sc[S_ALPHA] = sc[S_ZTR] / (s0 + 1e-12);
+      double den = s0 + sc[S_EPS];
+      sc[S_ALPHA] = den != 0.0 ? sc[S_ZTR] / den : 0.0;
       break;
+    }
     case F_XR: {
       // sim/mgpcg.py:248-259: rTr and the stop test; s1 = sum of r for recenter
       sc[S_RTR] = s0;
@@ -2003,6 +2007,12 @@ int set_scalars(Solver& s, hipStream_t st, int max_iters, int stop_mode, double 
   v[S_SOLID] = s.solid ? 1.0 : 0.0;
   v[S_NCELL] = (double)s.N;
   v[S_NEU] = (s.bt == 63) ? 1.0 : 0.0;
+  // The reference adds an absolute 1e-12 to the denominators of alpha and beta
+  // (sim/mgpcg.py:236-239, 270-276); plain mode keeps it.  Once z.r falls to
+  // ~1e-12 (a small right-hand side, or a good warm start) that constant
+  // damps both coefficients and the residual stalls; the other modes divide
+  // by the dot products themselves (0 only when the direction is 0).
+  v[S_EPS] = (s.mode == 0) ? 1e-12 : 0.0;
   // pass through pinned host memory (stream-ordered copy, no memset)
   for (int i = 0; i < S_COUNT; ++i) s.sc_host[32 + i] = v[i];
   EDGE_CHECK_HIP(hipMemcpyAsync(s.sc, s.sc_host + 32, S_COUNT * sizeof(double), hipMemcpyHostToDevice, st));

@@ -620,6 +620,18 @@ int edge_unet_loss_head_bwd(const float* p, const float* target, const float* sc
   return EDGE_OK;
 }
 
+// The head's gradients for a given dL/dp (FP32 [N]): gh [N][C], dw [C] and
+// db [1] accumulated, as in edge_unet_loss_head_bwd without its loss.
+int edge_unet_head_bwd(const float* dp, const float* scale, const void* h, const float* w, int N, int C, float* gh,
+                       float* dw, float* db, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dp && scale && h && w && gh && dw && db && N > 0 && C > 0, "edge_unet_head_bwd: bad arguments");
+  hipLaunchKernelGGL(k_head_bwd, dim3((unsigned)nb(N)), dim3(256), 0, (hipStream_t)stream, (const bf16*)h, w, scale,
+                     dp, N, C, gh, dw, db);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
 // BF16 forward weights wf and (if wt) data-gradient weights of one layer.
 int edge_unet_pack_weights(const float* w, int Cout, int Cin, void* wf, void* wt, void* stream) {
   clear_error();

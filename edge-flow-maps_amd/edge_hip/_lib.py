@@ -83,6 +83,7 @@ SIGNATURES = {
     "edge_unet_input": "ppipppp",
     "edge_unet_head": "ppppiipp",
     "edge_unet_loss_head_bwd": "pppppiipppppp",
+    "edge_unet_head_bwd": "ppppiipppp",
     "edge_unet_pack_weights": "piippp",
     "edge_adam": "ppppiffffffp",
     "edge_zero_f32": "plp",

@@ -135,12 +135,16 @@ class Simulation:
     """The fields of run_edge.py:174-254 and one step of its loop."""
 
     def __init__(self, case="cfg1", solver_mode="plain", stop="relative", rel_tol=1e-6, res=None,
-                 reinit_every=None, check_every=None, num_samples=2000, seed=0, num_particles=None, scheme="edge"):
+                 reinit_every=None, check_every=None, num_samples=2000, seed=0, num_particles=None, scheme="edge",
+                 reuse_pressure=False):
         """scheme: "edge" (run_edge.py), "ed8" (run_ed8.py), "ge" (run_ge.py) or "efm"
         (run_efm.py): the backward map's march; ED8 and GE keep a full Hermite map and pull
         back through its face values; EFM keeps no cell map: it caches the velocity of every
         step since the last reinitialisation and re-marches the face map back through all
-        of them each step."""
+        of them each step.
+        reuse_pressure: start each projection from the same projection's pressure
+        of the previous step (the reference starts from zero, sim/mgpcg.py:67-75;
+        the answer meets the same stop rule either way)."""
         if scheme not in ("edge", "ed8", "ge", "efm"):
             raise ValueError(f"scheme must be edge, ed8, ge or efm, got {scheme!r}")
         self.scheme = scheme
@@ -177,6 +181,8 @@ class Simulation:
                                   rel_tol=rel_tol if stop == "relative" else 1e-12, **kw)
             self.poisson_max_iters = 400   # sim/mgpcg.py:436
         self.poisson_stats = []
+        self.reuse_pressure = bool(reuse_pressure)
+        self._p_prev = [None, None]
         V = ti.Vector.field
         F = ti.field
         cell, fx, fy, fz = (rx, ry, rz), (rx + 1, ry, rz), (rx, ry + 1, rz), (rx, ry, rz + 1)
@@ -312,13 +318,23 @@ class Simulation:
         return (self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm, self.dT_y_garm,
                 self.dT_z_garm)
 
-    def poisson(self, buffers=None):
+    def poisson(self, buffers=None, slot=0):
+        """The projection.  slot: which of the step's two projections (0 after
+        the semi-Lagrangian half step, 1 at the end of the step); with
+        `reuse_pressure` the solve starts from that projection's pressure of the
+        previous step (kept only when ||b - A x0|| <= ||b||, same stop rule)."""
         if self.rhs_snapshots is not None:
             self.rhs_snapshots.append([self.u_x.data.clone(), self.u_y.data.clone(), self.u_z.data.clone()])
+        x0 = self._p_prev[slot] if (self.reuse_pressure and self._p_prev[slot] is not None) else None
         if buffers is None:
-            self.solver.Poisson(self.u_x, self.u_y, self.u_z, max_iters=self.poisson_max_iters)
+            self.solver.Poisson(self.u_x, self.u_y, self.u_z, max_iters=self.poisson_max_iters, x0=x0)
         else:
-            self.solver.Poisson_with_buffer(self.u_x, self.u_y, self.u_z, *buffers, max_iters=self.poisson_max_iters)
+            self.solver.Poisson_with_buffer(self.u_x, self.u_y, self.u_z, *buffers, max_iters=self.poisson_max_iters,
+                                            x0=x0)
+        if self.reuse_pressure:
+            if self._p_prev[slot] is None:
+                self._p_prev[slot] = torch.empty_like(self.solver.p.data)
+            self._p_prev[slot].copy_(self.solver.p.data)
         self.poisson_stats.append(dict(self.solver.stats, time=self.solver.last_time))
 
     # -- one step (run_edge.py:588-711) --------------------------------------------
@@ -370,7 +386,7 @@ class Simulation:
         self.u_y.swap_storage(self.tmp_u_y)
         self.u_z.swap_storage(self.tmp_u_z)
         self._mark("poisson1")
-        self.poisson()
+        self.poisson(slot=0)
 
         self._mark("march")
         if self.scheme == "edge":
@@ -452,9 +468,9 @@ class Simulation:
             self.vis_x.fill(0.0)
             self.vis_y.fill(0.0)
             self.vis_z.fill(0.0)
-            self.poisson(buffers=(self.vis_x, self.vis_y, self.vis_z))
+            self.poisson(buffers=(self.vis_x, self.vis_y, self.vis_z), slot=1)
         else:
-            self.poisson()
+            self.poisson(slot=1)
         self._mark("end")
         return curr_dt
 

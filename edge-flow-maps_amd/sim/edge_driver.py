@@ -90,12 +90,13 @@ def map_u(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, T_x, T_y, T_z, psi_x, psi_y, psi_z
     nx, ny, nz = _cells_of_mac(u_x0)
     args = [as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1), as_ptr(T_x),
             as_ptr(T_y), as_ptr(T_z), as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z)]
-    if add_scale is None:
-        _lib.call("edge_map_u", *args, nx, ny, nz, float(dx), stream_handle())
-    else:
-        f_x, f_y, f_z, m, sc = add_scale
-        _lib.call("edge_map_u_add_scale", *args, as_ptr(f_x), as_ptr(f_y), as_ptr(f_z), float(m), float(sc), nx, ny,
-                  nz, float(dx), stream_handle())
+    with timing.span("k_map_u"):
+        if add_scale is None:
+            _lib.call("edge_map_u", *args, nx, ny, nz, float(dx), stream_handle())
+        else:
+            f_x, f_y, f_z, m, sc = add_scale
+            _lib.call("edge_map_u_add_scale", *args, as_ptr(f_x), as_ptr(f_y), as_ptr(f_z), float(m), float(sc), nx,
+                      ny, nz, float(dx), stream_handle())
     touch(u_x1, u_y1, u_z1)
 
 
@@ -152,8 +153,9 @@ def map_u_psi(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, dx, garm_map):
     step's two calls; the pull-back itself is map_u's kernel."""
     nx, ny, nz = _cells_of_mac(u_x0)
     phi, F = face_map_of(garm_map, dx)
-    _lib.call("edge_map_u", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
-              *[t.data_ptr() for t in F], *[t.data_ptr() for t in phi], nx, ny, nz, float(dx), stream_handle())
+    with timing.span("k_map_u"):
+        _lib.call("edge_map_u", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
+                  *[t.data_ptr() for t in F], *[t.data_ptr() for t in phi], nx, ny, nz, float(dx), stream_handle())
     touch(u_x1, u_y1, u_z1)
 
 

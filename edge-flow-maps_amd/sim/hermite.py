@@ -301,6 +301,7 @@ def advect_u_semi_lagrangian(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, dx, dt, X_x, X_
     (x_coords_func etc.); the kernel recomputes them from the indices."""
     nx, ny, nz = u_x0.shape[0] - 1, u_x0.shape[1], u_x0.shape[2]
     _check_mac(u_x0, u_y0, u_z0, nx, ny, nz)
-    _lib.call("edge_advect_u_sl", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
-              nx, ny, nz, float(dx), float(dt), stream_handle())
+    with timing.span("k_advect_sl"):
+        _lib.call("edge_advect_u_sl", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1),
+                  as_ptr(u_z1), nx, ny, nz, float(dx), float(dt), stream_handle())
     touch(u_x1, u_y1, u_z1)

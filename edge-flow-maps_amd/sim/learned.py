@@ -14,9 +14,10 @@ torch only holds the device memory.
 
 All 3x3x3 layers use ReLU.  Input channel 0 is b / rms(b), channel 1 the fluid
 mask, channels 2..7 zero (the first layer's input is padded to 8 channels).
-Training (`train`) uses online (b, p) pairs from the step loop itself: the
-mean-free squared error of p0 against the converged pressure, with Adam on
-FP32 master weights.
+Training (`train`) uses online (b, p) pairs from the step loop itself, with
+Adam on FP32 master weights; the loss is the energy norm of the guess's error
+(1/2 e'Ae, e = p0 - p, whose gradient is the guess's negative residual), or
+the mean-free squared error of p0 against the converged pressure.
 """
 import argparse
 import math
@@ -180,9 +181,14 @@ class UNet:
         return p
 
     # -- backward -----------------------------------------------------------------------
-    def backward(self, target):
-        """Gradients (into self.grads) of the mean-free squared error of the last
-        forward's p against target; returns the loss (host float)."""
+    def backward(self, target, loss="mse", apply_A=None):
+        """Gradients (into self.grads) of the last forward's p against the
+        converged pressure `target`; returns the loss (host float).
+        loss="mse": the mean-free squared error (in units of the input's RMS).
+        loss="energy": the error's energy norm 1/2 e'Ae / (N rms^2), e = p - target,
+        whose gradient A e / (N rms^2) is minus the guess's residual b - A p: what
+        the solver's stop rule and CG's error bound measure.  `apply_A(x, out)`
+        applies the projection's operator (the solver's edge_mg_apply_A)."""
         A, dims, N = self._fwd["A"], self._fwd["dims"], self._fwd["N"]
         s = stream_handle()
         f32, bf = torch.float32, torch.bfloat16
@@ -200,9 +206,27 @@ class UNet:
                             ("d0a", 0, 16), ("d0b", 0, 16)):
             gbuf(key, lev, C)
         dp = self._buf("g_p", (N[0],), f32)
-        _lib.call("edge_unet_loss_head_bwd", _ptr(self._acts["p"]), _ptr(target), _ptr(self._scale), _ptr(A["d0b"]),
-                  _ptr(self.view("head.w")), N[0], HEAD_C, _ptr(dp), _ptr(G["d0b"]),
-                  _ptr(self.view("head.w", True)), _ptr(self.view("head.b", True)), _ptr(self._work), s)
+        if loss == "mse":
+            _lib.call("edge_unet_loss_head_bwd", _ptr(self._acts["p"]), _ptr(target), _ptr(self._scale),
+                      _ptr(A["d0b"]), _ptr(self.view("head.w")), N[0], HEAD_C, _ptr(dp), _ptr(G["d0b"]),
+                      _ptr(self.view("head.w", True)), _ptr(self.view("head.b", True)), _ptr(self._work), s)
+            value = None
+        elif loss == "energy":
+            if apply_A is None:
+                raise ValueError("loss='energy' needs apply_A")
+            e = self._buf("g_e", (N[0],), f32)
+            Ae = self._buf("g_Ae", (N[0],), f32)
+            _lib.call("edge_axpy", _ptr(self._acts["p"]), _ptr(target), _ptr(e), -1.0, N[0], s)   # e = p - p*
+            apply_A(e, Ae)
+            sc = float(self._scale.item())
+            norm = 1.0 / (N[0] * sc * sc)
+            _lib.call("edge_scale", _ptr(Ae), norm, _ptr(dp), N[0], s)
+            _lib.call("edge_unet_head_bwd", _ptr(dp), _ptr(self._scale), _ptr(A["d0b"]), _ptr(self.view("head.w")),
+                      N[0], HEAD_C, _ptr(G["d0b"]), _ptr(self.view("head.w", True)),
+                      _ptr(self.view("head.b", True)), s)
+            value = 0.5 * norm * float(torch.dot(e, Ae).item())
+        else:
+            raise ValueError(f"loss must be 'mse' or 'energy', got {loss!r}")
 
         def layer_bwd(name, src0, src1, g_out, g0, g1):
             _, c0, c1, co, lev = next(L for L in LAYERS if L[0] == name)
@@ -227,7 +251,7 @@ class UNet:
         _lib.call("edge_unet_pool_bwd", _ptr(G["p0"]), _ptr(G["e0b"]), 16, *dims[0], s)
         layer_bwd("e0b", A["e0a"], None, G["e0b"], G["e0a"], None)
         layer_bwd("e0a", A["x"], None, G["e0a"], None, None)
-        return float(self._work[3].item())
+        return float(self._work[3].item()) if value is None else value
 
     def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         if self.m1 is None:
@@ -251,15 +275,20 @@ def load_default():
 # ---------------------------------------------------------------------------
 # Online training on (b, p) pairs from the step loop
 # ---------------------------------------------------------------------------
-def train(res=(128, 64, 64), pairs=400, steps_per_sim=16, lr=1e-3, seed=0, out=WEIGHTS, log=print, budget_s=None):
+def train(res=(128, 64, 64), pairs=400, steps_per_sim=16, lr=1e-3, seed=0, out=WEIGHTS, log=print, budget_s=None,
+          loss="energy", init=None):
     """Run the EDGE step loop on seeded ring flows at `res`; every projection's
     (b, converged p) is one training pair, used for one Adam step (mixed solver,
-    ||r|| <= 1e-6 ||b||).  A new flow (seed) every `steps_per_sim` steps."""
+    ||r|| <= 1e-6 ||b||).  A new flow (seed) every `steps_per_sim` steps.
+    loss: "energy" (the error's energy norm, see UNet.backward) or "mse";
+    init: weights to start from (a path), else a fresh network."""
     _here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     if _here not in sys.path:
         sys.path.insert(0, _here)
     from run_edge import Simulation
     net = UNet(seed=seed)
+    if init:
+        net.load(init)
     n = tuple(res)
     UNet.check_dims(n)
     b = torch.empty(n, dtype=torch.float32, device="cuda")
@@ -277,9 +306,10 @@ def train(res=(128, 64, 64), pairs=400, steps_per_sim=16, lr=1e-3, seed=0, out=W
             rv = _s.__class__.Poisson(_s, u_x, u_y, u_z, max_iters=max_iters, verbose=verbose, x0=x0)
             pairs_seen.append(True)
             net.forward(b)
-            loss = net.backward(_s.p.data)
+            val = net.backward(_s.p.data.reshape(-1), loss=loss,
+                               apply_A=lambda x, out: _s.get_Ap(x, out))
             net.adam_step(lr=lr * (0.5 ** (len(losses) // max(1, pairs // 4))))
-            losses.append(loss)
+            losses.append(val)
             return rv
 
         pairs_seen = []
@@ -304,9 +334,11 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--budget-s", type=float, default=None)
     ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--loss", default="energy", choices=["energy", "mse"])
+    ap.add_argument("--init", default=None, help="weights to start from")
     a = ap.parse_args(argv)
     train(tuple(a.res), a.pairs, a.steps_per_sim, a.lr, out=a.out, budget_s=a.budget_s,
-          log=lambda m: print(m, flush=True))
+          log=lambda m: print(m, flush=True), loss=a.loss, init=a.init)
 
 
 if __name__ == "__main__":

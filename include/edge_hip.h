@@ -268,6 +268,10 @@ int edge_unet_input(const float* b, const int* bm, int N, void* x, float* scale,
 /* Output head p = scale (h . w + bias). */
 int edge_unet_head(const void* h, const float* w, const float* bias, const float* scale, int N, int C, float* p,
                    void* stream);
+/* The head's gradients (gh, dw, db accumulated) for a given dL/dp [N]: used with
+ * the energy-norm loss, whose dL/dp is A (p - p*) from the solver's operator. */
+int edge_unet_head_bwd(const float* dp, const float* scale, const void* h, const float* w, int N, int C, float* gh,
+                       float* dw, float* db, void* stream);
 /* Mean-free squared error against a target pressure, its gradient and the head's gradients. */
 int edge_unet_loss_head_bwd(const float* p, const float* target, const float* scale, const void* h, const float* w,
                             int N, int C, float* dp, float* gh, float* dw, float* db, double* work, void* stream);

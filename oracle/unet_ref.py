@@ -88,11 +88,38 @@ def loss(p, target, scale):
     return torch.mean(q * q)
 
 
-def gradients(flat, b, target, mask=None):
-    """(loss, dloss/dflat) by autograd in flat's dtype (no BF16 rounding)."""
+def apply_A(e):
+    """The projection's operator with walls all round (sim/mgpcg.py:87-117):
+    (6 - #box faces) e - sum of the in-box neighbours, by tensor slicing."""
+    out = 6.0 * e
+    for a in range(3):
+        n = e.shape[a]
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[a], hi[a] = slice(0, n - 1), slice(1, n)
+        out = out.clone()
+        out[tuple(lo)] -= e[tuple(hi)]
+        out[tuple(hi)] -= e[tuple(lo)]
+        first = [slice(None)] * 3
+        last = [slice(None)] * 3
+        first[a], last[a] = slice(0, 1), slice(n - 1, n)
+        out[tuple(first)] -= e[tuple(first)]
+        out[tuple(last)] -= e[tuple(last)]
+    return out
+
+
+def energy_loss(p, target, scale):
+    """The error's energy norm 1/2 e'Ae / (N scale^2), e = p - target."""
+    e = p - target
+    return 0.5 * torch.sum(e * apply_A(e)) / (e.numel() * scale * scale)
+
+
+def gradients(flat, b, target, mask=None, kind="mse"):
+    """(loss, dloss/dflat) by autograd in flat's dtype (no BF16 rounding);
+    kind "mse" (loss) or "energy" (energy_loss)."""
     f = flat.clone().requires_grad_(True)
     p, scale = forward(f, b, mask)
-    L = loss(p, target, scale.detach())
+    L = (loss if kind == "mse" else energy_loss)(p, target, scale.detach())
     L.backward()
     return float(L), f.grad.detach()
 

@@ -154,3 +154,65 @@ def test_full_size_dye_particles_follow_the_ring(dev):
     print(f"dye centroid moved {d / sim.dx} cells in two steps")
     assert abs(d[0]) > 1e-3 * sim.dx
     assert abs(d[1]) < 0.05 * abs(d[0]) and abs(d[2]) < 0.05 * abs(d[0])
+
+
+def test_full_size_learned_warm_start_gain(dev):
+    """What the committed trained network (sim/unet_warmstart.npz) gains at
+    256x128x128: the two projections of an EDGE step of the four-ring flow,
+    solved by the mixed solver from zero and from the network's p0 (inference
+    included), on the same right-hand sides, to ||r|| <= 1e-6 ||b||.  The
+    iterations, milliseconds and the guess's relative residual are printed;
+    the test holds the invariants: both starts converge to the same velocity
+    (within the tolerance), and the guess is kept exactly when
+    ||b - A p0|| <= ||b||."""
+    from run_edge import Simulation
+    from sim.learned import load_default
+    from sim.mgpcg import MGPCG_3
+    net = load_default()
+    assert net is not None, "sim/unet_warmstart.npz is missing"
+    sim = Simulation("cfg2", solver_mode="mixed", res=RES, num_samples=400, check_every=0)
+    sim.initialize()
+    sim.rhs_snapshots = []
+    sim.substep()
+    rhs = sim.rhs_snapshots
+    bt = np.full((3, 2), 2)
+    solvers = {"zero": MGPCG_3(bt, list(RES), mode="mixed", stop="relative", rel_tol=1e-6),
+               "learned": MGPCG_3(bt, list(RES), mode="mixed", stop="relative", rel_tol=1e-6, warm_start=net),
+               "zero_dense": MGPCG_3(bt, list(RES), mode="dense", stop="relative", rel_tol=1e-6),
+               "learned_dense": MGPCG_3(bt, list(RES), mode="dense", stop="relative", rel_tol=1e-6, warm_start=net)}
+    out = {}
+    for which, base in enumerate(rhs):
+        res = {}
+        for name, s in solvers.items():
+            times = []
+            for rep in range(3):
+                for f, b in zip((sim.tmp_u_x, sim.tmp_u_y, sim.tmp_u_z), base):
+                    f.data.copy_(b)
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                s.Poisson(sim.tmp_u_x, sim.tmp_u_y, sim.tmp_u_z)
+                e1.record()
+                torch.cuda.synchronize()
+                times.append(e0.elapsed_time(e1))
+            st = s.stats
+            assert st["converged"] == 1.0 and st["rTr"] <= 1e-12 * st["b2"] * 1.0001
+            res[name] = dict(ms=min(times[1:]), iters=st["iters"], u=[f.to_numpy() for f in
+                                                                     (sim.tmp_u_x, sim.tmp_u_y, sim.tmp_u_z)])
+            if name.startswith("learned"):
+                r0 = (st["r02"] / st["b2"]) ** 0.5
+                assert st["x0_used"] == (1.0 if st["r02"] <= st["b2"] else 0.0)
+                res[name]["r0"] = r0
+        for a in range(3):
+            assert scale_err(res["learned"]["u"][a], res["zero"]["u"][a]) < 1e-4
+            assert scale_err(res["learned_dense"]["u"][a], res["zero"]["u"][a]) < 1e-4
+        out[which] = res
+        for sfx, label in (("", "mixed"), ("_dense", "dense")):
+            z, l = res["zero" + sfx], res["learned" + sfx]
+            print(f"projection {which} ({label}): zero start {z['iters']} iterations {z['ms']:.3f} ms; "
+                  f"learned start {l['iters']} iterations {l['ms']:.3f} ms (||b - A p0|| / ||b|| = {l['r0']:.3g})")
+    for sfx, label in (("", "mixed"), ("_dense", "dense")):
+        gain_ms = sum(out[w]["zero" + sfx]["ms"] - out[w]["learned" + sfx]["ms"] for w in out)
+        gain_it = sum(out[w]["zero" + sfx]["iters"] - out[w]["learned" + sfx]["iters"] for w in out)
+        print(f"learned warm start gain over both projections ({label}): {gain_it} iterations, {gain_ms:.3f} ms "
+              f"(negative: a loss)")

@@ -148,6 +148,53 @@ def test_network_forward_and_backward_match_oracle(dev):
     assert float(g[off:off + cnt].reshape(16, 27, 8)[:, :, 2:].abs().max()) == 0.0
 
 
+def test_energy_loss_gradient_matches_oracle(dev):
+    """The energy-norm loss (1/2 e'Ae / (N rms^2), e = p - p*; its dL/dp is A e
+    through the solver's operator) and the network's gradient under it, against
+    FP32 autograd of the unrounded CPU network with the restated operator."""
+    from sim.learned import UNet
+    from sim.mgpcg import MGPCG_3
+    from oracle import unet_ref as U
+    n = (16, 16, 64)
+    net = UNet(seed=6)
+    rng = np.random.default_rng(17)
+    u = smooth_velocity(*n, 1.0 / 16, seed=8)
+    b = (u[0][1:] - u[0][:-1]) + (u[1][:, 1:] - u[1][:, :-1]) + (u[2][:, :, 1:] - u[2][:, :, :-1])
+    b = -(b - b.mean())
+    tb = torch.tensor(b, dtype=torch.float32)
+    p0 = net.forward(tb.cuda()).clone()
+    flat = net.params.detach().cpu().to(torch.float32)
+    with torch.no_grad():
+        _, scale = U.forward(flat, tb, bf16=True)
+    # a smooth target, as a converged pressure is (a rough one makes dL/dp = A e
+    # white noise, whose products with the activations cancel over the voxels
+    # far below the BF16 rounding of the backward chain)
+    idx = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+    smooth = sum(rng.normal() * np.cos(np.pi * k * (idx[d] + 0.5) / n[d]) for d in range(3) for k in (1, 2))
+    target = torch.tensor(smooth, dtype=torch.float32) * float(scale)
+    solver = MGPCG_3(np.full((3, 2), 2), list(n), levels=2)
+    loss = net.backward(target.cuda().reshape(-1), loss="energy", apply_A=lambda x, out: solver.get_Ap(x, out))
+    # dL/dp itself: A (p0 - target) / (N rms^2) with the restated operator, FP64
+    e = p0.cpu().double().reshape(n) - target.double()
+    want = U.apply_A(e) / (e.numel() * float(net._scale.item()) ** 2)
+    got = net._acts["g_p"].cpu().double().reshape(n)
+    assert float((got - want).abs().max() / want.abs().max()) < 1e-5
+    Lref, gref = U.gradients(flat.double(), tb.double(), target.double(), kind="energy")
+    print(f"energy loss GPU {loss:.5g}, oracle {Lref:.5g}")
+    assert loss == pytest.approx(Lref, rel=5e-2)
+    g = net.grads.cpu().double()
+    errs = {}
+    for key, (off, cnt) in net.offsets.items():
+        if key.startswith("e0a.w") or key == "head.b":
+            continue   # padding channels; the constant offset has zero gradient (A 1 = 0)
+        errs[key] = rel_l2(g[off:off + cnt], gref[off:off + cnt])
+    print("energy-loss gradient, rel. L2 vs FP32 autograd:", {k: round(v, 4) for k, v in errs.items()})
+    for key, err in errs.items():
+        assert err < 1e-1, key
+    off, cnt = net.offsets["head.b"]
+    assert abs(float(g[off])) < 1e-5 * float(g.abs().max())
+
+
 def test_training_reduces_the_loss(dev):
     from sim.learned import UNet
     n = (16, 16, 64)

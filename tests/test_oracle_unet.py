@@ -2,6 +2,7 @@
 brute-force convolution from its definition, and the gradient against central
 finite differences along random directions (FP64).  CPU only."""
 import numpy as np
+import pytest
 import torch
 
 from oracle import unet_ref as U
@@ -68,3 +69,29 @@ def test_zero_network_gives_the_head_bias():
     p, scale = U.forward(flat, b)
     np.testing.assert_allclose(p.numpy(), 0.5 * float(scale))
     np.testing.assert_allclose(float(scale), float(torch.sqrt(torch.mean(b ** 2))))
+
+
+def test_energy_loss_operator_and_eigenvector():
+    """The energy loss's operator is the projection's 7-point operator with
+    walls all round: it equals the oracle's restated operator on a random
+    field, and on a discrete cosine mode cos(pi k (i + 1/2) / n) per axis, an
+    eigenvector with eigenvalue sum(2 - 2 cos(pi k_d / n_d)), the loss is half
+    that eigenvalue times |v|^2 / N.  A missing wall term or a dropped 1/2
+    fails."""
+    import torch
+    from oracle import np_ref
+    from oracle import unet_ref as U
+    n = (6, 5, 7)
+    rng = np.random.default_rng(3)
+    e = rng.normal(size=n)
+    ns, dg = np_ref._nsum_diag(e)
+    np.testing.assert_allclose(U.apply_A(torch.tensor(e)).numpy(), dg * e - ns, atol=1e-12)
+    k = (2, 1, 3)
+    idx = np.meshgrid(*[np.arange(m) for m in n], indexing="ij")
+    v = np.ones(n)
+    lam = 0.0
+    for d in range(3):
+        v = v * np.cos(np.pi * k[d] * (idx[d] + 0.5) / n[d])
+        lam += 2 - 2 * np.cos(np.pi * k[d] / n[d])
+    got = float(U.energy_loss(torch.tensor(v), torch.zeros(n, dtype=torch.float64), torch.tensor(1.0)))
+    assert got == pytest.approx(0.5 * lam * (v ** 2).sum() / v.size, rel=1e-12)
