@@ -7,11 +7,15 @@ is one iteration of the reference's loop (run_edge.py:588-711): semi-Lagrangian
 half step + projection, EDGE march, face march, BFECC pull-back, projection.
 
 Prints ONE JSON line: ms/step with the chosen projection solver (value; by
-default "dense": FP32 multigrid with an FP64 dense coarsest solve), the plain
-FP32 reference solver beside it, Poisson ms and iterations of plain, mixed
-(BF16 multigrid) and dense on the same right-hand sides, per-phase times, the roofline of the dominant kernel (HIP events on the
-launch stream) and the CPU baseline (the oracle restatement running one step
-of the same workload on this host's cores).
+default "mixed": BF16 multigrid levels, FP64 dots and dense FP64 coarsest
+solve, each projection started from the same projection's pressure of the
+previous step), the plain FP32 reference solver from zero beside it, Poisson
+ms and iterations of plain, mixed, dense and the learned warm start on the
+same right-hand sides (from zero), per-phase times, the roofline of the
+dominant kernel and of the other main kernels (HIP events on the launch
+stream), the step's error against the oracle and the CPU baseline (the
+oracle restatement running one step of the same workload on this host's
+cores).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--res NX NY NZ]
 """
@@ -128,8 +132,11 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--poisson-reps", type=int, default=3, help="0 skips the same-RHS solver comparison")
     ap.add_argument("--no-io", action="store_true", help="skip timing one frame output and one checkpoint")
-    ap.add_argument("--solver", default="dense", choices=["plain", "mixed", "dense"],
+    ap.add_argument("--solver", default="mixed", choices=["plain", "mixed", "dense"],
                     help="projection solver of the timed steps")
+    ap.add_argument("--zero-start", action="store_true",
+                    help="start every projection from zero (the reference) instead of the same projection's "
+                         "pressure of the previous step")
     ap.add_argument("--case", default="cfg2", choices=["cfg2", "cfg3", "cfg3plate", "cfg5", "cfg5laden"],
                     help="cfg2: the headline EDGE step; cfg3: the reference's delta-wing plate with the solid "
                          "projection; cfg3plate: the four rings plus a plate of span 0.3 in an inflow of speed 1 at "
@@ -351,6 +358,7 @@ def main():
     big = a.case in ("cfg5", "cfg5laden")
     res = tuple(a.res) if (not big or a.res != [256, 128, 128]) else (512, 256, 256)
     sim = Simulation(a.case, solver_mode=a.solver, stop="relative", rel_tol=1e-6, res=res, check_every=0,
+                     reuse_pressure=not a.zero_start,
                      num_particles=a.particles, scheme=a.scheme)
     sim.initialize()
     freeze_setup_objects()   # as Simulation.run does before its loop
@@ -377,14 +385,16 @@ def main():
 
     plain = None
     if not a.no_plain:
-        step_solver = sim.solver
-        sim.solver = make_solver(sim, "plain")
+        # the reference's solver from the reference's zero start
+        step_solver, reuse = sim.solver, sim.reuse_pressure
+        sim.solver, sim.reuse_pressure = make_solver(sim, "plain"), False
         run_steps(sim, 1, torch)  # warm the plain solver
         pw, pph = run_steps(sim, a.steps, torch)
         plain = dict(ms_per_step=round(statistics.mean(pw), 3), step_ms=[round(w, 3) for w in pw],
                      phases_ms={k: round(statistics.median(p.get(k, 0.0) for p in pph), 3) for k in pph[0]},
-                     poisson_iters=[s["iters"] for s in sim.poisson_stats[-2 * a.steps:]])
-        sim.solver = step_solver
+                     poisson_iters=[s["iters"] for s in sim.poisson_stats[-2 * a.steps:]],
+                     projection_start="zero")
+        sim.solver, sim.reuse_pressure = step_solver, reuse
 
     pcmp = poisson_compare(sim, a.poisson_reps, torch) if a.poisson_reps > 0 else None
 
@@ -495,6 +505,7 @@ def main():
         "config": {"workload": wl, "cells": ncell, "reinit_every": sim.reinit_every,
                    "poisson_stop": "||r|| <= 1e-6 ||b||",
                    "solver": SOLVER_NOTE[a.solver],
+                   "projection_start": "zero" if a.zero_start else "the same projection's pressure of the previous step",
                    **({"particles": sim.laden_particle_num} if sim.lp is not None else {}),
                    **({"dye_particles": sim.dye_num} if sim.dye is not None else {})},
         "phases_ms": {k: round(v, 3) for k, v in ph.items()},

@@ -368,7 +368,11 @@ __global__ void __launch_bounds__(BS) k_apply_bc(Grid g, int bt, Faces3 U, const
   if (changed) U.f[a][l] = v;
 }
 
-__device__ __forceinline__ float divergence_at(int nx, int ny, int nz, const Faces3& U, int i, int j, int k) {
+// walls (bit 2a: low side of axis a, 2a+1: high side): when non-zero, a face on
+// such a wall reads as 0, the value apply_bc would have written there first
+// (the no-solid projection skips that pass: see edge_poisson).
+__device__ __forceinline__ float divergence_at(int nx, int ny, int nz, const Faces3& U, int i, int j, int k,
+                                               int walls = 0) {
   // sim/mgpcg.py:394-404 (the sample() clamps never bind for i+1 <= nx)
   int64_t qx = ((int64_t)i * ny + j) * nz + k;
   int64_t qy = ((int64_t)i * (ny + 1) + j) * nz + k;
@@ -376,6 +380,14 @@ __device__ __forceinline__ float divergence_at(int nx, int ny, int nz, const Fac
   float vl = U.f[0][qx], vr = U.f[0][qx + (int64_t)ny * nz];
   float vb = U.f[1][qy], vt = U.f[1][qy + nz];
   float va = U.f[2][qz], vc = U.f[2][qz + 1];
+  if (walls) {
+    if (i == 0 && (walls & 1)) vl = 0.f;
+    if (i == nx - 1 && (walls & 2)) vr = 0.f;
+    if (j == 0 && (walls & 4)) vb = 0.f;
+    if (j == ny - 1 && (walls & 8)) vt = 0.f;
+    if (k == 0 && (walls & 16)) va = 0.f;
+    if (k == nz - 1 && (walls & 32)) vc = 0.f;
+  }
   return vr - vl + vt - vb + vc - va;
 }
 
@@ -399,7 +411,7 @@ __global__ void __launch_bounds__(BS) k_div_sum(Grid g, Faces3 U, const int8_t* 
 // When b is given instead (solve API), r = b.
 __global__ void __launch_bounds__(BS) k_init(Grid g, Faces3 U, const float* __restrict__ b,
                                             const int8_t* __restrict__ bm0, float* __restrict__ r, float* __restrict__ x,
-                                            float* __restrict__ p, const float* __restrict__ x0, Red R) {
+                                            float* __restrict__ p, const float* __restrict__ x0, Red R, int walls) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int64_t N = (int64_t)nx * ny * nz;
   const double* sc = R.sc;
@@ -414,7 +426,7 @@ __global__ void __launch_bounds__(BS) k_init(Grid g, Faces3 U, const float* __re
     } else {
       int i, j, k;
       cell_ijk(g, q, i, j, k);
-      float d = divergence_at(nx, ny, nz, U, i, j, k);
+      float d = divergence_at(nx, ny, nz, U, i, j, k, walls);
       if (bm0) d -= corr;
       v = d * -1.0f;
     }
@@ -470,7 +482,11 @@ __global__ void __launch_bounds__(BS) k_apply_x0(int64_t N, float* __restrict__ 
 
 // subtract_grad_p (sim/mgpcg.py:406-432) with p = x, + apply_bc, and the
 // optional force buffer -(grad p) (sim/mgpcg.py:447-479).
-__global__ void __launch_bounds__(BS) k_subtract_grad(Grid g, Faces3 U, Faces3 F, const float* __restrict__ P) {
+// walls: non-zero in the no-solid projection, whose closing apply_bc (box
+// walls zeroed) is folded in here: a face on a wall gets 0, as apply_bc after
+// the subtraction would leave it; the force buffer is -grad p everywhere.
+__global__ void __launch_bounds__(BS) k_subtract_grad(Grid g, Faces3 U, Faces3 F, const float* __restrict__ P,
+                                                     int walls) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int a, i, j, k;
   int64_t l, nfa;
@@ -488,7 +504,11 @@ __global__ void __launch_bounds__(BS) k_subtract_grad(Grid g, Faces3 U, Faces3 F
   float pl = pat(i - (a == 0), j - (a == 1), k - (a == 2));
   if (I[a] - 1 < 0) pl = 0.f;
   if (I[a] >= n[a]) pr = 0.f;
-  U.f[a][l] -= (pr - pl);
+  const bool wall = walls && ((I[a] == 0 && ((walls >> (2 * a)) & 1)) || (I[a] == n[a] && ((walls >> (2 * a + 1)) & 1)));
+  if (wall)
+    U.f[a][l] = 0.f;
+  else
+    U.f[a][l] -= (pr - pl);
   if (F.f[a]) F.f[a][l] = -(pr - pl);
 }
 
@@ -1140,7 +1160,10 @@ __global__ void __launch_bounds__(BOT_THREADS) k_bottom_rbgs(Lv L, int bt, const
 // On entry level lt's r and z (first red half-sweep done) are in global
 // memory; on exit level lt's z is.
 constexpr int TAIL_THREADS = 1024;
-constexpr int TAIL_MAX_CELLS = 1024;   // larger levels are faster as row-chunk launches
+#ifndef EDGE_TAIL_MAX_CELLS
+#define EDGE_TAIL_MAX_CELLS 1024
+#endif
+constexpr int TAIL_MAX_CELLS = EDGE_TAIL_MAX_CELLS;   // larger levels are faster as row-chunk launches
 struct Tail {
   int lt, L;
   Lv lv[MAXL];
@@ -2398,14 +2421,20 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   if (rc) return rc;
   rc = prepare_masks(s, st);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), s.bt, U,
-                     s.solid ? s.user_bm : nullptr, s.solid ? s.user_bvel : nullptr);
+  // apply_bc (sim/mgpcg.py:373-392) first.  Without solids it only zeroes the
+  // box's wall faces, and everything after reads them through the divergence
+  // (k_init, walls as 0) or overwrites them (k_subtract_grad writes 0 there):
+  // the pass is skipped, with the same values.
+  const int walls = s.solid ? 0 : s.bt;
+  if (s.solid)
+    hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), s.bt,
+                       U, s.user_bm, s.user_bvel);
   if (s.solid) {
     hipLaunchKernelGGL(k_div_sum, dim3(red_blocks(s.N)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), U, (const int8_t*)s.bm[0], red(s));
     finish(s, st, red_blocks(s.N), F_DIVSUM, 0);
   }
   hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), U, nullptr,
-                     s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s));
+                     s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s), walls);
   finish(s, st, red_blocks(s.N), F_MEAN, 0);
   EDGE_CHECK_LAUNCH();
   rc = start_solve(s, st, x0);
@@ -2414,9 +2443,10 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   if (cg != EDGE_OK && cg != EDGE_ENOCONV) return cg;
   // get_result + subtract_grad_p + apply_bc
   hipLaunchKernelGGL(k_subtract_grad, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), U,
-                     Faces3{{fx, fy, fz}}, (const float*)s.x);
-  hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), s.bt, U,
-                     s.solid ? s.user_bm : nullptr, s.solid ? s.user_bvel : nullptr);
+                     Faces3{{fx, fy, fz}}, (const float*)s.x, walls);
+  if (s.solid)
+    hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), s.bt,
+                       U, s.user_bm, s.user_bvel);
   EDGE_CHECK_LAUNCH();
   if (p_out) EDGE_CHECK_HIP(hipMemcpyAsync(p_out, s.x, s.N * 4, hipMemcpyDeviceToDevice, st));
   if (cg == EDGE_ENOCONV) set_error("edge_poisson: no convergence within %d iterations", max_iters);
@@ -2445,7 +2475,7 @@ int edge_mg_rhs(void* handle, float* ux, float* uy, float* uz, float* b, void* s
     finish(s, st, red_blocks(s.N), F_DIVSUM, 0);
   }
   hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, g, U, nullptr,
-                     s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, nullptr, red(s));
+                     s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, nullptr, red(s), 0);
   finish(s, st, red_blocks(s.N), F_MEAN, 0);
   EDGE_CHECK_LAUNCH();
   rc = start_solve(s, st, nullptr);
@@ -2467,7 +2497,7 @@ int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int sto
   rc = prepare_masks(s, st);
   if (rc) return rc;
   hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), Faces3{}, b,
-                     s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s));
+                     s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s), 0);
   finish(s, st, red_blocks(s.N), F_MEAN, 0);
   EDGE_CHECK_LAUNCH();
   rc = start_solve(s, st, x0);
