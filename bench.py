@@ -45,8 +45,8 @@ SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 16550.0, "ge": 4430.
 FACE_MARCH_FLOP_PER_FACE = 2860.0
 # the reference formulation's count (64 weight products per Hermite evaluation, 27 per
 # B-spline evaluation), the algorithmic work the roofline record is priced with
-SCHEME_FLOP_PER_CELL_ALG = {"edge": 33600.0, "ed8": 21400.0, "ge": 6800.0}
-SCHEME_BYTES_PER_CELL = {"edge": 180.0, "ed8": 204.0, "ge": 324.0}
+SCHEME_FLOP_PER_CELL_ALG = {"edge": 33600.0, "ed8": 21400.0, "ed4": 14500.0, "ge": 6800.0}
+SCHEME_BYTES_PER_CELL = {"edge": 180.0, "ed8": 204.0, "ed4": 108.0, "ge": 324.0}
 MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0, "dense": 145.0}  # one CG iteration
 MARCH_BYTES_PER_CELL = 180.0       # u 12 + old map 84 + new map 84 (compulsory FP32)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
@@ -143,8 +143,9 @@ def parse():
                          "20 degrees (solid projection); cfg5: 512x256x256 with 16M passive dye particles; "
                          "cfg5laden: 512x256x256 with 16M laden particles")
     ap.add_argument("--particles", type=int, default=None, help="particle count for --case cfg5 / cfg5laden")
-    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge", "efm"],
-                    help="backward-map march: EDGE (run_edge.py), ED8 (run_ed8.py) or GE (run_ge.py)")
+    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ed4", "ge", "efm"],
+                    help="backward-map march: EDGE (run_edge.py), ED8 (run_ed8.py), ED4 (run_ed4.py), GE "
+                         "(run_ge.py) or EFM (run_efm.py)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, one plain copy kernel (k_scale, alpha 1) over MIB MiB: the known-size "
                          "launch that scales rocprofv3's FETCH_SIZE / WRITE_SIZE")
@@ -374,7 +375,7 @@ def main():
     torch.cuda.synchronize()
     total_ms = (time.perf_counter() - t0) * 1e3
     timing.enable(False)
-    march_kernel = {"edge": "k_march_edge", "ed8": "k_march_ed8", "ge": "k_march_ge",
+    march_kernel = {"edge": "k_march_edge", "ed8": "k_march_ed8", "ed4": "k_march_ed4", "ge": "k_march_ge",
                     "efm": "k_backtrack_faces"}[a.scheme]
     march_launch_ms = timing.durations(march_kernel)
     other_launch_ms = {k: timing.durations(k) for k in ("k_map_u", "k_advect_sl", "k_advance_faces")}
@@ -412,7 +413,7 @@ def main():
         # compulsory: the face map written once (24 B per face) and one velocity read per march
         bytes_launch = (24.0 + 4.0 * marches) * nface
     else:
-        flop_launch = SCHEME_FLOP_PER_CELL[a.scheme] * ncell
+        flop_launch = SCHEME_FLOP_PER_CELL.get(a.scheme, 0.0) * ncell   # 0: not counted for this kernel
         bytes_launch = SCHEME_BYTES_PER_CELL[a.scheme] * ncell
     march_tflops = flop_launch / (march_ms * 1e-3) / 1e12
     # the same launches priced with the reference formulation's count (64 weight products per
@@ -431,8 +432,8 @@ def main():
                       flop_source="algorithmic: the reference formulation's flops per cell",
                       flop_per_cell=SCHEME_FLOP_PER_CELL_ALG.get(a.scheme), flop_per_launch=flop_launch_alg,
                       counter_flop_per_cell=SCHEME_FLOP_PER_CELL.get(a.scheme),
-                      counter_achieved=round(march_tflops, 2),
-                      counter_frac=round(march_tflops / FP32_PEAK_TFLOPS, 4),
+                      counter_achieved=round(march_tflops, 2) if flop_launch else None,
+                      counter_frac=round(march_tflops / FP32_PEAK_TFLOPS, 4) if flop_launch else None,
                       algorithmic_bytes_per_launch=bytes_launch,
                       avg_launch_ms=round(march_ms, 4), launches=len(march_launch_ms))
     # the step's other main kernels, each against its own bound (algorithmic work per unit

@@ -40,8 +40,9 @@ from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smok
                              map_smoke, map_smoke_psi, map_u, map_u_psi, march_phi_grid, reset_to_identity)
 from sim import lp as lp_kernels  # noqa: E402
 from sim.edge_driver import backtrack_faces, clamp_smoke, split_psi_hermite  # noqa: E402
-from sim.hermite import (advect_ps_garm, advect_ps_hermite_ed8, advect_psi_garm_ed4_fusion_unroll,  # noqa: E402
-                         advect_u_semi_lagrangian, reset_to_identity_2, reset_to_identity_3)
+from sim.hermite import (advect_ps_garm, advect_ps_hermite_ed4_fd_unroll, advect_ps_hermite_ed8,  # noqa: E402
+                         advect_psi_garm_ed4_fusion_unroll, advect_u_semi_lagrangian, reset_to_identity_1,
+                         reset_to_identity_2, reset_to_identity_3)
 from sim.init_conditions import (add_inflow, init_delta, init_four_vorts, init_rings,  # noqa: E402
                                  init_vorts_leapfrog, seeded_rings, set_boundary_mask_delta,
                                  set_boundary_mask_plate)
@@ -137,16 +138,17 @@ class Simulation:
     def __init__(self, case="cfg1", solver_mode="plain", stop="relative", rel_tol=1e-6, res=None,
                  reinit_every=None, check_every=None, num_samples=2000, seed=0, num_particles=None, scheme="edge",
                  reuse_pressure=False):
-        """scheme: "edge" (run_edge.py), "ed8" (run_ed8.py), "ge" (run_ge.py) or "efm"
-        (run_efm.py): the backward map's march; ED8 and GE keep a full Hermite map and pull
-        back through its face values; EFM keeps no cell map: it caches the velocity of every
+        """scheme: "edge" (run_edge.py), "ed8" (run_ed8.py), "ed4" (run_ed4.py), "ge"
+        (run_ge.py) or "efm" (run_efm.py): the backward map's march; ED8 and GE keep a full
+        Hermite map, ED4 an FD-unrolled one (psi and T only), and they pull back through
+        the map's face values; EFM keeps no cell map: it caches the velocity of every
         step since the last reinitialisation and re-marches the face map back through all
         of them each step.
         reuse_pressure: start each projection from the same projection's pressure
         of the previous step (the reference starts from zero, sim/mgpcg.py:67-75;
         the answer meets the same stop rule either way)."""
-        if scheme not in ("edge", "ed8", "ge", "efm"):
-            raise ValueError(f"scheme must be edge, ed8, ge or efm, got {scheme!r}")
+        if scheme not in ("edge", "ed8", "ed4", "ge", "efm"):
+            raise ValueError(f"scheme must be edge, ed8, ed4, ge or efm, got {scheme!r}")
         self.scheme = scheme
         cfg = dict(CASES[case])
         if res is not None:
@@ -367,6 +369,8 @@ class Simulation:
             if self.scheme == "edge":
                 reset_to_identity_2(self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm,
                                     self.dT_y_garm, self.dT_z_garm, self.X)
+            elif self.scheme == "ed4":
+                reset_to_identity_1(self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.X)
             elif self.scheme != "efm":
                 reset_to_identity_3(self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm,
                                     self.dT_y_garm, self.dT_z_garm, self.d2T_x, self.d2T_y, self.d2T_z, self.X)
@@ -427,7 +431,7 @@ class Simulation:
                 else:
                     map_smoke_psi(s0, s1, dx, gm)
         else:
-            # run_ed8.py:409-433: through the face values of the full map
+            # run_ed8.py:409-433 (run_ed4.py:447-467): through the map's face values
             def pull_u(u0, u1):
                 map_u(*u0, *u1, self.T_x, self.T_y, self.T_z, self.psi_x, self.psi_y, self.psi_z, dx)
 
@@ -497,8 +501,16 @@ class Simulation:
 
     def march_full_map(self, curr_dt):
         """run_ed8.py:226-230 / run_ge.py:220-231: the ED8 or GE march of the full
-        Hermite map, then psi and T at the faces."""
+        Hermite map, then psi and T at the faces; run_ed4.py:167-169: the ED4 march
+        of the FD-unrolled map, whose face values run_ed4.py's map_u_psi and
+        map_smoke_psi evaluate (here once per step, shared by all pull-backs)."""
         dx = self.dx
+        if self.scheme == "ed4":
+            fd = (self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm)
+            advect_ps_hermite_ed4_fd_unroll(None, None, None, None, *fd, self.u_x, self.u_y, self.u_z, self.X, dx,
+                                            curr_dt, self.ed_eps)
+            split_psi_hermite(self.psi_x, self.psi_y, self.psi_z, self.T_x, self.T_y, self.T_z, fd, dx)
+            return
         full = (self.psi_garm, self.T_x_garm, self.T_y_garm, self.T_z_garm, self.dT_x_garm, self.dT_y_garm,
                 self.dT_z_garm, self.d2T_x, self.d2T_y, self.d2T_z)
         if self.scheme == "ed8":
@@ -578,7 +590,9 @@ class Simulation:
         the fields at the last reinitialisation and the particles' state."""
         names = ["init_u_x", "init_u_y", "init_u_z", "init_smoke", "phi_x", "phi_y", "phi_z", "F_x", "F_y", "F_z"]
         if self.scheme != "efm":
-            names += ["psi_garm", "T_x_garm", "T_y_garm", "T_z_garm", "dT_x_garm", "dT_y_garm", "dT_z_garm"]
+            names += ["psi_garm", "T_x_garm", "T_y_garm", "T_z_garm"]
+        if self.scheme not in ("efm", "ed4"):
+            names += ["dT_x_garm", "dT_y_garm", "dT_z_garm"]
         if self.scheme in ("ed8", "ge"):
             names += ["d2T_x", "d2T_y", "d2T_z"]
         if self.scheme == "efm":
@@ -695,7 +709,7 @@ def main(argv=None):
     ap.add_argument("--solver", default="plain", choices=["plain", "mixed", "dense"])
     ap.add_argument("--res", type=int, nargs=3, default=None)
     ap.add_argument("--particles", type=int, default=None, help="particle count of a particle case")
-    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ge", "efm"])
+    ap.add_argument("--scheme", default="edge", choices=["edge", "ed8", "ed4", "ge", "efm"])
     a = ap.parse_args(argv)
     case = int(a.case) if a.case.isdigit() else a.case
     sim = Simulation(case, solver_mode=a.solver, res=a.res, num_particles=a.particles, scheme=a.scheme)

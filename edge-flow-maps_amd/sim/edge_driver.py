@@ -225,10 +225,19 @@ def sub_clamp_u3(u_x, u_y, u_z, t_x, t_y, t_z):
 
 
 def split_psi_hermite(psi_x, psi_y, psi_z, T_x, T_y, T_z, full_map, dx):
-    """run_ed8.py:190-223 / run_ge.py:184-217; full_map = (psi, T_x/y/z, dT_x/y/z, d2T_x/y/z)."""
-    psi, Tx, Ty, Tz, dTx, dTy, dTz, d2x, d2y, d2z = full_map
+    """run_ed8.py:190-223 / run_ge.py:184-217; full_map = (psi, T_x/y/z, dT_x/y/z, d2T_x/y/z).
+
+    full_map = (psi, T_x/y/z) alone is run_ed4.py's map (FD-unrolled Hermite):
+    the map's value and column at every face, which run_ed4.py's map_u_psi
+    (:219-262, interp_grad_hermite_fd_unroll) and map_smoke_psi / clamp_smoke_psi
+    (:285-337, interp_hermite_fd_unroll at the six faces) evaluate in place."""
+    if len(full_map) == 4:
+        psi, Tx, Ty, Tz = full_map
+        tab = hermite_table("fd", psi, (Tx, Ty, Tz), None, None, dx)
+    else:
+        psi, Tx, Ty, Tz, dTx, dTy, dTz, d2x, d2y, d2z = full_map
+        tab = hermite_table("full", psi, (Tx, Ty, Tz), (dTx, dTy, dTz), (d2x, d2y, d2z), dx)
     nx, ny, nz = psi.shape
-    tab = hermite_table("full", psi, (Tx, Ty, Tz), (dTx, dTy, dTz), (d2x, d2y, d2z), dx)
     _lib.call("edge_split_psi_hermite", tab.data_ptr(), as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z), as_ptr(T_x),
               as_ptr(T_y), as_ptr(T_z), nx, ny, nz, float(dx), stream_handle())
     touch(psi_x, psi_y, psi_z, T_x, T_y, T_z)

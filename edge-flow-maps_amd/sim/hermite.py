@@ -242,9 +242,10 @@ def advect_ps_hermite_ed4_fd_unroll(psi_0, psi_1, psi_2, psi_3, psi, T_x, T_y, T
     nx, ny, nz = _dims(psi)
     _check_mac(u_x, u_y, u_z, nx, ny, nz)
     tab = hermite_table("fd", psi, (T_x, T_y, T_z), None, None, dx)
-    _lib.call("edge_march_ed4", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X), as_ptr(psi),
-              as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), nx, ny, nz, float(dx), float(dt), float(epsilon),
-              stream_handle())
+    with timing.span("k_march_ed4"):
+        _lib.call("edge_march_ed4", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X), as_ptr(psi),
+                  as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), nx, ny, nz, float(dx), float(dt), float(epsilon),
+                  stream_handle())
     touch(psi, T_x, T_y, T_z)
 
 

@@ -6,7 +6,7 @@ every kernel is the oracle's literal restatement (edge_oracle*.cpp).
 """
 import numpy as np
 
-from .oracle import HK_FULL, Oracle
+from .oracle import HK_FD, HK_FULL, Oracle
 
 
 def face_coords(a, nx, ny, nz, dx):
@@ -49,7 +49,8 @@ class OracleStep:
         self.i = -1
         self.iters = []
         self.lp = None
-        self.scheme = scheme   # "edge" (run_edge.py), "ed8" (run_ed8.py), "ge" (run_ge.py), "efm" (run_efm.py)
+        # "edge" (run_edge.py), "ed8" (run_ed8.py), "ed4" (run_ed4.py), "ge" (run_ge.py), "efm" (run_efm.py)
+        self.scheme = scheme
         self.u_cache = [None] * reinit_every   # EFM: u after each step's first projection
         self.dts = [0.0] * reinit_every
         self.dye = None
@@ -69,10 +70,22 @@ class OracleStep:
         self.dye_num = int(num)
 
     def march_full_map(self, dt):
-        """ED8 (sim/hermite.py:1290-1338) or GE (1398-1424), unfused, then
-        split_psi_hermite (run_ed8.py:190-223)."""
+        """ED8 (sim/hermite.py:1290-1338), ED4 through the FD-unrolled map
+        (1375-1395) or GE (1398-1424), unfused, then the map's values and
+        columns at the faces (ED8/GE: split_psi_hermite, run_ed8.py:190-223;
+        ED4: the same evaluations inside map_u_psi and map_smoke_psi,
+        run_ed4.py:219-262, 285-303)."""
         o, dx, n = self.o, self.dx, (self.nx, self.ny, self.nz)
         X = self.X.reshape(3, -1)
+        if self.scheme == "ed4":
+            P4 = []
+            for sg in ((-1, -1, -1), (1, 1, -1), (-1, 1, 1), (1, -1, 1)):   # sim/hermite.py:737-742
+                p = o.advance_map_0((X + self.eps * np.array(sg, float)[:, None]).astype(self.dt_), self.u, dx, -dt)
+                p, _ = o.compose(HK_FD, self.psi, self.T, None, None, p, None, dx)
+                P4.append(p.reshape((3,) + n))
+            self.psi, self.T = o.calc_ed4_1(np.stack(P4), self.eps)
+            self.psif, self.Tf = o.split_psi_hermite(HK_FD, self.psi, self.T, None, None, dx)
+            return
         if self.scheme == "ed8":
             P8 = []
             for e in range(8):   # bit 0: x sign, bit 1: y, bit 2: z (sim/hermite.py:704-725)

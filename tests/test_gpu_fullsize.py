@@ -49,9 +49,9 @@ def test_full_size_step_against_the_oracle(dev):
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("scheme", ["ed8", "ge"])
+@pytest.mark.parametrize("scheme", ["ed8", "ed4", "ge"])
 def test_full_size_ed8_ge_step_against_the_oracle(dev, scheme):
-    """Two ED8 (run_ed8.py:364-482) or GE (run_ge.py:363-481) steps at
+    """Two ED8 (run_ed8.py:364-482), ED4 (run_ed4.py:401-515) or GE (run_ge.py:363-481) steps at
     256x128x128 with converged projections, against the FP64 oracle's step,
     with the FP32 oracle's own distance from it as the yardstick (as for EDGE
     above)."""

@@ -76,12 +76,13 @@ def test_one_step_fixed_iterations(dev, clamp):
         assert em < 1e-5
 
 
-@pytest.mark.parametrize("scheme", ["ed8", "ge"])
+@pytest.mark.parametrize("scheme", ["ed8", "ed4", "ge"])
 def test_ed8_ge_steps_fixed_iterations(dev, scheme):
-    """Two steps of the run_ed8.py / run_ge.py loops.  Their maps carry
-    eps-differences (ED8: up to 1/(8 eps^3)) or finite differences whose FP32
-    rounding differs between any two implementations, so the GPU is compared
-    against the FP64 oracle with the FP32 oracle's own error as the yardstick."""
+    """Two steps of the run_ed8.py / run_ed4.py / run_ge.py loops.  Their maps
+    carry eps-differences (ED8: up to 1/(8 eps^3), ED4: 1/(4 eps)) or finite
+    differences whose FP32 rounding differs between any two implementations, so
+    the GPU is compared against the FP64 oracle with the FP32 oracle's own error
+    as the yardstick (the measured values are printed)."""
     from oracle.step import OracleStep
     sim, ref32 = make((32, 16, 16), fixed_iters=6, scheme=scheme)
     u0 = [np.asarray(c, np.float64) for c in ref32.u]
@@ -93,12 +94,14 @@ def test_ed8_ge_steps_fixed_iterations(dev, scheme):
         ref32.substep(dt=dt)
         ref64.substep(dt=dt)
     torch.cuda.synchronize()
-    for g, r32, r64 in zip(sim.velocity_numpy(), ref32.u, ref64.u):
+    for a, (g, r32, r64) in enumerate(zip(sim.velocity_numpy(), ref32.u, ref64.u)):
         e_gpu, e_32 = scale_err(g, r64), scale_err(r32, r64)
+        print(f"{scheme} velocity {'xyz'[a]}: GPU vs FP64 oracle {e_gpu:.3g}, FP32 oracle vs FP64 {e_32:.3g}")
         assert e_gpu <= max(3.0 * e_32, 1e-4), (e_gpu, e_32)
-    # the map position: ED8's is the mean of 8 marched eps-points (FP32 position noise)
+    # the map position: ED8's (ED4's) is the mean of 8 (4) marched eps-points (FP32 position noise)
     e_gpu = scale_err(np.moveaxis(sim.psi_garm.to_numpy(), -1, 0), ref64.psi)
     e_32 = scale_err(ref32.psi, ref64.psi)
+    print(f"{scheme} map: GPU vs FP64 oracle {e_gpu:.3g}, FP32 oracle vs FP64 {e_32:.3g}")
     assert e_gpu <= max(3.0 * e_32, 1e-5), (e_gpu, e_32)
 
 
