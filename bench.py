@@ -39,8 +39,9 @@ FP32_PEAK_TFLOPS = 157.3
 # products per Hermite evaluation, 27 per B-spline) is 33,600 flop/cell for
 # EDGE.  Bytes: compulsory FP32 reads + writes.
 MARCH_FLOP_PER_CELL = 22250.0      # fused EDGE march: 5 points x (RK4 with Jacobian + Hermite compose)
-# ED8: 8 points x (RK4 of the point + full-Hermite value); GE: 1 point (RK4 with Jacobian + Hermite gradient)
-SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 16550.0, "ge": 4430.0}
+# ED8: 8 points x (RK4 of the point + full-Hermite value); ED4: 4 points x (RK4 + FD-unrolled Hermite value);
+# GE: 1 point (RK4 with Jacobian + Hermite gradient)
+SCHEME_FLOP_PER_CELL = {"edge": MARCH_FLOP_PER_CELL, "ed8": 16550.0, "ed4": 8050.0, "ge": 4430.0}
 # the face march (RK4 of a face point and its column): per face and velocity in EFM's backtrack
 FACE_MARCH_FLOP_PER_FACE = 2860.0
 # the reference formulation's count (64 weight products per Hermite evaluation, 27 per

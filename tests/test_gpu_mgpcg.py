@@ -1,10 +1,11 @@
 """GPU parity of the MGPCG projection (plain and mixed) against the CPU oracle.
 
 Tolerances (FP32): operator application and one V-cycle agree with the FP32
-oracle to 1e-5 of scale; a fixed number of CG iterations to 1e-4 (different
-FP64 reduction order changes alpha/beta in the last bits); converged solves are
-compared with the FP64 oracle through the true residual ||b - A p|| <= 1e-6
-||b|| (the relative stop rule) and the pressure difference.
+oracle to 1e-5 of scale; a fixed number of CG iterations to 1e-5 (different
+FP64 reduction order changes alpha/beta in the last bits: measured 5e-7 for the
+pressure, 1.3e-6 for the velocity); converged solves are compared with the
+FP64 oracle through the true residual ||b - A p|| <= 1e-6 ||b|| (the relative
+stop rule) and the pressure difference (1e-5; measured 1e-7 to 6e-7).
 """
 import ctypes
 
@@ -132,7 +133,7 @@ def test_plain_poisson_fixed_iterations(orc32, dev):
         ep = scale_err(N_(p), rp)
         eu = max(scale_err(N_(gu[a]), ru[a]) for a in range(3))
         print(f"{k} iterations: pressure {ep:.3g}, velocity {eu:.3g}")
-        assert ep < 1e-4 and eu < 1e-4
+        assert ep < 1e-5 and eu < 1e-5
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
@@ -152,7 +153,7 @@ def test_poisson_converges_to_relative_tolerance(orc64, dev, mode):
     # pressure against the FP64 oracle (pressure defined up to a constant)
     ep = scale_err(P - P.mean(), rp - rp.mean())
     print(f"mode {mode}: {int(st[0])} iterations, pressure vs the FP64 oracle {ep:.3g}")
-    assert ep < 1e-4
+    assert ep < 1e-5
     div = (N_(gu[0])[1:] - N_(gu[0])[:-1]) + (N_(gu[1])[:, 1:] - N_(gu[1])[:, :-1]) + \
         (N_(gu[2])[:, :, 1:] - N_(gu[2])[:, :, :-1])
     assert np.abs(div).max() < 1e-4 * np.abs(b).max()
@@ -187,7 +188,10 @@ def test_solid_poisson(orc32, orc64, dev, mode):
     assert code == 0
     fl = bm == 0
     P = N_(p)
-    assert scale_err((P - P[fl].mean())[fl], (rp - rp[fl].mean())[fl]) < 2e-4
+    ep = scale_err((P - P[fl].mean())[fl], (rp - rp[fl].mean())[fl])
+    eu = max(scale_err(N_(gu[a]), ru[a]) for a in range(3))
+    print(f"solid mode {mode}: {int(st[0])} iterations, pressure vs the FP64 oracle {ep:.3g}, velocity {eu:.3g}")
+    assert ep < 2e-4
     # the fluid divergence sum is ~1e-6 of rounding here and its sign decides the
     # one-sided correction (sim/mgpcg_solid.py:422): compare with the FP32 oracle
     _, _, rst32 = orc32.poisson(u, levels=3, bm=bm, bvel=bvel, max_iters=200, stop_mode=1, rel_tol=1e-6)
@@ -226,8 +230,10 @@ def test_drop_in_classes(orc32, dev):
     solver.Poisson(*fields)
     ru, rp, rst = orc32.poisson(u, levels=3, stop_mode=0)
     assert abs(solver.stats["iters"] - rst[0]) <= 1
-    for a in range(3):
-        assert scale_err(fields[a].to_numpy(), ru[a]) < 1e-4
+    # the reference stop rule may end one iteration apart (above): 1e-4
+    eu = max(scale_err(fields[a].to_numpy(), ru[a]) for a in range(3))
+    print(f"drop-in class: {solver.stats['iters']} iterations (oracle {int(rst[0])}), velocity {eu:.3g}")
+    assert eu < 1e-4
 
 
 def test_vcycle_parity_deeper_hierarchy(orc32, dev):
@@ -386,7 +392,9 @@ def test_dense_modes_without_the_dense_bottom(orc64, dev, mode):
     print(f"mode {mode}: {int(st[0])} iterations (FP64 oracle {int(rst6[0])}), true residual {res:.3g}, "
           f"FP32 rounding floor {floor:.3g}")
     assert res <= 6 * floor
-    assert scale_err(P - P.mean(), rp - rp.mean()) < 1e-4
+    ep = scale_err(P - P.mean(), rp - rp.mean())
+    print(f"mode {mode} without the dense bottom: pressure vs the FP64 oracle {ep:.3g}")
+    assert ep < 1e-4
     # about as many iterations as the FP64 oracle's reference V-cycle needs
     assert st[0] <= rst6[0] + 3
 
