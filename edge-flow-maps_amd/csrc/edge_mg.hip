@@ -70,11 +70,12 @@ enum Slot {
   S_ZTR0,
   S_NEU,         // all walls: recentre r (sim/mgpcg.py:211-217)
   S_EPS,         // the guard added to the alpha and beta denominators (see set_scalars)
+  S_X0ACT,       // warm start, for the first level-0 pass: 0 nothing, 1 r = w (x0 kept), 2 x = 0 (x0 dropped)
   S_COUNT
 };
 
 // finalize actions of the last block
-enum Fin { F_NONE = 0, F_MEAN, F_B2, F_ZR_INIT, F_ZR, F_ALPHA, F_XR, F_DIVSUM, F_X0, F_SUM2 };
+enum Fin { F_NONE = 0, F_MEAN, F_B2, F_ZR_INIT, F_ZR, F_ALPHA, F_XR, F_DIVSUM, F_X0, F_SUM2, F_MEAN_B2 };
 
 template <class S>
 __device__ __forceinline__ float ld(const S* p, int64_t i);
@@ -208,7 +209,19 @@ __device__ void finalize(double* sc, int fin, double s0, double s1) {
   switch (fin) {
     case F_MEAN:
       sc[S_MEAN] = s0 / sc[S_NCELL];
+      sc[S_X0ACT] = 0.0;
       break;
+    case F_MEAN_B2: {
+      // the recenter folded into the first level-0 pass: the mean (all walls) stays
+      // in S_MEAN for it, and ||r - mean||^2 = sum r^2 - n mean^2 (no solids here)
+      const double m = sc[S_NEU] != 0.0 ? s0 / sc[S_NCELL] : 0.0;
+      const double b2 = s1 - sc[S_NCELL] * m * m;
+      sc[S_MEAN] = m;
+      sc[S_B2] = b2;
+      sc[S_TOL2] = sc[S_RELTOL] * sc[S_RELTOL] * b2;
+      sc[S_X0ACT] = 0.0;
+      break;
+    }
     case F_SUM2:
       sc[S_B2] = s0;
       sc[S_MEAN] = 0.0;
@@ -219,6 +232,7 @@ __device__ void finalize(double* sc, int fin, double s0, double s1) {
       sc[S_TOL2] = sc[S_RELTOL] * sc[S_RELTOL] * s0;  // relative rule; replaced after z.r in reference mode
       break;
     case F_ZR_INIT:
+      sc[S_X0ACT] = 0.0;   // the first level-0 pass has applied it
       sc[S_ZTR] = s0;
       sc[S_ZTR0] = s0;
       sc[S_BETA] = 0.0;
@@ -264,6 +278,8 @@ __device__ void finalize(double* sc, int fin, double s0, double s1) {
     case F_X0:
       sc[S_R02] = s0;
       sc[S_USEX0] = (s0 <= sc[S_B2]) ? 1.0 : 0.0;
+      sc[S_X0ACT] = (s0 <= sc[S_B2]) ? 1.0 : 2.0;
+      if (s0 <= sc[S_B2]) sc[S_MEAN] = 0.0;   // r = b - A x0 is mean-free (1'A = 0 with walls all round)
       break;
     default:
       break;
@@ -407,7 +423,7 @@ __global__ void __launch_bounds__(BS) k_div_sum(Grid g, Faces3 U, const int8_t* 
 }
 
 // init(u_div, -1) (sim/mgpcg.py:67-75): r = -(div - corr) on fluid cells (0 on
-// solids), x = x0 or 0, p = 0; partial sum of r for the recenter.
+// solids), x = x0 or 0, p = 0; partial sums of r (the recenter's mean) and r^2.
 // When b is given instead (solve API), r = b.
 __global__ void __launch_bounds__(BS) k_init(Grid g, Faces3 U, const float* __restrict__ b,
                                             const int8_t* __restrict__ bm0, float* __restrict__ r, float* __restrict__ x,
@@ -416,7 +432,7 @@ __global__ void __launch_bounds__(BS) k_init(Grid g, Faces3 U, const float* __re
   const int64_t N = (int64_t)nx * ny * nz;
   const double* sc = R.sc;
   float corr = (float)sc[S_CORR];
-  double s = 0;
+  double s = 0, s2 = 0;
   FOR_CELLS(q, N) {
     float v;
     if (b) {
@@ -434,8 +450,9 @@ __global__ void __launch_bounds__(BS) k_init(Grid g, Faces3 U, const float* __re
     x[q] = x0 ? x0[q] : 0.f;
     p[q] = 0.f;
     s += (double)v;
+    s2 += (double)v * (double)v;
   }
-  block_partial(R, s, 0.0);
+  block_partial(R, s, s2);
 }
 
 // r -= mean over all cells (sim/mgpcg.py:169-175) and ||r||^2 over fluid cells.
@@ -451,10 +468,12 @@ __global__ void __launch_bounds__(BS) k_recenter(Lv L, float* __restrict__ r, Re
   block_partial(R, s, 0.0);
 }
 
-// warm start: w = r - A x0 and ||w||^2; decide whether to keep x0.
+// warm start: w = (r - mean) - A x0 and ||w||^2; decide whether to keep x0.
+// mean: S_MEAN, 0 once r is recentred, else the recenter still to come.
 __global__ void __launch_bounds__(BS) k_residual_x0(Lv L, int bt, const float* __restrict__ r, const float* __restrict__ x,
                                                    float* __restrict__ w, Red R) {
   const int64_t N = (int64_t)L.n0 * L.n1 * L.n2;
+  const float mean = (float)R.sc[S_MEAN];
   double s = 0;
   FOR_CELLS(q, N) {
     int i, j, k;
@@ -465,7 +484,7 @@ __global__ void __launch_bounds__(BS) k_residual_x0(Lv L, int bt, const float* _
     }
     float ax = num_fluid(L, bt, i, j, k) * x[q] -
                neighbor_sum(L, i, j, k, [&](int a, int b, int c) { return x[lid(L, a, b, c)]; });
-    float v = r[q] - ax;
+    float v = (r[q] - mean) - ax;
     w[q] = v;
     s += (double)v * (double)v;
   }
@@ -832,10 +851,13 @@ __global__ void __launch_bounds__(BS) k_smooth8(Lv L, int bt, const S* __restric
   st8(z + q0, out);
 }
 
-// Level-0 entry (as k_first0).
+// Level-0 entry (as k_first0).  w, x: the warm start's decision, taken here
+// in the first iteration (S_X0ACT, set by its finish): 1 r = w (the residual
+// of the kept x0), 2 x = 0 (x0 dropped, r is the recentred b).
 template <class S>
 __global__ void __launch_bounds__(BS) k_first08(Lv L, int bt, float* __restrict__ r, S* __restrict__ rl,
-                                               S* __restrict__ z, const double* __restrict__ sc, int ignore_done) {
+                                               S* __restrict__ z, const double* __restrict__ sc, int ignore_done,
+                                               const float* __restrict__ w, float* __restrict__ x) {
   if (!ignore_done && sc[S_DONE] != 0.0) return;
   const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
   if (t_ >= (uint32_t)((int64_t)L.n0 * L.n1 * L.n2 >> 3)) return;
@@ -843,9 +865,18 @@ __global__ void __launch_bounds__(BS) k_first08(Lv L, int bt, float* __restrict_
   chunk_ijk(L, t_, i, j, k0);
   const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
   const float mean = (float)sc[S_MEAN];
+  const double act = sc[S_X0ACT];
   const uint64_t M = nb8(L, q0);
   float v[8], zv[8];
-  ld8(r + q0, v);
+  if (act == 1.0) {
+    ld8(w + q0, v);
+  } else {
+    ld8(r + q0, v);
+    if (act == 2.0) {
+      const float zero[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      st8(x + q0, zero);
+    }
+  }
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int k = k0 + c;
@@ -1005,6 +1036,33 @@ __global__ void __launch_bounds__(BS) k_p_Ap8(Lv L, int bt, const S* __restrict_
       }
       st8(Ap + q0, out);
     }
+  }
+  block_partial(R, s, 0.0);
+}
+
+// k_residual_x0 on a chunked level (no solids: every cell is fluid).
+__global__ void __launch_bounds__(BS) k_residual_x08(Lv L, int bt, const float* __restrict__ r,
+                                                    const float* __restrict__ x, float* __restrict__ w, Red R) {
+  const float mean = (float)R.sc[S_MEAN];
+  double s = 0;
+  const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t_ < (uint32_t)((int64_t)L.n0 * L.n1 * L.n2 >> 3)) {
+    int i, j, k0;
+    chunk_ijk(L, t_, i, j, k0);
+    const int64_t q0 = ((int64_t)i * L.n1 + j) * L.n2 + k0;
+    const uint64_t M = nb8(L, q0);
+    Nb8 X;
+    load_nb(L, x, i, j, k0, q0, X);
+    float rv[8], out[8];
+    ld8(r + q0, rv);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const uint32_t mb = nbc(M, c);
+      const float ax = diag8(L, bt, i, j, k0 + c, mb) * X.c[c] - nsum8(L, X, i, j, k0, c, mb);
+      out[c] = (rv[c] - mean) - ax;
+      s += (double)out[c] * (double)out[c];
+    }
+    st8(w + q0, out);
   }
   block_partial(R, s, 0.0);
 }
@@ -1836,7 +1894,8 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig, bool z_ready) {
   if (z_ready) {
     // k_update_xr8 of the previous iteration wrote z (and, mixed, level 0's r)
   } else if (chunked(s.lv[0]))
-    hipLaunchKernelGGL(k_first08<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig);
+    hipLaunchKernelGGL(k_first08<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig,
+                       (const float*)s.Ap, s.x);
   else
     hipLaunchKernelGGL(k_first0<S>, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig);
   const int lt = tail_level(s);
@@ -2224,22 +2283,37 @@ int run_cg_any(Solver& s, hipStream_t st, int max_iters, int check_every, double
   return run_cg<float>(s, st, max_iters, check_every, stats);
 }
 
-// common start: r prepared by k_init; mean in S_MEAN.  Recentre (all-Neumann),
-// ||b||^2, optional warm start.
-int start_solve(Solver& s, hipStream_t st, const float* x0) {
+// Mixed and dense modes without solids: the recenter of r is left to the
+// first level-0 pass (k_first08 / k_first0 subtract S_MEAN, the same
+// arithmetic as k_recenter), ||b||^2 comes from k_init's sums (F_MEAN_B2).
+inline bool fold_recenter(const Solver& s) { return s.mode >= 1 && !s.solid; }
+
+// common start: r prepared by k_init (its finish: F_MEAN_B2 when
+// fold_recenter(s), else F_MEAN); mean in S_MEAN.  Recentre (all-Neumann) and
+// ||b||^2 unless folded, then the optional warm start: w = b - A x0, kept when
+// ||w|| <= ||b||; r = w or x = 0 is applied by the first level-0 pass on a
+// chunked level, here otherwise.
+int start_solve(Solver& s, hipStream_t st, const float* x0, bool fold) {
   Red R = red(s);
   const Lv& L0 = s.lv[0];
   bool neu = (s.bt == 63);
-  if (!neu) hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, st, s.sc, (int)S_MEAN, 0.0);
-  hipLaunchKernelGGL(k_recenter, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.r, R);
-  finish(s, st, red_blocks(s.N), F_B2, 0);
-  EDGE_CHECK_LAUNCH();
-  if (x0) {
-    hipLaunchKernelGGL(k_residual_x0, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, s.r, s.x, s.Ap, R);
-    finish(s, st, red_blocks(s.N), F_X0, 0);
-    hipLaunchKernelGGL(k_apply_x0, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N, s.r, s.Ap, s.x, s.sc);
+  if (!fold) {
+    if (!neu) hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, st, s.sc, (int)S_MEAN, 0.0);
+    hipLaunchKernelGGL(k_recenter, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.r, R);
+    finish(s, st, red_blocks(s.N), F_B2, 0);
     EDGE_CHECK_LAUNCH();
-    // with walls all round, 1'A = 0, so r = b - A x0 stays mean-free
+  }
+  if (x0) {
+    if (chunked(L0) && !s.solid) {
+      hipLaunchKernelGGL(k_residual_x08, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, s.r, s.x, s.Ap, R);
+      finish(s, st, chunk_blocks(s.N), F_X0, 0);
+    } else {
+      hipLaunchKernelGGL(k_residual_x0, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, s.r, s.x, s.Ap, R);
+      finish(s, st, red_blocks(s.N), F_X0, 0);
+    }
+    if (!chunked(L0))   // k_first0 does not read S_X0ACT
+      hipLaunchKernelGGL(k_apply_x0, dim3(blocks_for(s.N, BS)), dim3(BS), 0, st, s.N, s.r, s.Ap, s.x, s.sc);
+    EDGE_CHECK_LAUNCH();
   }
   return EDGE_OK;
 }
@@ -2439,9 +2513,9 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   }
   hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), U, nullptr,
                      s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s), walls);
-  finish(s, st, red_blocks(s.N), F_MEAN, 0);
+  finish(s, st, red_blocks(s.N), fold_recenter(s) ? F_MEAN_B2 : F_MEAN, 0);
   EDGE_CHECK_LAUNCH();
-  rc = start_solve(s, st, x0);
+  rc = start_solve(s, st, x0, fold_recenter(s));
   if (rc) return rc;
   int cg = run_cg_any(s, st, max_iters, check_every, stats);
   if (cg != EDGE_OK && cg != EDGE_ENOCONV) return cg;
@@ -2482,7 +2556,7 @@ int edge_mg_rhs(void* handle, float* ux, float* uy, float* uz, float* b, void* s
                      s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, nullptr, red(s), 0);
   finish(s, st, red_blocks(s.N), F_MEAN, 0);
   EDGE_CHECK_LAUNCH();
-  rc = start_solve(s, st, nullptr);
+  rc = start_solve(s, st, nullptr, false);   // the recentred b is the result
   if (rc) return rc;
   EDGE_CHECK_HIP(hipMemcpyAsync(b, s.r, s.N * 4, hipMemcpyDeviceToDevice, st));
   return EDGE_OK;
@@ -2502,9 +2576,9 @@ int edge_mg_solve(void* handle, const float* b, float* x, int max_iters, int sto
   if (rc) return rc;
   hipLaunchKernelGGL(k_init, dim3(red_blocks(s.N)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), Faces3{}, b,
                      s.solid ? (const int8_t*)s.bm[0] : nullptr, s.r, s.x, s.p, x0, red(s), 0);
-  finish(s, st, red_blocks(s.N), F_MEAN, 0);
+  finish(s, st, red_blocks(s.N), fold_recenter(s) ? F_MEAN_B2 : F_MEAN, 0);
   EDGE_CHECK_LAUNCH();
-  rc = start_solve(s, st, x0);
+  rc = start_solve(s, st, x0, fold_recenter(s));
   if (rc) return rc;
   int cg = run_cg_any(s, st, max_iters, check_every, stats);
   if (cg != EDGE_OK && cg != EDGE_ENOCONV) return cg;
