@@ -531,6 +531,63 @@ __global__ void __launch_bounds__(BS) k_subtract_grad(Grid g, Faces3 U, Faces3 F
   if (F.f[a]) F.f[a][l] = -(pr - pl);
 }
 
+// k_subtract_grad with four faces per thread (the same values): x and y faces
+// as aligned float4 runs along k (nz % 4 == 0, 16-byte aligned fields), z faces
+// (rows of nz + 1) four consecutive faces of a row per thread.
+__global__ void __launch_bounds__(BS) k_subtract_grad4(Grid g, Faces3 U, Faces3 F, const float* __restrict__ P,
+                                                      int walls) {
+  const uint32_t nx = g.nx, ny = g.ny, nz = g.nz, q4 = nz >> 2;
+  const uint32_t n0 = (nx + 1) * ny * q4, n1 = nx * (ny + 1) * q4;
+  const uint32_t zq = (nz + 4) >> 2;   // quads per row of z faces
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n0 + n1) {
+    const int a = t < n0 ? 0 : 1;
+    if (a == 1) t -= n0;
+    const uint32_t row = t / q4, k0 = (t - row * q4) << 2;
+    const uint32_t d1 = ny + (uint32_t)a;   // the face grid's j extent
+    const uint32_t i = row / d1, j = row - i * d1;
+    const uint32_t I = a == 0 ? i : j, n = a == 0 ? nx : ny;
+    float4 pr = make_float4(0.f, 0.f, 0.f, 0.f), pl = pr;
+    if (I < n) pr = *(const float4*)(P + ((size_t)i * ny + j) * nz + k0);
+    if (I > 0) pl = *(const float4*)(P + ((size_t)(i - (a == 0)) * ny + (j - (a == 1))) * nz + k0);
+    const size_t l = (size_t)row * nz + k0;
+    const float4 d = make_float4(pr.x - pl.x, pr.y - pl.y, pr.z - pl.z, pr.w - pl.w);
+    const bool wall = walls && ((I == 0 && ((walls >> (2 * a)) & 1)) || (I == n && ((walls >> (2 * a + 1)) & 1)));
+    float4* u = (float4*)(U.f[a] + l);
+    if (wall) {
+      *u = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      float4 v = *u;
+      v.x -= d.x;
+      v.y -= d.y;
+      v.z -= d.z;
+      v.w -= d.w;
+      *u = v;
+    }
+    if (F.f[a]) *(float4*)(F.f[a] + l) = make_float4(-d.x, -d.y, -d.z, -d.w);
+    return;
+  }
+  t -= n0 + n1;
+  if (t >= nx * ny * zq) return;
+  const uint32_t row = t / zq, k0 = (t - row * zq) << 2;
+  const uint32_t i = row / ny, j = row - i * ny;
+  const float* pc = P + ((size_t)i * ny + j) * nz;
+#pragma unroll
+  for (uint32_t c = 0; c < 4; ++c) {
+    const uint32_t k = k0 + c;
+    if (k > nz) break;
+    const float pr = k < nz ? pc[k] : 0.f;
+    const float pl = k > 0 ? pc[k - 1] : 0.f;
+    const size_t l = (size_t)row * (nz + 1) + k;
+    const bool wall = walls && ((k == 0 && ((walls >> 4) & 1)) || (k == nz && ((walls >> 5) & 1)));
+    if (wall)
+      U.f[2][l] = 0.f;
+    else
+      U.f[2][l] -= (pr - pl);
+    if (F.f[2]) F.f[2][l] = -(pr - pl);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // CG vector kernels
 // ---------------------------------------------------------------------------
@@ -2520,8 +2577,17 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   int cg = run_cg_any(s, st, max_iters, check_every, stats);
   if (cg != EDGE_OK && cg != EDGE_ENOCONV) return cg;
   // get_result + subtract_grad_p + apply_bc
-  hipLaunchKernelGGL(k_subtract_grad, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), U,
-                     Faces3{{fx, fy, fz}}, (const float*)s.x, walls);
+  auto al16 = [](const void* q) { return ((uintptr_t)q & 15u) == 0; };
+  if (s.nz % 4 == 0 && al16(U.f[0]) && al16(U.f[1]) && (!fx || al16(fx)) && (!fy || al16(fy)) &&
+      (int64_t)s.nx * s.ny * (s.nz + 4) < (int64_t)1 << 31) {
+    const int64_t quads = ((int64_t)(s.nx + 1) * s.ny + (int64_t)s.nx * (s.ny + 1)) * (s.nz / 4) +
+                          (int64_t)s.nx * s.ny * ((s.nz + 4) / 4);
+    hipLaunchKernelGGL(k_subtract_grad4, dim3(blocks_for(quads, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f),
+                       U, Faces3{{fx, fy, fz}}, (const float*)s.x, walls);
+  } else {
+    hipLaunchKernelGGL(k_subtract_grad, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f),
+                       U, Faces3{{fx, fy, fz}}, (const float*)s.x, walls);
+  }
   if (s.solid)
     hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), s.bt,
                        U, s.user_bm, s.user_bvel);
