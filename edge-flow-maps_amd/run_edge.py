@@ -36,8 +36,9 @@ if _HERE not in sys.path:
 
 from edge_hip import _lib  # noqa: E402
 from edge_hip.field import Field, stream_handle, ti  # noqa: E402
-from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smoke_psi, sub_clamp_u3,  # noqa: E402
-                             map_smoke, map_smoke_psi, map_u, map_u_psi, march_phi_grid, reset_to_identity)
+from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smoke_psi, map_smoke,  # noqa: E402
+                             map_smoke_psi, map_u, map_u_psi, march_phi_grid, prefetch_max_particle_speed,
+                             prefetch_max_speed, reset_to_identity, sub_clamp_u3)
 from sim import lp as lp_kernels  # noqa: E402
 from sim.edge_driver import backtrack_faces, clamp_smoke, split_psi_hermite  # noqa: E402
 from sim.hermite import (advect_ps_garm, advect_ps_hermite_ed4_fd_unroll, advect_ps_hermite_ed8,  # noqa: E402
@@ -475,6 +476,10 @@ class Simulation:
             self.poisson(buffers=(self.vis_x, self.vis_y, self.vis_z), slot=1)
         else:
             self.poisson(slot=1)
+        # the next step's CFL input, launched now so that it is ready when that step starts
+        prefetch_max_speed(self.u_x, self.u_y, self.u_z)
+        if self.lp is not None:
+            prefetch_max_particle_speed(self.laden_particles_vel)
         self._mark("end")
         return curr_dt
 

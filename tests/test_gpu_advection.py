@@ -225,6 +225,29 @@ def test_driver_maps_parity(orc32, orc64, dev):
     assert ops.max_speed(gu) == pytest.approx(orc32.max_speed(u), rel=1e-6)
 
 
+def test_prefetched_max_speed(orc32, dev):
+    """prefetch_max_speed (launched at the end of a step) gives calc_max_speed's
+    value; a field written since (every wrapper bumps its version) is measured again."""
+    from edge_hip import Field
+    from sim.edge_driver import calc_max_speed, prefetch_max_speed
+    from util.taichi_utils import scale_field
+    u = smooth_velocity(*N3, DX, seed=11)
+    f = []
+    for c in u:
+        g = Field(None, float, c.shape)
+        g.from_numpy(c.astype(np.float32))
+        f.append(g)
+    direct = calc_max_speed(*f)
+    assert direct == pytest.approx(orc32.max_speed(u), rel=1e-6)
+    prefetch_max_speed(*f)
+    assert calc_max_speed(*f) == direct
+    prefetch_max_speed(*f)
+    scale_field(f[0], 3.0, f[0])
+    u3 = [u[0] * 3.0, u[1], u[2]]
+    assert calc_max_speed(*f) == pytest.approx(orc32.max_speed(u3), rel=1e-6)
+    assert calc_max_speed(*f) != direct
+
+
 def test_split_psi_and_variant_marches(orc32, orc64, dev):
     """ED8 (sim/hermite.py:1290-1338), GE (1398-1424), ED4 (1375-1395) and
     split_psi_hermite (run_ed8.py:190-223) against the oracle's unfused sequence."""
