@@ -531,14 +531,13 @@ __global__ void __launch_bounds__(BS) k_subtract_grad(Grid g, Faces3 U, Faces3 F
   if (F.f[a]) F.f[a][l] = -(pr - pl);
 }
 
-// k_subtract_grad with four faces per thread (the same values): x and y faces
-// as aligned float4 runs along k (nz % 4 == 0, 16-byte aligned fields), z faces
-// (rows of nz + 1) four consecutive faces of a row per thread.
+// k_subtract_grad with four faces per thread for the x and y faces (the same
+// values): aligned float4 runs along k (nz % 4 == 0, 16-byte aligned fields);
+// z faces (rows of nz + 1, not 16-byte aligned) one per thread.
 __global__ void __launch_bounds__(BS) k_subtract_grad4(Grid g, Faces3 U, Faces3 F, const float* __restrict__ P,
                                                       int walls) {
   const uint32_t nx = g.nx, ny = g.ny, nz = g.nz, q4 = nz >> 2;
   const uint32_t n0 = (nx + 1) * ny * q4, n1 = nx * (ny + 1) * q4;
-  const uint32_t zq = (nz + 4) >> 2;   // quads per row of z faces
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n0 + n1) {
     const int a = t < n0 ? 0 : 1;
@@ -568,24 +567,18 @@ __global__ void __launch_bounds__(BS) k_subtract_grad4(Grid g, Faces3 U, Faces3 
     return;
   }
   t -= n0 + n1;
-  if (t >= nx * ny * zq) return;
-  const uint32_t row = t / zq, k0 = (t - row * zq) << 2;
+  if (t >= nx * ny * (nz + 1)) return;
+  const uint32_t row = t / (nz + 1), k = t - row * (nz + 1);
   const uint32_t i = row / ny, j = row - i * ny;
   const float* pc = P + ((size_t)i * ny + j) * nz;
-#pragma unroll
-  for (uint32_t c = 0; c < 4; ++c) {
-    const uint32_t k = k0 + c;
-    if (k > nz) break;
-    const float pr = k < nz ? pc[k] : 0.f;
-    const float pl = k > 0 ? pc[k - 1] : 0.f;
-    const size_t l = (size_t)row * (nz + 1) + k;
-    const bool wall = walls && ((k == 0 && ((walls >> 4) & 1)) || (k == nz && ((walls >> 5) & 1)));
-    if (wall)
-      U.f[2][l] = 0.f;
-    else
-      U.f[2][l] -= (pr - pl);
-    if (F.f[2]) F.f[2][l] = -(pr - pl);
-  }
+  const float pr = k < nz ? pc[k] : 0.f;
+  const float pl = k > 0 ? pc[k - 1] : 0.f;
+  const bool wall = walls && ((k == 0 && ((walls >> 4) & 1)) || (k == nz && ((walls >> 5) & 1)));
+  if (wall)
+    U.f[2][t] = 0.f;
+  else
+    U.f[2][t] -= (pr - pl);
+  if (F.f[2]) F.f[2][t] = -(pr - pl);
 }
 
 // ---------------------------------------------------------------------------
@@ -2581,7 +2574,7 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   if (s.nz % 4 == 0 && al16(U.f[0]) && al16(U.f[1]) && (!fx || al16(fx)) && (!fy || al16(fy)) &&
       (int64_t)s.nx * s.ny * (s.nz + 4) < (int64_t)1 << 31) {
     const int64_t quads = ((int64_t)(s.nx + 1) * s.ny + (int64_t)s.nx * (s.ny + 1)) * (s.nz / 4) +
-                          (int64_t)s.nx * s.ny * ((s.nz + 4) / 4);
+                          (int64_t)s.nx * s.ny * (s.nz + 1);
     hipLaunchKernelGGL(k_subtract_grad4, dim3(blocks_for(quads, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f),
                        U, Faces3{{fx, fy, fz}}, (const float*)s.x, walls);
   } else {
