@@ -117,10 +117,36 @@ __global__ void __launch_bounds__(256) k_split_psi(Grid g, const float4* __restr
 
 // util/taichi_utils.py:171-227: trilinear value or 8-tap min/max of a
 // cell-centred field with nc <= 4 channels, BL 0.5.
+// A smoke field keeps its NC channels together per cell (channel c of cell q
+// at q * NC + c): a 4-channel tap is one 16-byte load.
+template <int NC>
+__device__ __forceinline__ void ld_cell(const float* __restrict__ s, int64_t q, float v[NC]);
+template <>
+__device__ __forceinline__ void ld_cell<1>(const float* __restrict__ s, int64_t q, float v[1]) {
+  v[0] = s[q];
+}
+template <>
+__device__ __forceinline__ void ld_cell<4>(const float* __restrict__ s, int64_t q, float v[4]) {
+  const float4 t = ((const float4*)s)[q];
+  v[0] = t.x;
+  v[1] = t.y;
+  v[2] = t.z;
+  v[3] = t.w;
+}
+template <int NC>
+__device__ __forceinline__ void st_cell(float* __restrict__ s, int64_t q, const float v[NC]);
+template <>
+__device__ __forceinline__ void st_cell<1>(float* __restrict__ s, int64_t q, const float v[1]) {
+  s[q] = v[0];
+}
+template <>
+__device__ __forceinline__ void st_cell<4>(float* __restrict__ s, int64_t q, const float v[4]) {
+  ((float4*)s)[q] = make_float4(v[0], v[1], v[2], v[3]);
+}
+
 template <int NC>
 __device__ __forceinline__ void linear_taps(const Grid& g, const float* __restrict__ s0, const float p[3], int mode,
                                             float out[NC], float mn[NC], float mx[NC]) {
-  int64_t N = ncell(g);
   float ss[3];
   int i0[3], i1[3];
   float fr[3];
@@ -148,10 +174,11 @@ __device__ __forceinline__ void linear_taps(const Grid& g, const float* __restri
         float wx = a ? fr[0] : 1.0f - fr[0];
         float wy = b ? fr[1] : 1.0f - fr[1];
         float wz = cz ? fr[2] : 1.0f - fr[2];
-        int64_t q = cell_id(g, ii, jj, kk);
+        float tv[NC];
+        ld_cell<NC>(s0, cell_id(g, ii, jj, kk), tv);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          float v = s0[c * N + q];
+          float v = tv[c];
           if (mode == 0) {
             out[c] += v * wx * wy * wz;
           } else {
@@ -190,21 +217,29 @@ __global__ void __launch_bounds__(256) k_map_smoke(Grid g, int mode, const float
   }
   float out[NC], mn[NC], mx[NC];
   linear_taps<NC>(g, s0, p, mode, out, mn, mx);
+  float v2[NC], res[NC];
+  if (s2) ld_cell<NC>(s2, q, v2);
+  if (mode == 0) {
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    if (mode == 0) {
+    for (int c = 0; c < NC; ++c) {
       float r = out[c];
       if (s2) {
-        r = r + m * s2[c * N + q];
+        r = r + m * v2[c];
         r = sc * r;
       }
-      s1[c * N + q] = r;
-    } else {
-      float v = s1[c * N + q];
-      if (s2) v = v + m * s2[c * N + q];
-      s1[c * N + q] = fminf(fmaxf(v, mn[c]), mx[c]);
+      res[c] = r;
+    }
+  } else {
+    float v1[NC];
+    ld_cell<NC>(s1, q, v1);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      float v = v1[c];
+      if (s2) v = v + m * v2[c];
+      res[c] = fminf(fmaxf(v, mn[c]), mx[c]);
     }
   }
+  st_cell<NC>(s1, q, res);
 }
 
 // run_edge.py:443-466 map_smoke_psi (mode 0) and 480-505 clamp_smoke_psi
@@ -231,11 +266,13 @@ __global__ void __launch_bounds__(256) k_map_smoke_psi(Grid g, int mode, const f
   float p[3] = {(1.0f / 6.0f) * acc[0], (1.0f / 6.0f) * acc[1], (1.0f / 6.0f) * acc[2]};
   float out[NC], mn[NC], mx[NC];
   linear_taps<NC>(g, s0, p, mode, out, mn, mx);
+  if (mode == 1) {
+    float v1[NC];
+    ld_cell<NC>(s1, q, v1);
 #pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    if (mode == 0) s1[c * N + q] = out[c];
-    else s1[c * N + q] = fminf(fmaxf(s1[c * N + q], mn[c]), mx[c]);
+    for (int c = 0; c < NC; ++c) out[c] = fminf(fmaxf(v1[c], mn[c]), mx[c]);
   }
+  st_cell<NC>(s1, q, out);
 }
 
 
@@ -429,18 +466,12 @@ __global__ void __launch_bounds__(RING_CHUNK) k_vortex_ring(Grid g, float w, flo
   vf[q] += acc[0];
   vf[N + q] += acc[1];
   vf[2 * N + q] += acc[2];
-  float s3 = smokef[3 * N + q] + sm;
-  if (s3 > 0.002f) {
-    smokef[q] = color.x;
-    smokef[N + q] = color.y;
-    smokef[2 * N + q] = color.z;
-    smokef[3 * N + q] = 1.0f;
-  } else {
-    smokef[q] = 0.f;
-    smokef[N + q] = 0.f;
-    smokef[2 * N + q] = 0.f;
-    smokef[3 * N + q] = 0.f;
-  }
+  float4* sq = (float4*)smokef + q;   // the 4 channels of cell q, together
+  float s3 = sq->w + sm;
+  if (s3 > 0.002f)
+    *sq = make_float4(color.x, color.y, color.z, 1.0f);
+  else
+    *sq = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // util/taichi_utils.py:103-116 split_central_vector: cell vector -> MAC faces.
@@ -551,7 +582,7 @@ __global__ void __launch_bounds__(256) k_vtk_frame(Grid g, Mac U, const float* _
   }
   wn[t] = sqrtf(w2);   // correctly rounded (hipcc's default for f32 sqrt)
   const int64_t q = ((int64_t)i * g.ny + j) * g.nz + k;
-  for (int c = 0; c < nc; ++c) sm[t * nc + c] = smoke[c * N + q];
+  for (int c = 0; c < nc; ++c) sm[t * nc + c] = smoke[q * nc + c];
 }
 
 }  // namespace
@@ -621,6 +652,9 @@ int edge_split_psi_hermite(const float* tab, float* phix, float* phiy, float* ph
   return EDGE_OK;
 }
 
+// NULL counts as aligned (an absent optional field)
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
 static int map_smoke_launch(const char* who, int mode, const float* s0, float* s1, const float* s2, float m,
                             float sc, int nc, const float* phix, const float* phiy, const float* phiz, int nx, int ny,
                             int nz, float dx, void* stream) {
@@ -628,6 +662,8 @@ static int map_smoke_launch(const char* who, int mode, const float* s0, float* s
   EDGE_REQUIRE(s0 && s1 && phix && phiy && phiz && (mode == 0 || mode == 1), "%s: bad arguments", who);
   EDGE_REQUIRE(nc == 4 || nc == 1, "%s: nc must be 1 or 4 (got %d)", who, nc);
   EDGE_REQUIRE(s0 != s1 && s2 != s1 && s2 != s0, "%s: the smoke fields must be distinct", who);
+  EDGE_REQUIRE(nc == 1 || (aligned16(s0) && aligned16(s1) && aligned16(s2)),
+               "%s: 4-channel smoke fields must be 16-byte aligned", who);
   Grid g = make_grid(nx, ny, nz, dx);
   dim3 grid(blocks_for(ncell(g), 256));
   if (nc == 4)
@@ -664,6 +700,8 @@ int edge_map_smoke_psi(int mode, const float* s0, float* s1, int nc, const float
   EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_map_smoke_psi: bad dims");
   EDGE_REQUIRE(s0 && s1 && tab && (mode == 0 || mode == 1), "edge_map_smoke_psi: bad arguments");
   EDGE_REQUIRE(nc == 4 || nc == 1, "edge_map_smoke_psi: nc must be 1 or 4 (got %d)", nc);
+  EDGE_REQUIRE(nc == 1 || (aligned16(s0) && aligned16(s1)),
+               "edge_map_smoke_psi: 4-channel smoke fields must be 16-byte aligned");
   Grid g = make_grid(nx, ny, nz, dx);
   dim3 grid(blocks_for(ncell(g), 256));
   if (nc == 4)
@@ -789,6 +827,7 @@ int edge_vortex_ring(float w, float rad, float delta, const float* c, const floa
   clear_error();
   EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_vortex_ring: bad dims");
   EDGE_REQUIRE(c && ex && ey && pf && vf && smokef && color && num_samples > 0, "edge_vortex_ring: bad arguments");
+  EDGE_REQUIRE(aligned16(smokef), "edge_vortex_ring: the smoke field must be 16-byte aligned");
   Grid g = make_grid(nx, ny, nz, 1.0f);
   hipLaunchKernelGGL(k_vortex_ring, dim3(blocks_for(ncell(g), RING_CHUNK)), dim3(RING_CHUNK), 0, (hipStream_t)stream,
                      g, w, rad, delta, make_float3(c[0], c[1], c[2]), make_float3(ex[0], ex[1], ex[2]),

@@ -1120,12 +1120,11 @@ __global__ void __launch_bounds__(BS) k_residual_x08(Lv L, int bt, const float* 
 // p = z + beta p, x += alpha p, r -= alpha Ap, r.r and sum r (as k_update_xr).
 template <class S>
 // zn (may equal z, hence no __restrict__ on either): when not null, also the next
-// V-cycle's fill(0) + first red half-sweep from the updated residual (the dense
-// coarsest solve handles the constant: no recentering); rln, when not null, the
-// level-0 copy of r in the levels' storage type (mixed mode), else r is it.
+// V-cycle's fill(0) + first red half-sweep from the updated residual (dense mode:
+// FP32 levels whose level-0 r is the CG residual itself, no recentering).
 __global__ void __launch_bounds__(BS) k_update_xr8(Lv L, int bt, const S* z, float* __restrict__ x,
                                                   float* __restrict__ r, float* __restrict__ p,
-                                                  const float* __restrict__ Ap, S* zn, S* __restrict__ rln, Red R) {
+                                                  const float* __restrict__ Ap, S* zn, Red R) {
   EXIT_IF_DONE(R.sc);
   const float alpha = (float)R.sc[S_ALPHA];
   const float beta = (float)R.sc[S_BETA];
@@ -1154,7 +1153,6 @@ __global__ void __launch_bounds__(BS) k_update_xr8(Lv L, int bt, const S* z, flo
     st8(p + q0, pv);
     st8(x + q0, xv);
     st8(r + q0, rv);
-    if (rln) st8(rln + q0, rv);
     if (zn) {
       int i, j, k0;
       chunk_ijk(L, t_, i, j, k0);
@@ -1942,7 +1940,7 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig, bool z_ready) {
   S* r0 = (S*)s.rl[0];
   S* z0 = (S*)s.zl[0];
   if (z_ready) {
-    // k_update_xr8 of the previous iteration wrote z (and, mixed, level 0's r)
+    // k_update_xr8 of the previous iteration wrote z (dense mode)
   } else if (chunked(s.lv[0]))
     hipLaunchKernelGGL(k_first08<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig,
                        (const float*)s.Ap, s.x);
@@ -2185,14 +2183,16 @@ int prepare_masks(Solver& s, hipStream_t st) {
 
 // The update half of a CG iteration: Ap, alpha, x, r, r.r and the stop test
 // (sim/mgpcg.py:234-259).
-// Dense and mixed modes on a chunked level 0: r needs no recentering (the
-// dense coarsest solve handles the constant), so the update of r can also
-// write level 0's copy of r (mixed: BF16; dense: r itself) and the next
-// V-cycle's fill(0) + first red half-sweep, which saves k_first08's pass over
-// r and z.  Only while the dense bottom exists: without
+// Dense mode on a chunked level 0: r needs no recentering (the dense coarsest
+// solve handles the constant), and rl[0] is r itself, so the update of r can
+// also write the next V-cycle's fill(0) + first red half-sweep, which saves
+// k_first08's pass over r and z.  Only while the dense bottom exists: without
 // it (coarsest level above DENSE_MAX fluid cells, or a factorisation that
 // failed) the red/black bottom needs the recentred residual of k_first08.
-inline bool z_ready(const Solver& s) { return s.mode >= 1 && s.dense_ok && chunked(s.lv[0]); }
+// Not in mixed mode: with BF16 levels the unrecentred residual's drift stalls
+// the solve at 512x256x256 (|r|/|b| 0.85 after 400 iterations, against 8-9
+// iterations with the recentering of k_first08).
+inline bool z_ready(const Solver& s) { return s.mode == 2 && s.dense_ok && chunked(s.lv[0]); }
 
 template <class S>
 void cg_update(Solver& s, hipStream_t st) {
@@ -2201,11 +2201,10 @@ void cg_update(Solver& s, hipStream_t st) {
   if (chunked(L0)) {
     hipLaunchKernelGGL(k_p_Ap8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
     finish(s, st, chunk_blocks(s.N), F_ALPHA, 1);
-    // dense and mixed modes: the next V-cycle's first red half-sweep here (see z_ready)
+    // dense mode: the next V-cycle's first red half-sweep here (see z_ready)
     S* zn = z_ready(s) ? (S*)s.zl[0] : nullptr;
-    S* rln = (zn && s.rl[0] != (void*)s.r) ? (S*)s.rl[0] : nullptr;
     hipLaunchKernelGGL(k_update_xr8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.x,
-                       s.r, s.p, s.Ap, zn, rln, R);
+                       s.r, s.p, s.Ap, zn, R);
     finish(s, st, chunk_blocks(s.N), F_XR, 1);
   } else {
     hipLaunchKernelGGL(k_p_Ap<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);

@@ -54,7 +54,11 @@ def require_gpu():
 
 
 class Field:
-    """A scalar (n=None) or n-vector field on a grid `shape`, stored on the GPU."""
+    """A scalar (n=None) or n-vector field on a grid `shape`, stored on the GPU.
+
+    Layout: an n-vector field is n planes (data shape (n,) + shape), except a
+    4-vector (the smoke's RGBA), whose channels sit together per cell (data
+    shape shape + (4,)): the smoke kernels gather a 16-byte tap."""
 
     def __init__(self, n, dtype, shape, device="cuda"):
         require_gpu()
@@ -64,7 +68,14 @@ class Field:
         self.shape = tuple(int(s) for s in shape)
         self.n = n
         self.dtype = _torch_dtype(dtype)
-        tshape = ((n,) if n else ()) + (self.shape if self.shape else (1,))
+        self.channels_last = n == 4
+        grid = self.shape if self.shape else (1,)
+        if not n:
+            tshape = grid
+        elif self.channels_last:
+            tshape = grid + (n,)
+        else:
+            tshape = (n,) + grid
         self.data = torch.zeros(tshape, dtype=self.dtype, device=device)
         self.touch()
 
@@ -109,7 +120,7 @@ class Field:
         a = self.data.detach().cpu().numpy()
         if not self.shape:
             return a.reshape(())
-        if self.n:
+        if self.n and not self.channels_last:
             return np.ascontiguousarray(np.moveaxis(a, 0, -1))
         return a.copy()
 
@@ -117,7 +128,8 @@ class Field:
         arr = np.asarray(arr)
         if self.n:
             assert arr.shape == self.shape + (self.n,), (arr.shape, self.shape, self.n)
-            arr = np.moveaxis(arr, -1, 0)
+            if not self.channels_last:
+                arr = np.moveaxis(arr, -1, 0)
         else:
             assert arr.shape == self.shape, (arr.shape, self.shape)
         t = torch.from_numpy(np.ascontiguousarray(arr)).to(self.dtype)
@@ -138,6 +150,8 @@ class Field:
 
     # -- SoA planes (kernels) -----------------------------------------------------
     def plane_ptr(self, c):
+        if self.channels_last:
+            raise TypeError("a 4-vector field keeps its channels per cell: it has no planes")
         return self.data[c].data_ptr()
 
     def __repr__(self):

@@ -124,7 +124,9 @@ def map_u_psi(u0, tab, dx):
 
 
 def map_smoke(mode, s0, s1, phi, dx):
-    nc, nx, ny, nz = s0.shape
+    """s0, s1: (nx, ny, nz, nc) channels per cell (nc 4), or (nx, ny, nz)."""
+    nx, ny, nz = s0.shape[:3]
+    nc = s0.shape[3] if s0.dim() == 4 else 1
     s1 = s1.clone() if s1 is not None else torch.empty_like(s0)
     _lib.call("edge_map_smoke", mode, _p(s0), _p(s1), nc, *[_p(c) for c in phi], nx, ny, nz, float(dx),
               stream_handle())
@@ -132,7 +134,9 @@ def map_smoke(mode, s0, s1, phi, dx):
 
 
 def map_smoke_psi(mode, s0, s1, tab, dx):
-    nc, nx, ny, nz = s0.shape
+    """s0, s1: (nx, ny, nz, nc) channels per cell (nc 4), or (nx, ny, nz)."""
+    nx, ny, nz = s0.shape[:3]
+    nc = s0.shape[3] if s0.dim() == 4 else 1
     s1 = s1.clone() if s1 is not None else torch.empty_like(s0)
     _lib.call("edge_map_smoke_psi", mode, _p(s0), _p(s1), nc, _p(tab), nx, ny, nz, float(dx), stream_handle())
     return s1
@@ -181,6 +185,7 @@ def calc_fd3(T, dx):
 
 
 def vortex_ring(w, rad, delta, c, ex, ey, X, vf, smoke, color, num_samples):
+    """vf: (3, nx, ny, nz) planes; smoke: (nx, ny, nz, 4) channels per cell."""
     nx, ny, nz = X.shape[1:]
     arr = lambda v: (ctypes.c_float * 3)(*[float(x) for x in v])  # noqa: E731
     a_c, a_ex, a_ey, a_col = arr(c), arr(ex), arr(ey), arr(color)

@@ -10,7 +10,9 @@
  *    comment says "host".  `stream` is a hipStream_t (torch's
  *    current_stream().cuda_stream); work is enqueued, not synchronised.
  *  - Grids: nx, ny, nz cells; a field of n components is n contiguous planes
- *    of nx*ny*nz floats, index (i*ny + j)*nz + k.  MAC faces: u_x
+ *    of nx*ny*nz floats, index (i*ny + j)*nz + k, except the smoke: its nc
+ *    channels (RGBA, nc = 4) sit together per cell, channel c of cell q at
+ *    q*nc + c (one 16-byte load per interpolation tap; 16-byte aligned).  MAC faces: u_x
  *    (nx+1, ny, nz), u_y (nx, ny+1, nz), u_z (nx, ny, nz+1).  A map row T_c
  *    is grad psi_c (3 planes); dT_c holds [d2/dydz, d2/dxdz, d2/dxdy] psi_c.
  *  - All arithmetic is FP32 unless a function says otherwise.
@@ -173,7 +175,8 @@ int edge_scale(const float* a, float alpha, float* r, long n, void* stream);
 /* Taichi Field.copy_from / fill (run_edge.py:621-624, 644). */
 int edge_copy(const float* src, float* dst, long n, void* stream);
 int edge_fill(float* p, float v, long n, void* stream);
-/* add_vortex_ring_and_smoke (sim/init_conditions.py:13-36); c, ex, ey, color: host float[3]. */
+/* add_vortex_ring_and_smoke (sim/init_conditions.py:13-36); c, ex, ey, color: host float[3];
+ * vf: 3 planes, smokef: 4 channels per cell. */
 int edge_vortex_ring(float w, float rad, float delta, const float* c, const float* ex, const float* ey,
                      const float* pf, float* vf, float* smokef, const float* color, int num_samples, int nx, int ny,
                      int nz, void* stream);
@@ -184,8 +187,8 @@ int edge_get_central_vector(const float* ux, const float* uy, const float* uz, f
 int edge_curl(const float* vf, float* cf, int nx, int ny, int nz, float dx, void* stream);
 /* No counterpart in the reference (it runs get_central_vector, curl, numpy's norm and
  * vtk's writer, run_edge.py:720-725, util/io_utils.py:53-75): the device part of a frame
- * output.  wn = |curl u| (nx*ny*nz floats) and sm = the nc smoke planes interleaved per
- * point (nx*ny*nz*nc floats), both in VTK point order (x fastest); nc in 0..4. */
+ * output.  wn = |curl u| (nx*ny*nz floats) and sm = the smoke's nc channels per point
+ * (nx*ny*nz*nc floats), both in VTK point order (x fastest); nc in 0..4. */
 int edge_vtk_frame(const float* ux, const float* uy, const float* uz, const float* smoke, int nc, float* wn,
                    float* sm, int nx, int ny, int nz, float dx, void* stream);
 /* split_central_vector (util/taichi_utils.py:103-116). */

@@ -26,6 +26,15 @@ def N_(t):
     return t.detach().cpu().numpy().astype(np.float64)
 
 
+def cl(a):
+    """(4, nx, ny, nz) channel planes (the oracle's smoke) -> channels per cell (the kernels')."""
+    return np.moveaxis(a, 0, -1)
+
+
+def cf(a):
+    return np.moveaxis(a, -1, 0)
+
+
 def scale_err(gpu, ref):
     ref = np.asarray(ref, np.float64)
     return float(np.max(np.abs(np.asarray(gpu, np.float64) - ref)) / max(np.max(np.abs(ref)), 1e-30))
@@ -156,9 +165,9 @@ def test_driver_maps_parity(orc32, orc64, dev):
     s0 = rng.uniform(0, 1, size=(4,) + n)
     s1 = rng.uniform(-0.5, 1.5, size=(4,) + n)
     for mode in (0, 1):
-        g = ops.map_smoke_psi(mode, T_(s0, dev), T_(s1, dev) if mode else None, tab, dx)
+        g = ops.map_smoke_psi(mode, T_(cl(s0), dev), T_(cl(s1), dev) if mode else None, tab, dx)
         r = orc32.map_smoke_psi(mode, s0, s1 if mode else None, psi, T, dT, dx)
-        assert scale_err(N_(g), r) < 1e-5
+        assert scale_err(cf(N_(g)), r) < 1e-5
     # face maps phi/F from an oracle face march, then map_u / map_smoke / clamp_smoke
     from helpers import face_coords
     phi = [face_coords(a, *n, dx) for a in range(3)]
@@ -171,9 +180,9 @@ def test_driver_maps_parity(orc32, orc64, dev):
     for a in range(3):
         assert scale_err(N_(out[a]), ref[a]) < 1e-5
     for mode in (0, 1):
-        g = ops.map_smoke(mode, T_(s0, dev), T_(s1, dev) if mode else None, gphi, dx)
+        g = ops.map_smoke(mode, T_(cl(s0), dev), T_(cl(s1), dev) if mode else None, gphi, dx)
         r = orc32.map_smoke(mode, s0, s1 if mode else None, phi, dx)
-        assert scale_err(N_(g), r) < 1e-5
+        assert scale_err(cf(N_(g)), r) < 1e-5
     # the BFECC arithmetic folded into the maps: the bits of the separate passes
     from edge_hip import _lib
     from edge_hip.field import stream_handle
@@ -187,7 +196,7 @@ def test_driver_maps_parity(orc32, orc64, dev):
         _lib.call("edge_axpy", _p(out[a]), _p(gi[a]), _p(seq), -1., seq.numel(), stream_handle())
         _lib.call("edge_scale", _p(seq), 0.5, _p(seq), seq.numel(), stream_handle())
         assert torch.equal(fused[a], seq), a
-    gs0, gs1, gs2 = T_(s0, dev), T_(s1, dev), T_(rng.uniform(-0.5, 1.5, size=(4,) + n), dev)
+    gs0, gs1, gs2 = T_(cl(s0), dev), T_(cl(s1), dev), T_(rng.uniform(-0.5, 1.5, size=n + (4,)), dev)
     for mode in (0, 1):
         sm = gs1.clone()
         _lib.call("edge_map_smoke_add_scale", mode, _p(gs0), _p(sm), _p(gs2), -1., 0.5, 4,
@@ -322,10 +331,10 @@ def test_vortex_ring_init(orc32, dev):
     vf, sm = np.zeros((3,) + n, np.float32), np.zeros((4,) + n, np.float32)
     args = (0.02, 0.2, 0.03, (0.6, 0.5, 0.5), (0, 0, -1), (0, 1, 0))
     orc32.vortex_ring(*args, X, vf, sm, (1, 0, 0), 500)
-    gv, gs = torch.zeros((3,) + n, device=dev), torch.zeros((4,) + n, device=dev)
+    gv, gs = torch.zeros((3,) + n, device=dev), torch.zeros(n + (4,), device=dev)   # smoke: channels per cell
     ops.vortex_ring(*args, T_(X, dev), gv, gs, (1, 0, 0), 500)
     assert scale_err(N_(gv), vf) < 1e-4
-    assert np.mean(N_(gs) != sm) < 1e-3  # threshold flips only at |s - 0.002| ~ rounding
+    assert np.mean(cf(N_(gs)) != sm) < 1e-3  # threshold flips only at |s - 0.002| ~ rounding
 
 
 @pytest.mark.parametrize("nvel", [0, 3, 35])
