@@ -93,7 +93,8 @@ TRAFFIC_KERNELS = {
     "k_advance_faces": (52.0 * 12664832, "face: phi, F read and written 48 + u 4 B"),
     "k_map_smoke": (68.0 * 4194304, "cell: faces of phi 36 + smoke in 16 + out 16 B"),
     "k_split_psi": (96.0 * 4194304 + 24.0 * 12664832, "cell: the garm table 96 B read once; face: phi, F out 24 B"),
-    "k_smooth8": (12.0 * 4194304, "cell: r 4 + z read 4 + z written 4 B (one half-sweep)"),
+    "k_smooth8": (6.0 * 4194304, "cell: r 2 + z read 2 + z written 2 B (one half-sweep of the mixed projection's "
+                                 "BF16 level 0)"),
 }
 
 
@@ -133,8 +134,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path comparison")
     ap.add_argument("--poisson-reps", type=int, default=3, help="0 skips the same-RHS solver comparison")
     ap.add_argument("--no-io", action="store_true", help="skip timing one frame output and one checkpoint")
-    ap.add_argument("--solver", default="mixed", choices=["plain", "mixed", "dense"],
-                    help="projection solver of the timed steps")
+    ap.add_argument("--solver", default=None, choices=["plain", "mixed", "dense"],
+                    help="projection solver of the timed steps (default: mixed; dense for the 512x256x256 cases, "
+                         "where the BF16 levels take 5 + 8 iterations against 4 + 5)")
     ap.add_argument("--zero-start", action="store_true",
                     help="start every projection from zero (the reference) instead of the same projection's "
                          "pressure of the previous step")
@@ -153,7 +155,10 @@ def parse():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="turn two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script with "
                          "--copy-probe into profiles/traffic.json and exit")
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.solver is None:
+        a.solver = "dense" if a.case in ("cfg5", "cfg5laden") else "mixed"
+    return a
 
 
 def phase_times(sim):
