@@ -71,6 +71,7 @@ enum Slot {
   S_NEU,         // all walls: recentre r (sim/mgpcg.py:211-217)
   S_EPS,         // the guard added to the alpha and beta denominators (see set_scalars)
   S_X0ACT,       // warm start, for the first level-0 pass: 0 nothing, 1 r = w (x0 kept), 2 x = 0 (x0 dropped)
+  S_APMEAN,      // mean of Ap over all cells (all walls), for the recenter folded into the CG update
   S_COUNT
 };
 
@@ -233,6 +234,7 @@ __device__ void finalize(double* sc, int fin, double s0, double s1) {
       break;
     case F_ZR_INIT:
       sc[S_X0ACT] = 0.0;   // the first level-0 pass has applied it
+      sc[S_MEAN] = 0.0;    // and recentred r (k_update_xr8's prediction starts from 0)
       sc[S_ZTR] = s0;
       sc[S_ZTR0] = s0;
       sc[S_BETA] = 0.0;
@@ -250,8 +252,9 @@ __device__ void finalize(double* sc, int fin, double s0, double s1) {
       break;
     }
     case F_ALPHA: {
-      // sim/mgpcg.py:236-239
+      // sim/mgpcg.py:236-239; s1 = sum of Ap (the update's recenter, see k_update_xr8)
       sc[S_PAP] = s0;
+      sc[S_APMEAN] = sc[S_NEU] != 0.0 ? s1 / sc[S_NCELL] : 0.0;
       double den = s0 + sc[S_EPS];
       sc[S_ALPHA] = den != 0.0 ? sc[S_ZTR] / den : 0.0;
       break;
@@ -1052,7 +1055,7 @@ __global__ void __launch_bounds__(BS) k_p_Ap8(Lv L, int bt, const S* __restrict_
                                              float* __restrict__ Ap, Red R) {
   EXIT_IF_DONE(R.sc);
   const float beta = (float)R.sc[S_BETA];
-  double s = 0;
+  double s = 0, sa = 0;
   const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
   if (t_ < (uint32_t)((int64_t)L.n0 * L.n1 * L.n2 >> 3)) {
     int i, j, k0;
@@ -1083,11 +1086,12 @@ __global__ void __launch_bounds__(BS) k_p_Ap8(Lv L, int bt, const S* __restrict_
         if (mb & NB_SELF) continue;
         out[c] = diag8(L, bt, i, j, k0 + c, mb) * pq - nsum8(L, N, i, j, k0, c, mb);
         s += (double)pq * (double)out[c];
+        sa += (double)out[c];
       }
       st8(Ap + q0, out);
     }
   }
-  block_partial(R, s, 0.0);
+  block_partial(R, s, sa);
 }
 
 // k_residual_x0 on a chunked level (no solids: every cell is fluid).
@@ -1120,14 +1124,22 @@ __global__ void __launch_bounds__(BS) k_residual_x08(Lv L, int bt, const float* 
 // p = z + beta p, x += alpha p, r -= alpha Ap, r.r and sum r (as k_update_xr).
 template <class S>
 // zn (may equal z, hence no __restrict__ on either): when not null, also the next
-// V-cycle's fill(0) + first red half-sweep from the updated residual (dense mode:
-// FP32 levels whose level-0 r is the CG residual itself, no recentering).
+// V-cycle's fill(0) + first red half-sweep from the updated residual (the
+// first level-0 pass of the next V-cycle, k_first08, is then skipped); rln,
+// when not null, the level-0 copy of r in the levels' storage type (mixed
+// mode; in dense mode r is level 0's r).  recentre: r also loses its mean, as
+// k_first08 would subtract it: the mean of r - alpha Ap is predicted as
+// mean(r) - alpha mean(Ap), both known (S_MEAN from the last update's sum,
+// S_APMEAN from k_p_Ap8's), so the next update's sum measures what rounding
+// left and the next prediction removes it too.
 __global__ void __launch_bounds__(BS) k_update_xr8(Lv L, int bt, const S* z, float* __restrict__ x,
                                                   float* __restrict__ r, float* __restrict__ p,
-                                                  const float* __restrict__ Ap, S* zn, Red R) {
+                                                  const float* __restrict__ Ap, S* zn, S* __restrict__ rln, int recentre,
+                                                  Red R) {
   EXIT_IF_DONE(R.sc);
   const float alpha = (float)R.sc[S_ALPHA];
   const float beta = (float)R.sc[S_BETA];
+  const float mean = recentre ? (float)(R.sc[S_MEAN] - R.sc[S_ALPHA] * R.sc[S_APMEAN]) : 0.f;
   double s2 = 0, s1 = 0;
   const uint32_t t_ = blockIdx.x * blockDim.x + threadIdx.x;
   if (t_ < (uint32_t)((int64_t)L.n0 * L.n1 * L.n2 >> 3)) {
@@ -1146,6 +1158,7 @@ __global__ void __launch_bounds__(BS) k_update_xr8(Lv L, int bt, const S* z, flo
         pv[c] = pn;
         xv[c] += alpha * pn;
         rv[c] -= alpha * av[c];
+        if (recentre) rv[c] -= mean;
         s2 += (double)rv[c] * (double)rv[c];
       }
       s1 += (double)rv[c];
@@ -1153,6 +1166,7 @@ __global__ void __launch_bounds__(BS) k_update_xr8(Lv L, int bt, const S* z, flo
     st8(p + q0, pv);
     st8(x + q0, xv);
     st8(r + q0, rv);
+    if (rln) st8(rln + q0, rv);
     if (zn) {
       int i, j, k0;
       chunk_ijk(L, t_, i, j, k0);
@@ -1940,7 +1954,7 @@ void Ops<S>::vcycle(Solver& s, hipStream_t st, int ig, bool z_ready) {
   S* r0 = (S*)s.rl[0];
   S* z0 = (S*)s.zl[0];
   if (z_ready) {
-    // k_update_xr8 of the previous iteration wrote z (dense mode)
+    // k_update_xr8 of the previous iteration wrote z (and, mixed, level 0's r)
   } else if (chunked(s.lv[0]))
     hipLaunchKernelGGL(k_first08<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, s.lv[0], s.bt, s.r, r0, z0, sc, ig,
                        (const float*)s.Ap, s.x);
@@ -2183,16 +2197,19 @@ int prepare_masks(Solver& s, hipStream_t st) {
 
 // The update half of a CG iteration: Ap, alpha, x, r, r.r and the stop test
 // (sim/mgpcg.py:234-259).
-// Dense mode on a chunked level 0: r needs no recentering (the dense coarsest
-// solve handles the constant), and rl[0] is r itself, so the update of r can
-// also write the next V-cycle's fill(0) + first red half-sweep, which saves
-// k_first08's pass over r and z.  Only while the dense bottom exists: without
-// it (coarsest level above DENSE_MAX fluid cells, or a factorisation that
-// failed) the red/black bottom needs the recentred residual of k_first08.
-// Not in mixed mode: with BF16 levels the unrecentred residual's drift stalls
-// the solve at 512x256x256 (|r|/|b| 0.85 after 400 iterations, against 8-9
-// iterations with the recentering of k_first08).
-inline bool z_ready(const Solver& s) { return s.mode == 2 && s.dense_ok && chunked(s.lv[0]); }
+// The update of r also writes the next V-cycle's fill(0) + first red
+// half-sweep (and, mixed, level 0's BF16 r), which saves k_first08's pass
+// over r and z, on a chunked level 0 while the dense bottom exists (without it
+// the red/black bottom needs the exactly recentred residual of k_first08):
+// * no solids, mixed or dense mode: r recentred in the update by its predicted
+//   mean (update_recentres; without any recentering the BF16 levels' residual
+//   drifts and the 512x256x256 solve stalled at |r|/|b| = 0.85);
+// * dense mode with solids: without the recenter (the dense coarsest solve
+//   handles the constant; FP32 levels whose level-0 r is the CG residual).
+inline bool update_recentres(const Solver& s) { return s.mode >= 1 && !s.solid; }
+inline bool z_ready(const Solver& s) {
+  return s.dense_ok && chunked(s.lv[0]) && (update_recentres(s) || s.mode == 2);
+}
 
 template <class S>
 void cg_update(Solver& s, hipStream_t st) {
@@ -2201,10 +2218,11 @@ void cg_update(Solver& s, hipStream_t st) {
   if (chunked(L0)) {
     hipLaunchKernelGGL(k_p_Ap8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
     finish(s, st, chunk_blocks(s.N), F_ALPHA, 1);
-    // dense mode: the next V-cycle's first red half-sweep here (see z_ready)
+    // the next V-cycle's first level-0 pass here (see z_ready)
     S* zn = z_ready(s) ? (S*)s.zl[0] : nullptr;
+    S* rln = (zn && s.rl[0] != (void*)s.r) ? (S*)s.rl[0] : nullptr;
     hipLaunchKernelGGL(k_update_xr8<S>, dim3(chunk_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.x,
-                       s.r, s.p, s.Ap, zn, R);
+                       s.r, s.p, s.Ap, zn, rln, (zn && update_recentres(s)) ? 1 : 0, R);
     finish(s, st, chunk_blocks(s.N), F_XR, 1);
   } else {
     hipLaunchKernelGGL(k_p_Ap<S>, dim3(red_blocks(s.N)), dim3(BS), 0, st, L0, s.bt, (const S*)s.zl[0], s.p, s.Ap, R);
