@@ -17,6 +17,11 @@ stream), the step's error against the oracle and the CPU baseline (the
 oracle restatement running one step of the same workload on this host's
 cores).
 
+Timing: `value` is K steps run back to back (as the step loop runs them)
+between two synchronisations, divided by K; `step_ms`, `phases_ms` and the
+per-launch kernel times come from K further steps, each synchronised and
+carrying HIP events, which add a little host time per step.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--res NX NY NZ]
 """
 import argparse
@@ -170,8 +175,20 @@ def phase_times(sim):
     return out
 
 
+def timed_steps(sim, k, torch):
+    """k steps back to back between two synchronisations, nothing recorded in
+    between (as the step loop runs them); returns ms per step."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        sim.substep()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / k
+
+
 def run_steps(sim, k, torch):
-    """k timed steps; returns wall ms per step (list) and phase ms (list of dicts)."""
+    """k instrumented steps, each synchronised on both sides, with HIP events at
+    the phase boundaries; returns wall ms per step (list) and phase ms (list of dicts)."""
     walls, phases = [], []
     for _ in range(k):
         sim.timing = {}
@@ -371,24 +388,25 @@ def main():
     freeze_setup_objects()   # as Simulation.run does before its loop
     for _ in range(a.warmup):
         sim.substep()
+    # the timed steps: back to back, as Simulation.run steps (the projection's
+    # convergence checks and the CFL readback are the only host waits)
+    ms_step = timed_steps(sim, a.steps, torch)
+    pstats = sim.poisson_stats[-2 * a.steps:]
+    step_iters = [s["iters"] for s in pstats]
+    # then as many instrumented steps: per-step walls, phases and the HIP-event
+    # times of the roofline kernels' launches (events and synchronisations add
+    # a little host time that the timed steps do not carry)
     from edge_hip import timing
     timing.reset()
     timing.enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
     i_first = sim.i + 1
     walls, phases = run_steps(sim, a.steps, torch)
-    torch.cuda.synchronize()
-    total_ms = (time.perf_counter() - t0) * 1e3
     timing.enable(False)
     march_kernel = {"edge": "k_march_edge", "ed8": "k_march_ed8", "ed4": "k_march_ed4", "ge": "k_march_ge",
                     "efm": "k_backtrack_faces"}[a.scheme]
     march_launch_ms = timing.durations(march_kernel)
     other_launch_ms = {k: timing.durations(k) for k in ("k_map_u", "k_advect_sl", "k_advance_faces")}
-    ms_step = total_ms / a.steps
     ph = {k: statistics.median(p.get(k, 0.0) for p in phases) for k in phases[0]}
-    pstats = sim.poisson_stats[-2 * a.steps:]
-    step_iters = [s["iters"] for s in pstats]
 
     plain = None
     if not a.no_plain:
@@ -396,10 +414,12 @@ def main():
         step_solver, reuse = sim.solver, sim.reuse_pressure
         sim.solver, sim.reuse_pressure = make_solver(sim, "plain"), False
         run_steps(sim, 1, torch)  # warm the plain solver
+        pms = timed_steps(sim, a.steps, torch)
+        plain_iters = [s["iters"] for s in sim.poisson_stats[-2 * a.steps:]]
         pw, pph = run_steps(sim, a.steps, torch)
-        plain = dict(ms_per_step=round(statistics.mean(pw), 3), step_ms=[round(w, 3) for w in pw],
+        plain = dict(ms_per_step=round(pms, 3), step_ms=[round(w, 3) for w in pw],
                      phases_ms={k: round(statistics.median(p.get(k, 0.0) for p in pph), 3) for k in pph[0]},
-                     poisson_iters=[s["iters"] for s in sim.poisson_stats[-2 * a.steps:]],
+                     poisson_iters=plain_iters,
                      projection_start="zero")
         sim.solver, sim.reuse_pressure = step_solver, reuse
 
@@ -515,6 +535,8 @@ def main():
                    "projection_start": "zero" if a.zero_start else "the same projection's pressure of the previous step",
                    **({"particles": sim.laden_particle_num} if sim.lp is not None else {}),
                    **({"dye_particles": sim.dye_num} if sim.dye is not None else {})},
+        "timing": "value: the K timed steps back to back between two synchronisations; step_ms, phases_ms and "
+                  "the kernels' launch times: K further steps, each synchronised, with HIP events",
         "phases_ms": {k: round(v, 3) for k, v in ph.items()},
         "step_ms": [round(w, 3) for w in walls],
         "poisson_iters": step_iters,
