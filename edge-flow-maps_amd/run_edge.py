@@ -328,16 +328,21 @@ class Simulation:
         previous step (kept only when ||b - A x0|| <= ||b||, same stop rule)."""
         if self.rhs_snapshots is not None:
             self.rhs_snapshots.append([self.u_x.data.clone(), self.u_y.data.clone(), self.u_z.data.clone()])
-        x0 = self._p_prev[slot] if (self.reuse_pressure and self._p_prev[slot] is not None) else None
+        x0 = None
+        if self.reuse_pressure:
+            # each projection keeps its own pressure buffer: the solver's p is pointed
+            # at it, so the solve reads the previous pressure from it (x0) at its start
+            # and writes the new one into it at its end, with no copy in between
+            if self._p_prev[slot] is None:
+                self._p_prev[slot] = torch.zeros_like(self.solver.p.data)
+            else:
+                x0 = self._p_prev[slot]
+            self.solver.p.data = self._p_prev[slot]
         if buffers is None:
             self.solver.Poisson(self.u_x, self.u_y, self.u_z, max_iters=self.poisson_max_iters, x0=x0)
         else:
             self.solver.Poisson_with_buffer(self.u_x, self.u_y, self.u_z, *buffers, max_iters=self.poisson_max_iters,
                                             x0=x0)
-        if self.reuse_pressure:
-            if self._p_prev[slot] is None:
-                self._p_prev[slot] = torch.empty_like(self.solver.p.data)
-            self._p_prev[slot].copy_(self.solver.p.data)
         self.poisson_stats.append(dict(self.solver.stats, time=self.solver.last_time))
 
     # -- one step (run_edge.py:588-711) --------------------------------------------

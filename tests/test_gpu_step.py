@@ -383,3 +383,36 @@ def test_dye_steps_fixed_iterations(dev):
     moved = float(np.abs(ref.dye - pos0).max()) / sim.dx
     print(f"dye 32x16x16: position error {err:.3g} of the domain, largest move {moved:.3g} cells")
     assert err < 1e-6 and moved > 0.1
+
+
+@pytest.mark.parametrize("mode", ["mixed", "dense"])
+def test_previous_pressure_start(dev, mode):
+    """Simulation(reuse_pressure=True): each projection starts from the same
+    projection's pressure of the previous step (bench.py's default).  Four steps
+    against the zero start: the same velocities within the stop rule's reach
+    (both solves end at ||r|| <= 1e-6 ||b||), the guess kept from the second
+    step on, no more than one extra iteration in total (the counts are
+    printed), and the solver's p is the last
+    solve's pressure (it lives in the projection's own buffer)."""
+    from run_edge import Simulation
+    out = {}
+    for reuse in (False, True):
+        sim = Simulation("cfg1", solver_mode=mode, res=(64, 32, 32), num_samples=300, check_every=0,
+                         reuse_pressure=reuse)
+        sim.initialize()
+        for _ in range(4):
+            sim.substep()
+        torch.cuda.synchronize()
+        out[reuse] = (sim.velocity_numpy(), [dict(s) for s in sim.poisson_stats[-8:]], sim)
+    (u0, st0, _), (u1, st1, sim1) = out[False], out[True]
+    for a, (g, r) in enumerate(zip(u1, u0)):
+        e = scale_err(g, r)
+        print(f"{mode} u_{'xyz'[a]}: previous-pressure start vs zero start {e:.3g}")
+        assert e < 1e-4
+    for s in st0 + st1:
+        assert s["converged"] == 1 and s["rTr"] <= 1e-12 * s["b2"] * (1 + 1e-9), s
+    assert all(s["x0_used"] == 1 for s in st1[2:])
+    it0, it1 = sum(s["iters"] for s in st0), sum(s["iters"] for s in st1)
+    print(f"{mode}: iterations over four steps, zero start {it0}, previous pressure {it1}")
+    assert it1 <= it0 + 1   # a kept guess has ||b - A x0|| <= ||b||: at most one iteration more
+    assert sim1.solver.p.data.data_ptr() == sim1._p_prev[1].data_ptr()
