@@ -416,7 +416,16 @@ EDGE_DEV float hermite_eval_dir(const Grid& g, Rsrc tab, const HPoint& P, float&
       for (int k = 0; k < 2; ++k) {
         const int cr[3] = {i, j, k};
         if ((cr[A1] == 1 && z1) || (cr[A2] == 1 && z2)) continue;
-        const float4 lo = bload4(tab, vb + 32u * k, so), hi = bload4(tab, vb + 32u * k + 16u, so);
+        const float w1f = ax[A1]->F[cr[A1]], w1g = ax[A1]->G[cr[A1]];
+        const float w2f = ax[A2]->F[cr[A2]], w2g = ax[A2]->G[cr[A2]];
+        // the upper four coefficients (mixed derivatives) all carry a derivative
+        // along A1 or A2: with both G weights exactly zero (a face centre: the
+        // point sits on the corner's A1 and A2 coordinates) they add +0, and
+        // their 16 bytes are not fetched (the same value, half the bytes)
+        const bool hi_needed = w1g != 0.f || w2g != 0.f;
+        const float4 lo = bload4(tab, vb + 32u * k, so);
+        float4 hi = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (hi_needed) hi = bload4(tab, vb + 32u * k + 16u, so);
         float c[2][2][2];
         c[0][0][0] = lo.x;
         c[1][0][0] = lo.y;
@@ -433,8 +442,6 @@ EDGE_DEV float hermite_eval_dir(const Grid& g, Rsrc tab, const HPoint& P, float&
           t[D] = td;
           return c[t[0]][t[1]][t[2]];
         };
-        const float w1f = ax[A1]->F[cr[A1]], w1g = ax[A1]->G[cr[A1]];
-        const float w2f = ax[A2]->F[cr[A2]], w2g = ax[A2]->G[cr[A2]];
         float u[2];
 #pragma unroll
         for (int td = 0; td < 2; ++td) {
