@@ -31,6 +31,7 @@ inline Grid make_grid(int nx, int ny, int nz, float dx) {
   g.ny = ny;
   g.nz = nz;
   g.dx = dx;
+  g.idx = 1.0f / dx;
   g.cz = make_fdiv((uint32_t)nz);
   g.cy = make_fdiv((uint32_t)ny);
   for (int a = 0; a < 3; ++a) {

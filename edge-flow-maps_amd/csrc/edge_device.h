@@ -72,6 +72,8 @@ EDGE_DEV uint32_t fdiv(uint32_t q, const FDiv& f) { return (__umulhi(q, f.m) + q
 struct Grid {
   int nx, ny, nz;   // cells
   float dx;
+  float idx;        // 1/dx (f32): positions enter the index space as p * idx, as the
+                    // reference's fast-math `p / dx` compiles (no IEEE division per lookup)
   FDiv cz, cy;          // division by nz, ny (cell index decode)
   FDiv fz[3], fy[3];    // division by the k and j extents of face grid a
   uint32_t nf[3];       // face counts
@@ -229,7 +231,7 @@ struct Mac {
 // face grid with BL = 0 on axis a and 0.5 elsewhere.  J rows = grad u_a (1/dx units).
 template <bool GRAD>
 EDGE_DEV void interp_u_mac(const Grid& g, const Mac& U, const float p[3], float out[3], float J[3][3]) {
-  float q0 = p[0] / g.dx, q1 = p[1] / g.dx, q2 = p[2] / g.dx;
+  float q0 = p[0] * g.idx, q1 = p[1] * g.idx, q2 = p[2] * g.idx;
   // own (BL 0, staggered dims) and other (BL 0.5, cell dims) weights per axis
   Axis3 ow0 = bspline_axis(q0, g.hi_f[0]), ot0 = bspline_axis(q0 - 0.5f, g.hi_c[0]);
   Axis3 ow1 = bspline_axis(q1, g.hi_f[1]), ot1 = bspline_axis(q1 - 0.5f, g.hi_c[1]);
@@ -237,16 +239,16 @@ EDGE_DEV void interp_u_mac(const Grid& g, const Mac& U, const float p[3], float 
   float gr[3];
   const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
   out[0] = bspline_eval<GRAD>(make_rsrc(U.ux, 4u * (nx + 1) * ny * nz), ny * nz, nz, ow0, ot1, ot2, gr);
-  if (GRAD) { float inv = 1.0f / g.dx; J[0][0] = gr[0] * inv; J[0][1] = gr[1] * inv; J[0][2] = gr[2] * inv; }
+  if (GRAD) { float inv = g.idx; J[0][0] = gr[0] * inv; J[0][1] = gr[1] * inv; J[0][2] = gr[2] * inv; }
   out[1] = bspline_eval<GRAD>(make_rsrc(U.uy, 4u * nx * (ny + 1) * nz), (ny + 1) * nz, nz, ot0, ow1, ot2, gr);
-  if (GRAD) { float inv = 1.0f / g.dx; J[1][0] = gr[0] * inv; J[1][1] = gr[1] * inv; J[1][2] = gr[2] * inv; }
+  if (GRAD) { float inv = g.idx; J[1][0] = gr[0] * inv; J[1][1] = gr[1] * inv; J[1][2] = gr[2] * inv; }
   out[2] = bspline_eval<GRAD>(make_rsrc(U.uz, 4u * nx * ny * (nz + 1)), ny * (nz + 1), nz + 1, ot0, ot1, ow2, gr);
-  if (GRAD) { float inv = 1.0f / g.dx; J[2][0] = gr[0] * inv; J[2][1] = gr[1] * inv; J[2][2] = gr[2] * inv; }
+  if (GRAD) { float inv = g.idx; J[2][0] = gr[0] * inv; J[2][1] = gr[1] * inv; J[2][2] = gr[2] * inv; }
 }
 
 // Component a of interp_u_mac<false> alone (same weights and taps).
 EDGE_DEV float interp_u_comp(const Grid& g, const Mac& U, const float p[3], int a) {
-  const float q0 = p[0] / g.dx, q1 = p[1] / g.dx, q2 = p[2] / g.dx;
+  const float q0 = p[0] * g.idx, q1 = p[1] * g.idx, q2 = p[2] * g.idx;
   const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
   if (a == 0)
     return bspline_eval<false>(make_rsrc(U.ux, 4u * (nx + 1) * ny * nz), ny * nz, nz, bspline_axis(q0, g.hi_f[0]),
@@ -299,9 +301,9 @@ struct HPoint {
 
 EDGE_DEV HPoint hermite_point(const Grid& g, const float p[3]) {
   HPoint P;
-  P.X = hermite_axis(p[0] / g.dx - 0.5f, g.hi_c[0], g.dx);
-  P.Y = hermite_axis(p[1] / g.dx - 0.5f, g.hi_c[1], g.dx);
-  P.Z = hermite_axis(p[2] / g.dx - 0.5f, g.hi_c[2], g.dx);
+  P.X = hermite_axis(p[0] * g.idx - 0.5f, g.hi_c[0], g.dx);
+  P.Y = hermite_axis(p[1] * g.idx - 0.5f, g.hi_c[1], g.dx);
+  P.Z = hermite_axis(p[2] * g.idx - 0.5f, g.hi_c[2], g.dx);
   return P;
 }
 
@@ -464,7 +466,7 @@ EDGE_DEV void map_eval_dir(const Grid& g, const float4* __restrict__ tab, const 
                            float gcol[3]) {
   HPoint P = hermite_point(g, p);
   int64_t N = ncell(g);
-  float inv = 1.0f / g.dx;
+  float inv = g.idx;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float d;
@@ -480,7 +482,7 @@ template <bool GRAD, bool FENCE = false>
 EDGE_DEV void map_eval(const Grid& g, const float4* __restrict__ tab, const float p[3], float v[3], float G[3][3]) {
   HPoint P = hermite_point(g, p);
   int64_t N = ncell(g);
-  float inv = 1.0f / g.dx;
+  float inv = g.idx;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float gr[3];

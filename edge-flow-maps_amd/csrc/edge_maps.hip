@@ -153,7 +153,7 @@ __device__ __forceinline__ void linear_taps(const Grid& g, const float* __restri
   const int dims[3] = {g.nx, g.ny, g.nz};
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    float u = p[d] / g.dx - 0.5f;
+    float u = p[d] * g.idx - 0.5f;
     ss[d] = fmaxf(1.0f, fminf(u, g.hi1_c[d]));
     float fl = floorf(ss[d]);
     fr[d] = ss[d] - fl;

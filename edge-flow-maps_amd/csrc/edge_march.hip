@@ -418,13 +418,13 @@ __global__ void __launch_bounds__(256) k_eval_points(Grid g, int what, const flo
   if (t >= n) return;
   float p[3] = {pts[3 * t], pts[3 * t + 1], pts[3 * t + 2]};
   if (what == 0) {
-    Axis3 A = bspline_axis(p[0] / g.dx - bl0, (float)((double)d0 - 2.0 - 1e-6));
-    Axis3 B = bspline_axis(p[1] / g.dx - bl1, (float)((double)d1 - 2.0 - 1e-6));
-    Axis3 C = bspline_axis(p[2] / g.dx - bl2, (float)((double)d2 - 2.0 - 1e-6));
+    Axis3 A = bspline_axis(p[0] * g.idx - bl0, (float)((double)d0 - 2.0 - 1e-6));
+    Axis3 B = bspline_axis(p[1] * g.idx - bl1, (float)((double)d1 - 2.0 - 1e-6));
+    Axis3 C = bspline_axis(p[2] * g.idx - bl2, (float)((double)d2 - 2.0 - 1e-6));
     float gr[3];
     if (grad) {
       val[t] = bspline_eval<true>(make_rsrc(f, 4u * d0 * d1 * d2), (uint32_t)d1 * d2, d2, A, B, C, gr);
-      float inv = 1.0f / g.dx;
+      float inv = g.idx;
       grad[3 * t] = gr[0] * inv;
       grad[3 * t + 1] = gr[1] * inv;
       grad[3 * t + 2] = gr[2] * inv;

@@ -137,7 +137,12 @@ def test_edge_march_parity(orc32, orc64, dev):
         torch.empty((3, 3) + n, device=dev)
     ops.march_edge([T_(c, dev) for c in u], tab, T_(X, dev), gp, gT, gdT, dx, dt, eps)
     assert scale_err(N_(gp), r32[0]) < 1e-6
-    assert np.max(np.abs(N_(gT) - r32[1])) < 2e-5
+    # T: the FP32 oracle is itself ~2e-5 from FP64 here (dx = 1/24: p / dx and the
+    # kernels' p * (1/dx) round differently); GPU against FP64 with that yardstick
+    eT_orc = np.max(np.abs(r32[1] - r64[1]))
+    eT_gpu = np.max(np.abs(N_(gT) - r64[1]))
+    print(f"march T: GPU vs FP64 {eT_gpu:.3g}, FP32 oracle vs FP64 {eT_orc:.3g}")
+    assert eT_gpu <= max(2e-5, 3 * eT_orc), (eT_gpu, eT_orc)
     # dT is an eps-difference: compare against FP64 with the FP32 oracle's own error as the yardstick
     e_orc = np.max(np.abs(r32[2] - r64[2]))
     e_gpu = np.max(np.abs(N_(gdT) - r64[2]))
