@@ -106,6 +106,49 @@ def test_full_size_mixed_steps_are_deterministic_and_converged(dev):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.timeout(300)
+def test_full_size_thirty_steps_invariants(dev):
+    """The bench's workload (256x128x128, four rings, mixed projection from the
+    previous step's pressure) for 30 steps, across two map reinitialisations
+    (every 12 steps): every projection converged (||r|| <= 1e-6 ||b||), the
+    velocity divergence after the step's last projection at the solver's
+    level (at most 1e-5 ||b||), the kinetic energy within 10% of the start (the reference scheme
+    is energy-preserving up to its BFECC clamp and the projections), the
+    smoke inside the initial range (the BFECC clamp), the map inside the
+    domain, and a dt within the CFL bound every step."""
+    from run_edge import Simulation
+    from test_gpu_step import _fluid_div
+    sim = Simulation("cfg2", solver_mode="mixed", res=RES, num_samples=400, check_every=0, reuse_pressure=True)
+    sim.initialize()
+    e0 = sum(float((f.data.double() ** 2).sum()) for f in (sim.u_x, sim.u_y, sim.u_z))
+    s_lo, s_hi = float(sim.smoke.data.min()), float(sim.smoke.data.max())
+    L = torch.tensor([RES[0], RES[1], RES[2]], device=sim.psi_garm.data.device) * sim.dx
+    dts = []
+    for step in range(30):
+        dts.append(sim.substep())
+        st = sim.poisson_stats[-1]
+        assert st["converged"] == 1 and st["rTr"] <= 1e-12 * st["b2"] * (1 + 1e-9), (step, st)
+    torch.cuda.synchronize()
+    assert sim.i == 29 and sim.i // sim.reinit_every == 2   # two reinitialisations passed
+    for st in sim.poisson_stats[-60:]:
+        assert st["converged"] == 1, st
+    e1 = sum(float((f.data.double() ** 2).sum()) for f in (sim.u_x, sim.u_y, sim.u_z))
+    print(f"30 steps: kinetic energy ratio {e1 / e0:.4f}, dt {min(dts):.4g} .. {max(dts):.4g}, "
+          f"iterations {[s['iters'] for s in sim.poisson_stats[-6:]]}")
+    assert abs(e1 / e0 - 1) < 0.10
+    # div u = -(b - A p) after the subtraction: its largest cell is at most the final
+    # residual's L2 norm, <= 1e-6 ||b|| (10x for the FP32 rounding of u and p)
+    div = _fluid_div(sim.velocity_numpy(), np.zeros(RES, np.int32))
+    b_norm = sim.poisson_stats[-1]["b2"] ** 0.5
+    print(f"largest divergence {div:.3g}, ||b|| of the last projection {b_norm:.3g}")
+    assert div <= 1e-5 * b_norm, (div, b_norm)
+    sm = sim.smoke.data
+    assert float(sm.min()) >= s_lo - 1e-6 and float(sm.max()) <= s_hi + 1e-6
+    psi = sim.psi_garm.data
+    assert bool(((psi >= 0) & (psi <= L.view(3, 1, 1, 1))).all())
+    assert all(0 < d <= sim.CFL * sim.dx / 1e-3 for d in dts)
+
+
 def test_full_size_plate_steps_converge_divergence_free(dev):
     """The triangular plate workload at 256x128x128 (four rings, inflow 1 at
     20 degrees, span 0.3), mixed projection: two steps, every projection at
