@@ -30,6 +30,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <vector>
 
@@ -356,8 +357,11 @@ struct Faces3 {
 // apply_bc: box walls zeroed, then the faces of solid cells set to boundary_vel;
 // where two solid cells share a face the higher-index cell's value stands (the
 // result of a sequential loop over cells).
+// done_guard (when not null): skip unless the solve has stopped (S_DONE set),
+// for the post-solve launches enqueued before the host has seen the flag.
 __global__ void __launch_bounds__(BS) k_apply_bc(Grid g, int bt, Faces3 U, const int* __restrict__ bm,
-                                                const float* __restrict__ bvel) {
+                                                const float* __restrict__ bvel, const double* __restrict__ done_guard) {
+  if (done_guard && done_guard[S_DONE] == 0.0) return;
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int a, i, j, k;
   int64_t l, nfa;
@@ -508,7 +512,8 @@ __global__ void __launch_bounds__(BS) k_apply_x0(int64_t N, float* __restrict__ 
 // walls zeroed) is folded in here: a face on a wall gets 0, as apply_bc after
 // the subtraction would leave it; the force buffer is -grad p everywhere.
 __global__ void __launch_bounds__(BS) k_subtract_grad(Grid g, Faces3 U, Faces3 F, const float* __restrict__ P,
-                                                     int walls) {
+                                                     int walls, const double* __restrict__ done_guard) {
+  if (done_guard && done_guard[S_DONE] == 0.0) return;
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int a, i, j, k;
   int64_t l, nfa;
@@ -538,7 +543,8 @@ __global__ void __launch_bounds__(BS) k_subtract_grad(Grid g, Faces3 U, Faces3 F
 // values): aligned float4 runs along k (nz % 4 == 0, 16-byte aligned fields);
 // z faces (rows of nz + 1, not 16-byte aligned) one per thread.
 __global__ void __launch_bounds__(BS) k_subtract_grad4(Grid g, Faces3 U, Faces3 F, const float* __restrict__ P,
-                                                      int walls) {
+                                                      int walls, const double* __restrict__ done_guard) {
+  if (done_guard && done_guard[S_DONE] == 0.0) return;
   const uint32_t nx = g.nx, ny = g.ny, nz = g.nz, q4 = nz >> 2;
   const uint32_t n0 = (nx + 1) * ny * q4, n1 = nx * (ny + 1) * q4;
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -582,6 +588,14 @@ __global__ void __launch_bounds__(BS) k_subtract_grad4(Grid g, Faces3 U, Faces3 
   else
     U.f[2][t] -= (pr - pl);
   if (F.f[2]) F.f[2][t] = -(pr - pl);
+}
+
+// dst = src (n floats), skipped unless the solve has stopped (see k_apply_bc)
+__global__ void __launch_bounds__(BS) k_copy_done(int64_t n, const float* __restrict__ src, float* __restrict__ dst,
+                                                 const double* __restrict__ done_guard) {
+  if (done_guard && done_guard[S_DONE] == 0.0) return;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    dst[q] = src[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1828,7 +1842,13 @@ struct Solver {
   int tail_from = 0;
   // halo-tile legs for the levels between the finest and the tail/bottom
   int use_tiles = 1;
+  // marks the stop flag's copy in a batch's stream (run_cg's post-solve launches)
+  hipEvent_t ev_check = nullptr;
 };
+
+// Launches that follow a solve (run_cg): guard = the scalars (skip unless the
+// solve has stopped) or NULL (unconditional).
+using PostSolve = std::function<void(hipStream_t, const double*)>;
 
 // First level of the one-workgroup tail for the current setup (0 = no tail):
 // the first level >= 1 small enough for LDS, used around a dense bottom (the
@@ -2138,6 +2158,7 @@ void free_solver(Solver* s) {
                   (void*)s->sc, (void*)s->Binv, (void*)s->fidx})
     if (p) (void)hipFree(p);
   if (s->sc_host) (void)hipHostFree(s->sc_host);
+  if (s->ev_check) (void)hipEventDestroy(s->ev_check);
   drop_graph(*s);
   if (s->cap) (void)hipStreamDestroy(s->cap);
   delete s;
@@ -2293,7 +2314,7 @@ int next_batch(const Solver& s, int check_every, int done_iters) {
 }
 
 template <class S>
-int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* stats) {
+int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* stats, const PostSolve* post) {
   int it = 0;
   int done = 0;
   const int maxit = max_iters < 0 ? (1 << 30) : max_iters;
@@ -2326,8 +2347,24 @@ int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* st
     }
     it += batch;
     EDGE_CHECK_HIP(hipMemcpyAsync(s.sc_host, s.sc, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st));
-    EDGE_CHECK_HIP(hipStreamSynchronize(st));
+    if (post) {
+      // the post-solve work goes in behind the flag's copy, guarded by the
+      // flag, so the GPU runs it while the host turns the flag around
+      if (!s.ev_check) EDGE_CHECK_HIP(hipEventCreateWithFlags(&s.ev_check, hipEventDisableTiming));
+      EDGE_CHECK_HIP(hipEventRecord(s.ev_check, st));
+      (*post)(st, s.sc);
+      EDGE_CHECK_LAUNCH();
+      EDGE_CHECK_HIP(hipEventSynchronize(s.ev_check));
+    } else {
+      EDGE_CHECK_HIP(hipStreamSynchronize(st));
+    }
     done = s.sc_host[S_DONE] != 0.0 || it >= maxit;
+  }
+  // stopped on the host's count with the device flag unset (not expected: the
+  // device stops at the same count): the guarded launches skipped, run them now
+  if (post && s.sc_host[S_DONE] == 0.0) {
+    (*post)(st, nullptr);
+    EDGE_CHECK_LAUNCH();
   }
   for (int q = 3; q > 0; --q) s.hist[q] = s.hist[q - 1];
   s.hist[0] = (int)s.sc_host[S_ITERS];
@@ -2345,9 +2382,10 @@ int run_cg(Solver& s, hipStream_t st, int max_iters, int check_every, double* st
   return s.sc_host[S_DONE] == 1.0 ? EDGE_OK : EDGE_ENOCONV;
 }
 
-int run_cg_any(Solver& s, hipStream_t st, int max_iters, int check_every, double* stats) {
-  if (s.mode == 1) return run_cg<__hip_bfloat16>(s, st, max_iters, check_every, stats);
-  return run_cg<float>(s, st, max_iters, check_every, stats);
+int run_cg_any(Solver& s, hipStream_t st, int max_iters, int check_every, double* stats,
+               const PostSolve* post = nullptr) {
+  if (s.mode == 1) return run_cg<__hip_bfloat16>(s, st, max_iters, check_every, stats, post);
+  return run_cg<float>(s, st, max_iters, check_every, stats, post);
 }
 
 // Mixed and dense modes without solids: the recenter of r is left to the
@@ -2573,7 +2611,7 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   const int walls = s.solid ? 0 : s.bt;
   if (s.solid)
     hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), s.bt,
-                       U, s.user_bm, s.user_bvel);
+                       U, s.user_bm, s.user_bvel, nullptr);
   if (s.solid) {
     hipLaunchKernelGGL(k_div_sum, dim3(red_blocks(s.N)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), U, (const int8_t*)s.bm[0], red(s));
     finish(s, st, red_blocks(s.N), F_DIVSUM, 0);
@@ -2584,25 +2622,34 @@ int edge_poisson(void* handle, float* ux, float* uy, float* uz, float* p_out, fl
   EDGE_CHECK_LAUNCH();
   rc = start_solve(s, st, x0, fold_recenter(s));
   if (rc) return rc;
-  int cg = run_cg_any(s, st, max_iters, check_every, stats);
+  // get_result + subtract_grad_p + apply_bc, and p out.  Enqueued by run_cg
+  // after every batch of iterations, before the host waits for the batch's
+  // stop flag, each launch guarded by that flag (guard = s.sc): the batch that
+  // ends the solve is followed by them at once, the others by launches that
+  // return at their first instruction.  guard = NULL: unconditional.
+  auto post = [&](hipStream_t pst, const double* guard) {
+    auto al16 = [](const void* q) { return ((uintptr_t)q & 15u) == 0; };
+    if (s.nz % 4 == 0 && al16(U.f[0]) && al16(U.f[1]) && (!fx || al16(fx)) && (!fy || al16(fy)) &&
+        (int64_t)s.nx * s.ny * (s.nz + 4) < (int64_t)1 << 31) {
+      const int64_t quads = ((int64_t)(s.nx + 1) * s.ny + (int64_t)s.nx * (s.ny + 1)) * (s.nz / 4) +
+                            (int64_t)s.nx * s.ny * (s.nz + 1);
+      hipLaunchKernelGGL(k_subtract_grad4, dim3(blocks_for(quads, BS)), dim3(BS), 0, pst,
+                         make_grid(s.nx, s.ny, s.nz, 1.f), U, Faces3{{fx, fy, fz}}, (const float*)s.x, walls, guard);
+    } else {
+      hipLaunchKernelGGL(k_subtract_grad, dim3(blocks_for(nf, BS)), dim3(BS), 0, pst, make_grid(s.nx, s.ny, s.nz, 1.f),
+                         U, Faces3{{fx, fy, fz}}, (const float*)s.x, walls, guard);
+    }
+    if (s.solid)
+      hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, pst, make_grid(s.nx, s.ny, s.nz, 1.f),
+                         s.bt, U, s.user_bm, s.user_bvel, guard);
+    if (p_out)
+      hipLaunchKernelGGL(k_copy_done, dim3(blocks_for(s.N / 4 + 1, BS)), dim3(BS), 0, pst, s.N, (const float*)s.x,
+                         p_out, guard);
+  };
+  PostSolve ps = post;
+  int cg = run_cg_any(s, st, max_iters, check_every, stats, &ps);
   if (cg != EDGE_OK && cg != EDGE_ENOCONV) return cg;
-  // get_result + subtract_grad_p + apply_bc
-  auto al16 = [](const void* q) { return ((uintptr_t)q & 15u) == 0; };
-  if (s.nz % 4 == 0 && al16(U.f[0]) && al16(U.f[1]) && (!fx || al16(fx)) && (!fy || al16(fy)) &&
-      (int64_t)s.nx * s.ny * (s.nz + 4) < (int64_t)1 << 31) {
-    const int64_t quads = ((int64_t)(s.nx + 1) * s.ny + (int64_t)s.nx * (s.ny + 1)) * (s.nz / 4) +
-                          (int64_t)s.nx * s.ny * (s.nz + 1);
-    hipLaunchKernelGGL(k_subtract_grad4, dim3(blocks_for(quads, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f),
-                       U, Faces3{{fx, fy, fz}}, (const float*)s.x, walls);
-  } else {
-    hipLaunchKernelGGL(k_subtract_grad, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f),
-                       U, Faces3{{fx, fy, fz}}, (const float*)s.x, walls);
-  }
-  if (s.solid)
-    hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, make_grid(s.nx, s.ny, s.nz, 1.f), s.bt,
-                       U, s.user_bm, s.user_bvel);
   EDGE_CHECK_LAUNCH();
-  if (p_out) EDGE_CHECK_HIP(hipMemcpyAsync(p_out, s.x, s.N * 4, hipMemcpyDeviceToDevice, st));
   if (cg == EDGE_ENOCONV) set_error("edge_poisson: no convergence within %d iterations", max_iters);
   return cg;
 }
@@ -2623,7 +2670,7 @@ int edge_mg_rhs(void* handle, float* ux, float* uy, float* uz, float* b, void* s
   if (rc) return rc;
   const Grid g = make_grid(s.nx, s.ny, s.nz, 1.f);
   hipLaunchKernelGGL(k_apply_bc, dim3(blocks_for(nf, BS)), dim3(BS), 0, st, g, s.bt, U, s.solid ? s.user_bm : nullptr,
-                     s.solid ? s.user_bvel : nullptr);
+                     s.solid ? s.user_bvel : nullptr, nullptr);
   if (s.solid) {
     hipLaunchKernelGGL(k_div_sum, dim3(red_blocks(s.N)), dim3(BS), 0, st, g, U, (const int8_t*)s.bm[0], red(s));
     finish(s, st, red_blocks(s.N), F_DIVSUM, 0);
