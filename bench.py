@@ -142,9 +142,11 @@ def parse():
     ap.add_argument("--solver", default=None, choices=["plain", "mixed", "dense"],
                     help="projection solver of the timed steps (default: mixed; dense for the 512x256x256 cases, "
                          "where the BF16 levels take 5 + 8 iterations against 4 + 5)")
-    ap.add_argument("--zero-start", action="store_true",
-                    help="start every projection from zero (the reference) instead of the same projection's "
-                         "pressure of the previous step")
+    ap.add_argument("--start", default="extrapolated", choices=["zero", "previous", "extrapolated"],
+                    help="where each projection's solve starts: zero (the reference), the same projection's "
+                         "pressure of the previous step, or (default) the linear extrapolation in time of that "
+                         "projection's last two pressures")
+    ap.add_argument("--zero-start", action="store_true", help="the same as --start zero")
     ap.add_argument("--case", default="cfg2", choices=["cfg2", "cfg3", "cfg3plate", "cfg5", "cfg5laden"],
                     help="cfg2: the headline EDGE step; cfg3: the reference's delta-wing plate with the solid "
                          "projection; cfg3plate: the four rings plus a plate of span 0.3 in an inflow of speed 1 at "
@@ -362,6 +364,14 @@ def cpu_baseline(sim):
     return base, err
 
 
+START_NOTE = {
+    "zero": "zero (the reference)",
+    "previous": "the same projection's pressure of the previous step",
+    "extrapolated": "the linear extrapolation in time of the same projection's pressures of the previous two "
+                    "steps (the previous pressure alone after a dropped guess)",
+}
+
+
 def main():
     a = parse()
     if a.summarize_pmc:
@@ -382,7 +392,7 @@ def main():
     big = a.case in ("cfg5", "cfg5laden")
     res = tuple(a.res) if (not big or a.res != [256, 128, 128]) else (512, 256, 256)
     sim = Simulation(a.case, solver_mode=a.solver, stop="relative", rel_tol=1e-6, res=res, check_every=0,
-                     reuse_pressure=not a.zero_start,
+                     pressure_start="zero" if a.zero_start else a.start,
                      num_particles=a.particles, scheme=a.scheme)
     sim.initialize()
     freeze_setup_objects()   # as Simulation.run does before its loop
@@ -532,7 +542,7 @@ def main():
         "config": {"workload": wl, "cells": ncell, "reinit_every": sim.reinit_every,
                    "poisson_stop": "||r|| <= 1e-6 ||b||",
                    "solver": SOLVER_NOTE[a.solver],
-                   "projection_start": "zero" if a.zero_start else "the same projection's pressure of the previous step",
+                   "projection_start": START_NOTE[sim.pressure_start],
                    **({"particles": sim.laden_particle_num} if sim.lp is not None else {}),
                    **({"dye_particles": sim.dye_num} if sim.dye is not None else {})},
         "timing": "value: the K timed steps back to back between two synchronisations; step_ms, phases_ms and "

@@ -138,16 +138,20 @@ class Simulation:
 
     def __init__(self, case="cfg1", solver_mode="plain", stop="relative", rel_tol=1e-6, res=None,
                  reinit_every=None, check_every=None, num_samples=2000, seed=0, num_particles=None, scheme="edge",
-                 reuse_pressure=False):
+                 reuse_pressure=False, pressure_start=None):
         """scheme: "edge" (run_edge.py), "ed8" (run_ed8.py), "ed4" (run_ed4.py), "ge"
         (run_ge.py) or "efm" (run_efm.py): the backward map's march; ED8 and GE keep a full
         Hermite map, ED4 an FD-unrolled one (psi and T only), and they pull back through
         the map's face values; EFM keeps no cell map: it caches the velocity of every
         step since the last reinitialisation and re-marches the face map back through all
         of them each step.
-        reuse_pressure: start each projection from the same projection's pressure
-        of the previous step (the reference starts from zero, sim/mgpcg.py:67-75;
-        the answer meets the same stop rule either way)."""
+        pressure_start: where each projection's solve starts: "zero" (the
+        reference, sim/mgpcg.py:67-75), "previous" (the same projection's
+        pressure of the previous step) or "extrapolated" (the linear
+        extrapolation in time of that projection's last two pressures,
+        p_n + (dt_new / dt_old) (p_n - p_{n-1})); the answer meets the same stop
+        rule either way, and a guess with ||b - A x0|| > ||b|| is dropped for
+        zero.  reuse_pressure=True without pressure_start means "previous"."""
         if scheme not in ("edge", "ed8", "ed4", "ge", "efm"):
             raise ValueError(f"scheme must be edge, ed8, ed4, ge or efm, got {scheme!r}")
         self.scheme = scheme
@@ -184,8 +188,17 @@ class Simulation:
                                   rel_tol=rel_tol if stop == "relative" else 1e-12, **kw)
             self.poisson_max_iters = 400   # sim/mgpcg.py:436
         self.poisson_stats = []
-        self.reuse_pressure = bool(reuse_pressure)
-        self._p_prev = [None, None]
+        if pressure_start is None:
+            pressure_start = "previous" if reuse_pressure else "zero"
+        if pressure_start not in ("zero", "previous", "extrapolated"):
+            raise ValueError(f"pressure_start must be zero, previous or extrapolated, got {pressure_start!r}")
+        self.pressure_start = pressure_start
+        self.reuse_pressure = pressure_start != "zero"
+        self._p_prev = [None, None]     # per projection: the last pressure
+        self._p_old = [None, None]      # per projection: the one before (extrapolated start)
+        self._p_time = [[], []]         # per projection: simulated times of those two solves
+        self._p_kept = [True, True]     # per projection: did the last solve keep its guess
+        self.t_sim = 0.0                # simulated time at the start of the current step
         V = ti.Vector.field
         F = ti.field
         cell, fx, fy, fz = (rx, ry, rz), (rx + 1, ry, rz), (rx, ry + 1, rz), (rx, ry, rz + 1)
@@ -323,27 +336,52 @@ class Simulation:
 
     def poisson(self, buffers=None, slot=0):
         """The projection.  slot: which of the step's two projections (0 after
-        the semi-Lagrangian half step, 1 at the end of the step); with
-        `reuse_pressure` the solve starts from that projection's pressure of the
-        previous step (kept only when ||b - A x0|| <= ||b||, same stop rule)."""
+        the semi-Lagrangian half step, 1 at the end of the step).  Unless
+        pressure_start is "zero" the solve starts from a guess built from that
+        projection's earlier pressures (kept only when ||b - A x0|| <= ||b||,
+        same stop rule)."""
         if self.rhs_snapshots is not None:
             self.rhs_snapshots.append([self.u_x.data.clone(), self.u_y.data.clone(), self.u_z.data.clone()])
         x0 = None
         if self.reuse_pressure:
-            # each projection keeps its own pressure buffer: the solver's p is pointed
-            # at it, so the solve reads the previous pressure from it (x0) at its start
-            # and writes the new one into it at its end, with no copy in between
-            if self._p_prev[slot] is None:
-                self._p_prev[slot] = torch.zeros_like(self.solver.p.data)
+            # each projection keeps its own pressure buffers: the solver's p is pointed
+            # at the guess, so the solve reads it (x0) at its start and writes the new
+            # pressure into the same buffer at its end, with no copy in between
+            t_now = self.t_sim + (0.5 if slot == 0 else 1.0) * self.curr_dt
+            prev, old, times = self._p_prev[slot], self._p_old[slot], self._p_time[slot]
+            if prev is None:
+                prev = self._p_prev[slot] = torch.zeros_like(self.solver.p.data)
+            elif self.pressure_start == "extrapolated":
+                if old is None:
+                    old = torch.empty_like(prev)
+                if len(times) == 2 and self._p_kept[slot]:
+                    # old <- prev + r (prev - old) = (1 + r) (prev - r / (1 + r) old)
+                    r = (t_now - times[1]) / (times[1] - times[0])
+                    _lib.call("edge_axpy_scale", prev.data_ptr(), old.data_ptr(), old.data_ptr(), -r / (1.0 + r),
+                              1.0 + r, old.numel(), stream_handle())
+                else:
+                    # one pressure so far, or the last solve dropped its guess (a regime
+                    # change, such as the second projection at a reinitialisation): start
+                    # from the last pressure itself
+                    old.copy_(prev)
+                # the guess's buffer receives the new pressure; prev becomes the older one
+                self._p_prev[slot], self._p_old[slot] = old, prev
+                x0 = old
             else:
-                x0 = self._p_prev[slot]
+                x0 = prev
             self.solver.p.data = self._p_prev[slot]
+            times.append(t_now)
+            del times[:-2]
+        else:
+            self._p_time[slot].clear()   # steps solved from zero break the time series
         if buffers is None:
             self.solver.Poisson(self.u_x, self.u_y, self.u_z, max_iters=self.poisson_max_iters, x0=x0)
         else:
             self.solver.Poisson_with_buffer(self.u_x, self.u_y, self.u_z, *buffers, max_iters=self.poisson_max_iters,
                                             x0=x0)
-        self.poisson_stats.append(dict(self.solver.stats, time=self.solver.last_time))
+        st = self.solver.stats
+        self._p_kept[slot] = x0 is None or bool(st.get("x0_used", 1))
+        self.poisson_stats.append(dict(st, time=self.solver.last_time))
 
     # -- one step (run_edge.py:588-711) --------------------------------------------
     def substep(self):
@@ -486,6 +524,7 @@ class Simulation:
         if self.lp is not None:
             prefetch_max_particle_speed(self.laden_particles_vel)
         self._mark("end")
+        self.t_sim += curr_dt
         return curr_dt
 
     def backtrack_psi_grid(self, j, curr_dt):
@@ -656,6 +695,8 @@ class Simulation:
         self.frame_idx = int(frame)
         self.sub_t = 0.0
         self.i = -1
+        # the pressures of the steps before the load are no guess for the steps after it
+        self._p_prev, self._p_old, self._p_time, self._p_kept = [None, None], [None, None], [[], []], [True, True]
         sp = io_utils.state_path(ckptdir, frame)
         if not os.path.exists(sp):
             return False

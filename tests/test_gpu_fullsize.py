@@ -108,8 +108,8 @@ def test_full_size_mixed_steps_are_deterministic_and_converged(dev):
 
 @pytest.mark.timeout(300)
 def test_full_size_thirty_steps_invariants(dev):
-    """The bench's workload (256x128x128, four rings, mixed projection from the
-    previous step's pressure) for 30 steps, across two map reinitialisations
+    """The bench's workload (256x128x128, four rings, mixed projection started
+    from the extrapolation of the last two steps' pressures) for 30 steps, across two map reinitialisations
     (every 12 steps): every projection converged (||r|| <= 1e-6 ||b||), the
     velocity divergence after the step's last projection at the solver's
     level (at most 1e-5 ||b||), the kinetic energy within 10% of the start (the reference scheme
@@ -118,7 +118,8 @@ def test_full_size_thirty_steps_invariants(dev):
     domain, and a dt within the CFL bound every step."""
     from run_edge import Simulation
     from test_gpu_step import _fluid_div
-    sim = Simulation("cfg2", solver_mode="mixed", res=RES, num_samples=400, check_every=0, reuse_pressure=True)
+    sim = Simulation("cfg2", solver_mode="mixed", res=RES, num_samples=400, check_every=0,
+                     pressure_start="extrapolated")
     sim.initialize()
     e0 = sum(float((f.data.double() ** 2).sum()) for f in (sim.u_x, sim.u_y, sim.u_z))
     s_lo, s_hi = float(sim.smoke.data.min()), float(sim.smoke.data.max())

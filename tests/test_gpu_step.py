@@ -7,6 +7,8 @@ the oracle's composition of the same step (oracle/step.py).
 * a 10-step trajectory with converged projections (1e-6 relative on both
   sides): velocity within 1e-3 of scale (the trajectory bound).
 """
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -385,34 +387,46 @@ def test_dye_steps_fixed_iterations(dev):
     assert err < 1e-6 and moved > 0.1
 
 
+@pytest.mark.parametrize("start", ["previous", "extrapolated"])
 @pytest.mark.parametrize("mode", ["mixed", "dense"])
-def test_previous_pressure_start(dev, mode):
-    """Simulation(reuse_pressure=True): each projection starts from the same
-    projection's pressure of the previous step (bench.py's default).  Four steps
-    against the zero start: the same velocities within the stop rule's reach
-    (both solves end at ||r|| <= 1e-6 ||b||), the guess kept from the second
-    step on, no more than one extra iteration in total (the counts are
-    printed), and the solver's p is the last
-    solve's pressure (it lives in the projection's own buffer)."""
+def test_previous_pressure_start(dev, mode, start):
+    """Simulation(pressure_start=...): each projection starts from the same
+    projection's pressure of the previous step ("previous"), or from the linear
+    extrapolation in time of its last two pressures ("extrapolated", bench.py's
+    default).  Six steps against the zero start: the same velocities within
+    the stop rule's reach (both solves end at ||r|| <= 1e-6 ||b||), the guess
+    kept from the second step on, no more than one extra iteration in total
+    (the counts are printed), the extrapolated guess closer to the answer than
+    the previous pressure from the third step on, and the solver's p is the
+    last solve's pressure (it lives in the projection's own buffer)."""
     from run_edge import Simulation
     out = {}
-    for reuse in (False, True):
+    steps = 6
+    for st in ("zero", "previous", start):
+        if st in out:
+            continue
         sim = Simulation("cfg1", solver_mode=mode, res=(64, 32, 32), num_samples=300, check_every=0,
-                         reuse_pressure=reuse)
+                         pressure_start=st)
         sim.initialize()
-        for _ in range(4):
+        for _ in range(steps):
             sim.substep()
         torch.cuda.synchronize()
-        out[reuse] = (sim.velocity_numpy(), [dict(s) for s in sim.poisson_stats[-8:]], sim)
-    (u0, st0, _), (u1, st1, sim1) = out[False], out[True]
+        out[st] = (sim.velocity_numpy(), [dict(s) for s in sim.poisson_stats[-2 * steps:]], sim)
+    (u0, st0, _), (u1, st1, sim1) = out["zero"], out[start]
     for a, (g, r) in enumerate(zip(u1, u0)):
         e = scale_err(g, r)
-        print(f"{mode} u_{'xyz'[a]}: previous-pressure start vs zero start {e:.3g}")
+        print(f"{mode} u_{'xyz'[a]}: {start} start vs zero start {e:.3g}")
         assert e < 1e-4
     for s in st0 + st1:
         assert s["converged"] == 1 and s["rTr"] <= 1e-12 * s["b2"] * (1 + 1e-9), s
     assert all(s["x0_used"] == 1 for s in st1[2:])
     it0, it1 = sum(s["iters"] for s in st0), sum(s["iters"] for s in st1)
-    print(f"{mode}: iterations over four steps, zero start {it0}, previous pressure {it1}")
+    print(f"{mode}: iterations over {steps} steps, zero start {it0}, {start} start {it1}")
     assert it1 <= it0 + 1   # a kept guess has ||b - A x0|| <= ||b||: at most one iteration more
+    if start == "extrapolated":
+        g_prev = [math.sqrt(s["r02"] / s["b2"]) for s in out["previous"][1][4:]]
+        g_extr = [math.sqrt(s["r02"] / s["b2"]) for s in st1[4:]]
+        print(f"{mode}: guess ||b - A x0|| / ||b|| from step 3, previous {['%.3g' % v for v in g_prev]}, "
+              f"extrapolated {['%.3g' % v for v in g_extr]}")
+        assert sum(g_extr) < sum(g_prev)
     assert sim1.solver.p.data.data_ptr() == sim1._p_prev[1].data_ptr()
