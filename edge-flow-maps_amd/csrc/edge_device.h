@@ -138,26 +138,32 @@ struct Axis3 {
   float dw[3];   // dN_2 at the taps
 };
 
-// u = p/dx - BL (already shifted); hi = dim-2-eps; dmax = dim-1.
+// u = p/dx - BL (already shifted); hi = dim-2-eps (>= 1: every grid has >= 3
+// cells along an axis).
 // The reference's 64-tap loop (util/taichi_utils.py:304-317) visits taps
 // iu-1..iu+2 where iu = floor(clamped u); the weight of iu-1 vanishes when the
-// fraction is >= 0.5 and that of iu+2 when it is <= 0.5.
+// fraction is >= 0.5 and that of iu+2 when it is <= 0.5.  The three taps with
+// weight are therefore c-1, c, c+1 around the nearest node c = rint(s), and
+// with t = s - c in [-0.5, 0.5] the weights are 0.5 (0.5 - t)^2, 0.75 - t^2,
+// 0.5 (0.5 + t)^2 and their derivatives t - 0.5, -2t, t + 0.5.  Every step is
+// exact in FP32 (s - c, 0.5 -+ t), so the weights carry the bits of the
+// floor form 0.5 (1.5 - x)^2 ... with x = s - (c - 1); at a tie (t = +-0.5,
+// where rint rounds to even) the taps shift by one and the tap dropped or
+// added has weight 0.  The clamp is one v_med3 (the same as max(1, min(u, hi))
+// for hi >= 1).
 EDGE_DEV Axis3 bspline_axis(float u, float hi) {
-  float s = fmaxf(1.0f, fminf(u, hi));
-  float iu = floorf(s);
-  float fr = s - iu;
+  const float s = __builtin_amdgcn_fmed3f(u, 1.0f, hi);
+  const float c = __builtin_rintf(s);
+  const float t = s - c;
   Axis3 A;
-  int start = (int)iu - (fr < 0.5f ? 1 : 0);
-  A.i0 = start;
-  float x0 = s - (float)start;         // in [0.5, 1.5)
-  float x1 = x0 - 1.0f;                // in [-0.5, 0.5)
-  float a2 = 2.0f - x0;                // |x2|, in (0.5, 1.5]
-  A.w[0] = 0.5f * ((1.5f - x0) * (1.5f - x0));
-  A.w[1] = 0.75f - x1 * x1;
-  A.w[2] = 0.5f * ((1.5f - a2) * (1.5f - a2));
-  A.dw[0] = 0.5f * (2.0f * x0 - 3.0f);
-  A.dw[1] = -2.0f * x1;
-  A.dw[2] = -(0.5f * (2.0f * a2 - 3.0f));
+  A.i0 = (int)(c - 1.0f);   // c - 1 exact; an integer -1 here splits the merged 3-tap loads
+  const float h0 = 0.5f - t, h2 = 0.5f + t;
+  A.w[0] = 0.5f * (h0 * h0);
+  A.w[1] = 0.75f - t * t;
+  A.w[2] = 0.5f * (h2 * h2);
+  A.dw[0] = -h0;
+  A.dw[1] = -2.0f * t;
+  A.dw[2] = h2;
   return A;
 }
 
@@ -189,6 +195,9 @@ EDGE_DEV float bspline_eval(Rsrc f, uint32_t s0, uint32_t s1, const Axis3& A, co
   // Separable contraction, one axis at a time (k, then j, then i): 93 VALU
   // operations with the gradient instead of 117 for the 27 weight products.
   const uint32_t vb = 4u * ((uint32_t)A.i0 * s0 + (uint32_t)B.i0 * s1 + (uint32_t)C.i0);
+  // Every sum is written as explicit fmas, so each term is rounded once
+  // whatever the weights' expressions (left to itself the compiler moves the
+  // -2 of dw[1] = -2t out of the product and rounds tap * t separately).
   float val = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -197,19 +206,19 @@ EDGE_DEV float bspline_eval(Rsrc f, uint32_t s0, uint32_t s1, const Axis3& A, co
     for (int b = 0; b < 3; ++b) {
       const uint32_t so = 4u * (a * s0 + b * s1);
       const float v0 = bload(f, vb, so), v1 = bload(f, vb + 4u, so), v2 = bload(f, vb + 8u, so);
-      const float sk = v0 * C.w[0] + v1 * C.w[1] + v2 * C.w[2];
-      t += sk * B.w[b];
+      const float sk = __builtin_fmaf(v2, C.w[2], __builtin_fmaf(v1, C.w[1], v0 * C.w[0]));
+      t = b ? __builtin_fmaf(sk, B.w[b], t) : sk * B.w[b];
       if (GRAD) {
-        const float sdk = v0 * C.dw[0] + v1 * C.dw[1] + v2 * C.dw[2];
-        ty += sk * B.dw[b];
-        tz += sdk * B.w[b];
+        const float sdk = __builtin_fmaf(v2, C.dw[2], __builtin_fmaf(v1, C.dw[1], v0 * C.dw[0]));
+        ty = b ? __builtin_fmaf(sk, B.dw[b], ty) : sk * B.dw[b];
+        tz = b ? __builtin_fmaf(sdk, B.w[b], tz) : sdk * B.w[b];
       }
     }
-    val += t * A.w[a];
+    val = a ? __builtin_fmaf(t, A.w[a], val) : t * A.w[a];
     if (GRAD) {
-      gx += t * A.dw[a];
-      gy += ty * A.w[a];
-      gz += tz * A.w[a];
+      gx = a ? __builtin_fmaf(t, A.dw[a], gx) : t * A.dw[a];
+      gy = a ? __builtin_fmaf(ty, A.w[a], gy) : ty * A.w[a];
+      gz = a ? __builtin_fmaf(tz, A.w[a], gz) : tz * A.w[a];
     }
   }
   if (GRAD) {
@@ -277,7 +286,7 @@ struct HAxis {
 // sim/hermite.py:4-41 for x = frac (corner 0) and x = frac - 1 (corner 1);
 // the "g" functions carry the dx of value_1..3 (sim/hermite.py:500-510).
 EDGE_DEV HAxis hermite_axis(float u, float hi, float dx) {
-  float s = fmaxf(1.0f, fminf(u, hi));
+  float s = __builtin_amdgcn_fmed3f(u, 1.0f, hi);   // max(1, min(u, hi)), hi >= 1
   float iu = floorf(s);
   HAxis H;
   H.i0 = (int)iu;

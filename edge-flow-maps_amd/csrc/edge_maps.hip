@@ -154,7 +154,7 @@ __device__ __forceinline__ void linear_taps(const Grid& g, const float* __restri
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     float u = p[d] * g.idx - 0.5f;
-    ss[d] = fmaxf(1.0f, fminf(u, g.hi1_c[d]));
+    ss[d] = __builtin_amdgcn_fmed3f(u, 1.0f, g.hi1_c[d]);   // max(1, min(u, hi)), hi >= 1
     float fl = floorf(ss[d]);
     fr[d] = ss[d] - fl;
     i0[d] = min((int)fl, dims[d] - 1);
