@@ -92,3 +92,22 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   if (per == 0 || b >= per * X) return b;
   return (b % X) * per + b / X;
 }
+
+#ifndef EDGE_BAND
+#define EDGE_BAND 0
+#endif
+// Cell kernels whose threads gather around their own cell (the marches): each
+// XCD still takes a contiguous slab of whole planes (xcd_block), but walks it
+// in bands of W blocks of every plane in turn instead of plane by plane, so the
+// blocks in flight on one XCD cover a narrow band through many planes and the
+// +-1-plane halo of their gathers stays in that XCD's L2.  bpp: blocks per
+// plane (0: off).
+__device__ __forceinline__ int band_block(int b, int nb, int bpp, int W) {
+  const int X = 8;
+  const int per = nb / X;
+  if (W <= 0 || bpp <= 0 || per == 0 || b >= per * X || per % bpp || bpp % W) return xcd_block(b, nb);
+  const int planes = per / bpp;
+  const int L = b / X;
+  const int band = L / (W * planes), rem = L % (W * planes);
+  return (b % X) * per + (rem / W) * bpp + band * W + rem % W;
+}

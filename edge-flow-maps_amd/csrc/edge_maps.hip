@@ -26,12 +26,34 @@ __device__ __forceinline__ int64_t nfaces(const Grid& g) {
   return (int64_t)(g.nx + 1) * g.ny * g.nz + (int64_t)g.nx * (g.ny + 1) * g.nz + (int64_t)g.nx * g.ny * (g.nz + 1);
 }
 
+// min and max of the six neighbours of face (i, j, k) on face grid a of u1
+// (indices clamped to the grid, as clamp_u's sample() does, run_edge.py:508-519).
+__device__ __forceinline__ void face_nbr_range(const Grid& g, int a, int i, int j, int k, const float* __restrict__ u1,
+                                               float& mn, float& mx) {
+  int d[3];
+  face_dims(g, a, d);
+  auto at = [&](int ii, int jj, int kk) {
+    ii = max(0, min(ii, d[0] - 1));
+    jj = max(0, min(jj, d[1] - 1));
+    kk = max(0, min(kk, d[2] - 1));
+    return u1[((int64_t)ii * d[1] + jj) * d[2] + kk];
+  };
+  float l_ = at(i - 1, j, k), r_ = at(i + 1, j, k), b_ = at(i, j - 1, k), t_ = at(i, j + 1, k),
+        a_ = at(i, j, k - 1), c_ = at(i, j, k + 1);
+  mx = fmaxf(fmaxf(fmaxf(l_, r_), fmaxf(b_, t_)), fmaxf(a_, c_));
+  mn = fminf(fminf(fminf(l_, r_), fminf(b_, t_)), fminf(a_, c_));
+}
+
 // run_edge.py:360-373 map_u: u1_a = F_a . interp_u_MAC(u0, phi_a).  With I
 // given (I.f[0] non-null) the stored value is sc * (u1_a + m * I_a), rounded as
 // add_fields(u1, I, u1, m) then scale_field(u1, sc, u1) round it (the BFECC
-// error (map - init) / 2 without the extra pass over u1).
+// error (map - init) / 2 without the extra pass over u1).  With C given
+// (C.f[0] non-null) the stored value is clamp(C_a - u1_a) to the min and max of
+// C_a's six neighbours: the BFECC correction u - map(err) and clamp_u in the same
+// pass (run_edge.py:683-693 with BFECC_clamp; the map's value is the tmp_u that
+// the separate passes would store and reload).
 __global__ void __launch_bounds__(256) k_map_u(Grid g, Mac U0, Faces U1, CFaces F, CFaces PHI, CFaces I, float m,
-                                               float sc) {
+                                               float sc, CFaces C) {
   int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   int a, i, j, k;
   int64_t l, nf;
@@ -45,6 +67,11 @@ __global__ void __launch_bounds__(256) k_map_u(Grid g, Mac U0, Faces U1, CFaces 
   if (I.f[0]) {
     r = r + m * I.f[a][l];
     r = sc * r;
+  }
+  if (C.f[0]) {
+    float mn, mx;
+    face_nbr_range(g, a, i, j, k, C.f[a], mn, mx);
+    r = fminf(fmaxf(C.f[a][l] - r, mn), mx);
   }
   U1.f[a][l] = r;
 }
@@ -194,7 +221,10 @@ __device__ __forceinline__ void linear_taps(const Grid& g, const float* __restri
 // psi_c = mean of the six face points of phi.
 // With s2 given, mode 0 stores sc * (map + m * s2) and mode 1 clamps s1 + m * s2,
 // rounded as add_fields then scale_field (mode 0) or add_fields (mode 1) would
-// round them in a pass of their own.
+// round them in a pass of their own.  Mode 2 clamps s1 + m * map(s2): s2 is
+// first pulled back through the same map (mode 0's value, kept in a register
+// instead of a field), so one pass does map_smoke(s2, tmp) and the clamp of
+// s1 - tmp (run_edge.py:684-689).
 template <int NC>
 __global__ void __launch_bounds__(256) k_map_smoke(Grid g, int mode, const float* __restrict__ s0, float* __restrict__ s1,
                                                    CFaces PHI, const float* __restrict__ s2, float m, float sc) {
@@ -216,9 +246,14 @@ __global__ void __launch_bounds__(256) k_map_smoke(Grid g, int mode, const float
     p[c] = (1.0f / 6.0f) * s;
   }
   float out[NC], mn[NC], mx[NC];
-  linear_taps<NC>(g, s0, p, mode, out, mn, mx);
   float v2[NC], res[NC];
-  if (s2) ld_cell<NC>(s2, q, v2);
+  if (mode == 2) {
+    linear_taps<NC>(g, s2, p, 0, v2, mn, mx);     // v2 = map(s2)
+    linear_taps<NC>(g, s0, p, 1, out, mn, mx);    // the clamp range from s0
+  } else {
+    linear_taps<NC>(g, s0, p, mode, out, mn, mx);
+    if (s2) ld_cell<NC>(s2, q, v2);
+  }
   if (mode == 0) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -288,19 +323,9 @@ __global__ void __launch_bounds__(256) k_clamp_u(Grid g, CFaces U, CFaces U1, Fa
   int a, i, j, k;
   int64_t l, nf;
   if (!face_of(g, t, a, l, nf, i, j, k)) return;
-  int d[3];
-  face_dims(g, a, d);
   const float* u1 = U1.f[a];
-  auto at = [&](int ii, int jj, int kk) {
-    ii = max(0, min(ii, d[0] - 1));
-    jj = max(0, min(jj, d[1] - 1));
-    kk = max(0, min(kk, d[2] - 1));
-    return u1[((int64_t)ii * d[1] + jj) * d[2] + kk];
-  };
-  float l_ = at(i - 1, j, k), r_ = at(i + 1, j, k), b_ = at(i, j - 1, k), t_ = at(i, j + 1, k),
-        a_ = at(i, j, k - 1), c_ = at(i, j, k + 1);
-  float mx = fmaxf(fmaxf(fmaxf(l_, r_), fmaxf(b_, t_)), fmaxf(a_, c_));
-  float mn = fminf(fminf(fminf(l_, r_), fminf(b_, t_)), fminf(a_, c_));
+  float mn, mx;
+  face_nbr_range(g, a, i, j, k, u1, mn, mx);
   float v = SUB ? u1[l] - U2.f[a][l] : U.f[a][l];
   U2.f[a][l] = fminf(fmaxf(v, mn), mx);
 }
@@ -602,7 +627,7 @@ int edge_map_u(const float* u0x, const float* u0y, const float* u0z, float* u1x,
                                   256)),
                      dim3(256), 0, (hipStream_t)stream, g, Mac{u0x, u0y, u0z}, Faces{{u1x, u1y, u1z}},
                      CFaces{{Fx, Fy, Fz}}, CFaces{{phix, phiy, phiz}}, CFaces{{nullptr, nullptr, nullptr}}, 0.f,
-                     0.f);
+                     0.f, CFaces{{nullptr, nullptr, nullptr}});
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }
@@ -621,7 +646,29 @@ int edge_map_u_add_scale(const float* u0x, const float* u0y, const float* u0z, f
                                   (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1),
                                   256)),
                      dim3(256), 0, (hipStream_t)stream, g, Mac{u0x, u0y, u0z}, Faces{{u1x, u1y, u1z}},
-                     CFaces{{Fx, Fy, Fz}}, CFaces{{phix, phiy, phiz}}, CFaces{{ix, iy, iz}}, m, s);
+                     CFaces{{Fx, Fy, Fz}}, CFaces{{phix, phiy, phiz}}, CFaces{{ix, iy, iz}}, m, s,
+                     CFaces{{nullptr, nullptr, nullptr}});
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_map_u_sub_clamp(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+                         const float* Fx, const float* Fy, const float* Fz, const float* phix, const float* phiy,
+                         const float* phiz, const float* cx, const float* cy, const float* cz, int nx, int ny, int nz,
+                         float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_map_u_sub_clamp: bad dims");
+  EDGE_REQUIRE(u0x && u0y && u0z && u1x && u1y && u1z && Fx && Fy && Fz && phix && phiy && phiz && cx && cy && cz,
+               "edge_map_u_sub_clamp: null pointer");
+  EDGE_REQUIRE(cx != u1x && cy != u1y && cz != u1z && u0x != u1x && u0y != u1y && u0z != u1z,
+               "edge_map_u_sub_clamp: the output must differ from the pulled-back and the clamped fields");
+  Grid g = make_grid(nx, ny, nz, dx);
+  hipLaunchKernelGGL(k_map_u, dim3(blocks_for(
+                                  (int64_t)(nx + 1) * ny * nz + (int64_t)nx * (ny + 1) * nz + (int64_t)nx * ny * (nz + 1),
+                                  256)),
+                     dim3(256), 0, (hipStream_t)stream, g, Mac{u0x, u0y, u0z}, Faces{{u1x, u1y, u1z}},
+                     CFaces{{Fx, Fy, Fz}}, CFaces{{phix, phiy, phiz}}, CFaces{{nullptr, nullptr, nullptr}}, 0.f, 0.f,
+                     CFaces{{cx, cy, cz}});
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }
@@ -659,7 +706,8 @@ static int map_smoke_launch(const char* who, int mode, const float* s0, float* s
                             float sc, int nc, const float* phix, const float* phiy, const float* phiz, int nx, int ny,
                             int nz, float dx, void* stream) {
   EDGE_REQUIRE(dims_ok(nx, ny, nz), "%s: bad dims", who);
-  EDGE_REQUIRE(s0 && s1 && phix && phiy && phiz && (mode == 0 || mode == 1), "%s: bad arguments", who);
+  EDGE_REQUIRE(s0 && s1 && phix && phiy && phiz && (mode == 0 || mode == 1 || (mode == 2 && s2)),
+               "%s: bad arguments", who);
   EDGE_REQUIRE(nc == 4 || nc == 1, "%s: nc must be 1 or 4 (got %d)", who, nc);
   EDGE_REQUIRE(s0 != s1 && s2 != s1 && s2 != s0, "%s: the smoke fields must be distinct", who);
   EDGE_REQUIRE(nc == 1 || (aligned16(s0) && aligned16(s1) && aligned16(s2)),

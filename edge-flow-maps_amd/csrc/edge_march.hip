@@ -34,7 +34,9 @@ __device__ __forceinline__ void write_map_row(const MapOut& o, int64_t N, int64_
 __global__ void __launch_bounds__(256) EDGE_MARCH_ATTR k_march_edge(Grid g, Mac U, const float4* __restrict__ tab,
                                                     const float* __restrict__ X, MapOut o, float dt, float eps) {
   const int64_t N = ncell(g);
-  const int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int plane = g.ny * g.nz;
+  const int64_t q =
+      (int64_t)band_block(blockIdx.x, gridDim.x, plane % 256 ? 0 : plane / 256, EDGE_BAND) * blockDim.x + threadIdx.x;
   if (q >= N) return;
   const float x0[3] = {X[q], X[N + q], X[2 * N + q]};
   // dT accumulators: dT[c][0] = sum s_z T[c][1] + s_y T[c][2], etc.

@@ -47,6 +47,7 @@ SIGNATURES = {
     "edge_clamp_u": "pppppppppiiip",
     "edge_sub_clamp_u": "ppppppiiip",
     "edge_map_u_add_scale": "pppppppppppppppffiiifp",
+    "edge_map_u_sub_clamp": "pppppppppppppppiiifp",
     "edge_map_smoke_add_scale": "ipppffipppiiifp",
     "edge_advect_u_sl": "ppppppiiiffp",
     "edge_max_speed": "ppppiiip",

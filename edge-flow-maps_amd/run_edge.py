@@ -38,7 +38,7 @@ from edge_hip import _lib  # noqa: E402
 from edge_hip.field import Field, stream_handle, ti  # noqa: E402
 from sim.edge_driver import (calc_max_particle_speed, calc_max_speed, clamp_smoke_psi, map_smoke,  # noqa: E402
                              map_smoke_psi, map_u, map_u_psi, march_phi_grid, prefetch_max_particle_speed,
-                             prefetch_max_speed, reset_to_identity, sub_clamp_u3)
+                             prefetch_max_speed, reset_to_identity)
 from sim import lp as lp_kernels  # noqa: E402
 from sim.edge_driver import backtrack_faces, clamp_smoke, split_psi_hermite  # noqa: E402
 from sim.hermite import (advect_ps_garm, advect_ps_hermite_ed4_fd_unroll, advect_ps_hermite_ed8,  # noqa: E402
@@ -466,22 +466,23 @@ class Simulation:
         if self.scheme == "edge":
             gm = self.garm_map
 
-            def pull_u(u0, u1):
-                map_u_psi(*u0, *u1, dx, gm)
+            def pull_u(u0, u1, sub_clamp=None):
+                map_u_psi(*u0, *u1, dx, gm, sub_clamp=sub_clamp)
 
-            def pull_s(s0, s1, clamp=False, sub=None):
+            def pull_s(s0, s1, clamp=False, sub_map=None):
                 if clamp:
-                    clamp_smoke_psi(s0, s1, dx, gm, sub=sub)
+                    clamp_smoke_psi(s0, s1, dx, gm, sub_map=sub_map)
                 else:
                     map_smoke_psi(s0, s1, dx, gm)
         else:
             # run_ed8.py:409-433 (run_ed4.py:447-467): through the map's face values
-            def pull_u(u0, u1):
-                map_u(*u0, *u1, self.T_x, self.T_y, self.T_z, self.psi_x, self.psi_y, self.psi_z, dx)
+            def pull_u(u0, u1, sub_clamp=None):
+                map_u(*u0, *u1, self.T_x, self.T_y, self.T_z, self.psi_x, self.psi_y, self.psi_z, dx,
+                      sub_clamp=sub_clamp)
 
-            def pull_s(s0, s1, clamp=False, sub=None):
+            def pull_s(s0, s1, clamp=False, sub_map=None):
                 if clamp:
-                    clamp_smoke(s0, s1, self.psi_x, self.psi_y, self.psi_z, dx, sub=sub)
+                    clamp_smoke(s0, s1, self.psi_x, self.psi_y, self.psi_z, dx, sub_map=sub_map)
                 else:
                     map_smoke(s0, s1, self.psi_x, self.psi_y, self.psi_z, dx)
         U, IU = (self.u_x, self.u_y, self.u_z), (self.init_u_x, self.init_u_y, self.init_u_z)
@@ -493,16 +494,18 @@ class Simulation:
               add_scale=(*IU, -1., 0.5))
         map_smoke(self.smoke, self.err_smoke, self.phi_x, self.phi_y, self.phi_z, dx,
                   add_scale=(self.init_smoke, -1., 0.5))
-        pull_u(EU, TU)
-        pull_s(self.err_smoke, self.tmp_smoke)
-        if not self.BFECC_clamp:
+        if self.BFECC_clamp:
+            # run_edge.py:683-693: tmp_u = map(err_u), err_u = u - tmp_u, tmp_u =
+            # clamp_u(err_u) in one pass (neither intermediate is stored)
+            pull_u(EU, TU, sub_clamp=U)
+        else:
+            pull_u(EU, TU)
             for u_, t_, e_ in zip(U, TU, EU):
                 add_fields(u_, t_, e_, -1.)
-        # run_edge.py:688-689: smoke = clamp(smoke - tmp_smoke), one pass
-        pull_s(self.init_smoke, self.smoke, clamp=True, sub=self.tmp_smoke)
+        # run_edge.py:684-689: tmp_smoke = map(err_smoke), smoke = clamp(smoke -
+        # tmp_smoke), one pass (the pulled-back error stays in registers)
+        pull_s(self.init_smoke, self.smoke, clamp=True, sub_map=self.err_smoke)
         if self.BFECC_clamp:
-            # err = u - tmp, then tmp = clamp(err) in one pass (err_u stays unused)
-            sub_clamp_u3(self.u_x, self.u_y, self.u_z, self.tmp_u_x, self.tmp_u_y, self.tmp_u_z)
             self.u_x.swap_storage(self.tmp_u_x)
             self.u_y.swap_storage(self.tmp_u_y)
             self.u_z.swap_storage(self.tmp_u_z)

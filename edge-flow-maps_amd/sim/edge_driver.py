@@ -143,24 +143,37 @@ def backtrack_faces(psi_x, psi_y, psi_z, T_x, T_y, T_z, velocities, dts, dx):
     touch(psi_x, psi_y, psi_z, T_x, T_y, T_z)
 
 
-def map_u(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, T_x, T_y, T_z, psi_x, psi_y, psi_z, dx, add_scale=None):
+def map_u(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, T_x, T_y, T_z, psi_x, psi_y, psi_z, dx, add_scale=None,
+          sub_clamp=None):
     """run_edge.py:360-373.
 
     add_scale = (f_x, f_y, f_z, multiplier, scale) folds the BFECC arithmetic
     into the same pass: u1 = scale * (map_u(u0) + multiplier * f), the values of
     map_u, add_fields(u1, f, u1, multiplier), scale_field(u1, scale, u1)
-    (run_edge.py:671-681)."""
+    (run_edge.py:671-681).
+    sub_clamp = (c_x, c_y, c_z) stores clamp(c - map_u(u0)) to the min and max
+    of c's six neighbours: map_u into tmp_u, add_fields(c, tmp_u, err, -1) and
+    clamp_u(err, c, tmp_u) (run_edge.py:683-693) in one pass, the same values."""
     nx, ny, nz = _cells_of_mac(u_x0)
-    args = [as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1), as_ptr(T_x),
-            as_ptr(T_y), as_ptr(T_z), as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z)]
+    _map_u_ptrs([as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1), as_ptr(T_x),
+                 as_ptr(T_y), as_ptr(T_z), as_ptr(psi_x), as_ptr(psi_y), as_ptr(psi_z)], nx, ny, nz, dx,
+                add_scale, sub_clamp)
+    touch(u_x1, u_y1, u_z1)
+
+
+def _map_u_ptrs(args, nx, ny, nz, dx, add_scale=None, sub_clamp=None):
+    if add_scale is not None and sub_clamp is not None:
+        raise ValueError("give add_scale or sub_clamp, not both")
     with timing.span("k_map_u"):
-        if add_scale is None:
-            _lib.call("edge_map_u", *args, nx, ny, nz, float(dx), stream_handle())
-        else:
+        if add_scale is not None:
             f_x, f_y, f_z, m, sc = add_scale
             _lib.call("edge_map_u_add_scale", *args, as_ptr(f_x), as_ptr(f_y), as_ptr(f_z), float(m), float(sc), nx,
                       ny, nz, float(dx), stream_handle())
-    touch(u_x1, u_y1, u_z1)
+        elif sub_clamp is not None:
+            _lib.call("edge_map_u_sub_clamp", *args, *[as_ptr(c) for c in sub_clamp], nx, ny, nz, float(dx),
+                      stream_handle())
+        else:
+            _lib.call("edge_map_u", *args, nx, ny, nz, float(dx), stream_handle())
 
 
 def _garm_table(m, dx):
@@ -208,17 +221,18 @@ def face_map_of(m, dx):
     return d["face"]
 
 
-def map_u_psi(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, dx, garm_map):
+def map_u_psi(u_x0, u_y0, u_z0, u_x1, u_y1, u_z1, dx, garm_map, sub_clamp=None):
     """run_edge.py:376-417; garm_map = (psi_garm, T_x/y/z_garm, dT_x/y/z_garm).
 
     u1_a(X_a) = sum_c d psi_c / d x_a (X_a) u0_c(psi(X_a)): the map's value and
     column at the faces are evaluated once per map state and shared by the
-    step's two calls; the pull-back itself is map_u's kernel."""
+    step's two calls; the pull-back itself is map_u's kernel.  sub_clamp as in
+    map_u (the BFECC correction and clamp_u of run_edge.py:683-693 in the same
+    pass)."""
     nx, ny, nz = _cells_of_mac(u_x0)
     phi, F = face_map_of(garm_map, dx)
-    with timing.span("k_map_u"):
-        _lib.call("edge_map_u", as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
-                  *[t.data_ptr() for t in F], *[t.data_ptr() for t in phi], nx, ny, nz, float(dx), stream_handle())
+    _map_u_ptrs([as_ptr(u_x0), as_ptr(u_y0), as_ptr(u_z0), as_ptr(u_x1), as_ptr(u_y1), as_ptr(u_z1),
+                 *[t.data_ptr() for t in F], *[t.data_ptr() for t in phi]], nx, ny, nz, dx, None, sub_clamp)
     touch(u_x1, u_y1, u_z1)
 
 
@@ -246,10 +260,23 @@ def map_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx, add_scale=None):
     _smoke(0, smoke0, smoke1, psi_x, psi_y, psi_z, dx, add_scale)
 
 
-def clamp_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx, sub=None):
+def _sub_args(sub, sub_map):
+    """(mode, add) of a clamp: smoke1 - sub (mode 1) or smoke1 - map(sub_map) (mode 2)."""
+    if sub is not None and sub_map is not None:
+        raise ValueError("give sub or sub_map, not both")
+    if sub_map is not None:
+        return 2, (sub_map, -1., 1.)
+    return 1, None if sub is None else (sub, -1., 1.)
+
+
+def clamp_smoke(smoke0, smoke1, psi_x, psi_y, psi_z, dx, sub=None, sub_map=None):
     """run_edge.py:469-477 (smoke1 clamped in place).  With sub, smoke1 - sub is
-    clamped: add_fields(smoke1, sub, smoke1, -1) (run_edge.py:688) in the same pass."""
-    _smoke(1, smoke0, smoke1, psi_x, psi_y, psi_z, dx, None if sub is None else (sub, -1., 1.))
+    clamped: add_fields(smoke1, sub, smoke1, -1) (run_edge.py:688) in the same pass.
+    With sub_map, smoke1 - map_smoke(sub_map) is clamped: run_edge.py:684-689
+    (the BFECC error pulled back, subtracted and clamped) in one pass, the
+    pulled-back error kept in registers (tmp_smoke is not written)."""
+    mode, add = _sub_args(sub, sub_map)
+    _smoke(mode, smoke0, smoke1, psi_x, psi_y, psi_z, dx, add)
 
 
 def _smoke_psi(mode, smoke0, smoke1, dx, garm_map, add=None):
@@ -264,10 +291,13 @@ def map_smoke_psi(smoke0, smoke1, dx, garm_map):
     _smoke_psi(0, smoke0, smoke1, dx, garm_map)
 
 
-def clamp_smoke_psi(smoke0, smoke1, dx, garm_map, sub=None):
+def clamp_smoke_psi(smoke0, smoke1, dx, garm_map, sub=None, sub_map=None):
     """run_edge.py:480-505 (smoke1 clamped in place).  With sub, smoke1 - sub is
-    clamped: add_fields(smoke1, sub, smoke1, -1) (run_edge.py:688) in the same pass."""
-    _smoke_psi(1, smoke0, smoke1, dx, garm_map, None if sub is None else (sub, -1., 1.))
+    clamped: add_fields(smoke1, sub, smoke1, -1) (run_edge.py:688) in the same pass.
+    With sub_map, smoke1 - map_smoke_psi(sub_map) is clamped (run_edge.py:684-689
+    in one pass; tmp_smoke is not written)."""
+    mode, add = _sub_args(sub, sub_map)
+    _smoke_psi(mode, smoke0, smoke1, dx, garm_map, add)
 
 
 def clamp_u3(u_x, u_y, u_z, u1_x, u1_y, u1_z, u2_x, u2_y, u2_z):

@@ -132,6 +132,15 @@ int edge_map_u_add_scale(const float* u0x, const float* u0y, const float* u0z, f
                          const float* Fx, const float* Fy, const float* Fz, const float* phix, const float* phiy,
                          const float* phiz, const float* ix, const float* iy, const float* iz, float m, float s,
                          int nx, int ny, int nz, float dx, void* stream);
+/* No single counterpart: map_u (run_edge.py:360-373, or map_u_psi 376-417 through the
+   map's face values) of the BFECC error u0 into tmp_u, then add_fields(c, tmp_u, err, -1)
+   and clamp_u(err, c, tmp_u) (run_edge.py:683-693 with BFECC_clamp):
+   u1 = clamp(c - map_u(u0)) to the min and max of c's six neighbours, in one pass (the
+   same values; tmp_u and err are not stored).  u1 must differ from u0 and c. */
+int edge_map_u_sub_clamp(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
+                         const float* Fx, const float* Fy, const float* Fz, const float* phix, const float* phiy,
+                         const float* phiz, const float* cx, const float* cy, const float* cz, int nx, int ny, int nz,
+                         float dx, void* stream);
 /* map_u_psi (run_edge.py:376-417); tab: garm table of the cell map. */
 int edge_map_u_psi(const float* u0x, const float* u0y, const float* u0z, float* u1x, float* u1y, float* u1z,
                    const float* tab, int nx, int ny, int nz, float dx, void* stream);
@@ -144,7 +153,9 @@ int edge_map_smoke(int mode, const float* s0, float* s1, int nc, const float* ph
 /* No single counterpart: edge_map_smoke with a third smoke field s2 folded in, the same
    values as the separate calls: mode 0 is map_smoke then add_fields(s1, s2, s1, m) and
    scale_field(s1, s, s1) (run_edge.py:672-682); mode 1 is add_fields(s1, s2, s1, m) then
-   clamp_smoke (run_edge.py:688-689).  s0, s1, s2 distinct. */
+   clamp_smoke (run_edge.py:688-689); mode 2 is map_smoke(s2, tmp) then mode 1 with tmp
+   (run_edge.py:684-689: the error pulled back and the corrected smoke clamped in one
+   pass; tmp is not stored).  s0, s1, s2 distinct. */
 int edge_map_smoke_add_scale(int mode, const float* s0, float* s1, const float* s2, float m, float s, int nc,
                              const float* phix, const float* phiy, const float* phiz, int nx, int ny, int nz, float dx,
                              void* stream);

@@ -232,6 +232,27 @@ def test_driver_maps_parity(orc32, orc64, dev):
         u1f, uf = u1[a].astype(np.float32), u[a].astype(np.float32)
         ref = orc32.clamp_u(u1f + np.float32(-1.) * uf, u1f)
         assert np.array_equal(N_(gt[a]).astype(np.float32), ref)
+    # the error's pull-back folded in too: clamp(u1 - map_u(err)) in one pass, the
+    # bits of map_u into t and then the fused tail above
+    gt = ops.map_u(gi, gF, gphi, dx)
+    _lib.call("edge_sub_clamp_u", *[_p(c) for c in gu1], *[_p(c) for c in gt], *n, stream_handle())
+    one = [torch.empty_like(c) for c in gu1]
+    _lib.call("edge_map_u_sub_clamp", *[_p(c) for c in gi], *[_p(c) for c in one], *[_p(c) for c in gF],
+              *[_p(c) for c in gphi], *[_p(c) for c in gu1], *n, dx, stream_handle())
+    torch.cuda.synchronize()
+    for a in range(3):
+        assert torch.equal(one[a], gt[a]), a
+    # and for the smoke: clamp(s1 - map_smoke(s2)) with s0's range (mode 2), the
+    # bits of map_smoke into tmp then mode 1 with tmp
+    tmp = ops.map_smoke(0, gs2, None, gphi, dx)
+    seq = gs1.clone()
+    _lib.call("edge_map_smoke_add_scale", 1, _p(gs0), _p(seq), _p(tmp), -1., 1., 4, *[_p(c) for c in gphi], *n, dx,
+              stream_handle())
+    sm = gs1.clone()
+    _lib.call("edge_map_smoke_add_scale", 2, _p(gs0), _p(sm), _p(gs2), -1., 1., 4, *[_p(c) for c in gphi], *n, dx,
+              stream_handle())
+    torch.cuda.synchronize()
+    assert torch.equal(sm, seq)
     sl = ops.advect_u_sl(gu, dx, 0.25 * dx)
     rsl = orc32.advect_u_sl(u, dx, 0.25 * dx)
     for a in range(3):
