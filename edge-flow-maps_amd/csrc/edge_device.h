@@ -271,6 +271,107 @@ EDGE_DEV float interp_u_comp(const Grid& g, const Mac& U, const float p[3], int 
                              bspline_axis(q2, g.hi_f[2]), nullptr);
 }
 
+// Component A's grid of the MAC velocity: buffer, strides (elements) of axes 0
+// and 1, and the clamp limit of each axis for that component's positions
+// (its own axis is staggered).
+template <int A>
+struct MacComp {
+  Rsrc f;
+  uint32_t s0, s1;
+  EDGE_DEV MacComp(const Grid& g, const Mac& U) {
+    const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
+    if (A == 0) { f = make_rsrc(U.ux, 4u * (nx + 1) * ny * nz); s0 = ny * nz; s1 = nz; }
+    if (A == 1) { f = make_rsrc(U.uy, 4u * nx * (ny + 1) * nz); s0 = (ny + 1) * nz; s1 = nz; }
+    if (A == 2) { f = make_rsrc(U.uz, 4u * nx * ny * (nz + 1)); s0 = ny * (nz + 1); s1 = nz + 1; }
+  }
+};
+
+// The 27 taps of component A around face (i, j, k) of its own grid, held in
+// registers.  For component A the face sits on a node in every axis (index
+// position (i, j, k) after the BL shift), so every B-spline evaluation within
+// half a cell of the face, after the same clamp, has its first taps at
+// w0 = rint(clamp(i, j, k)) - 1: the semi-Lagrangian RK4 stages and foot of
+// that face read these taps again and again.
+template <int A>
+struct OwnTaps {
+  float v[9][3];
+  int w0[3];
+  EDGE_DEV OwnTaps(const Grid& g, const MacComp<A>& M, int i, int j, int k) {
+    const int idx[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float hi = d == A ? g.hi_f[d] : g.hi_c[d];
+      w0[d] = (int)(__builtin_rintf(__builtin_amdgcn_fmed3f((float)idx[d], 1.0f, hi)) - 1.0f);
+    }
+    const uint32_t vb = 4u * ((uint32_t)w0[0] * M.s0 + (uint32_t)w0[1] * M.s1 + (uint32_t)w0[2]);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const uint32_t so = 4u * ((r / 3) * M.s0 + (r % 3) * M.s1);
+      v[r][0] = bload(M.f, vb, so);
+      v[r][1] = bload(M.f, vb + 4u, so);
+      v[r][2] = bload(M.f, vb + 8u, so);
+    }
+  }
+};
+
+// bspline_eval<false>'s contraction of taps already in registers (the same
+// operations in the same order, so the same bits).
+EDGE_DEV float bspline_contract(const float v[9][3], const Axis3& A, const Axis3& B, const Axis3& C) {
+  float val = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float t = 0.f;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const float v0 = v[3 * a + b][0], v1 = v[3 * a + b][1], v2 = v[3 * a + b][2];
+      const float sk = __builtin_fmaf(v2, C.w[2], __builtin_fmaf(v1, C.w[1], v0 * C.w[0]));
+      t = b ? __builtin_fmaf(sk, B.w[b], t) : sk * B.w[b];
+    }
+    val = a ? __builtin_fmaf(t, A.w[a], val) : t * A.w[a];
+  }
+  return val;
+}
+
+// Component A at p from the held taps when p's taps are those, else gathered.
+template <int A>
+EDGE_DEV float own_eval(const MacComp<A>& M, const OwnTaps<A>& W, const Axis3& X, const Axis3& Y, const Axis3& Z) {
+  if (X.i0 == W.w0[0] && Y.i0 == W.w0[1] && Z.i0 == W.w0[2]) return bspline_contract(W.v, X, Y, Z);
+  return bspline_eval<false>(M.f, M.s0, M.s1, X, Y, Z, nullptr);
+}
+
+// interp_u_mac<false> with component A's taps held (the same values).
+template <int A>
+EDGE_DEV void interp_u_mac_own(const Grid& g, const Mac& U, const float p[3], const MacComp<A>& M,
+                               const OwnTaps<A>& W, float out[3]) {
+  float q0 = p[0] * g.idx, q1 = p[1] * g.idx, q2 = p[2] * g.idx;
+  Axis3 ow0 = bspline_axis(q0, g.hi_f[0]), ot0 = bspline_axis(q0 - 0.5f, g.hi_c[0]);
+  Axis3 ow1 = bspline_axis(q1, g.hi_f[1]), ot1 = bspline_axis(q1 - 0.5f, g.hi_c[1]);
+  Axis3 ow2 = bspline_axis(q2, g.hi_f[2]), ot2 = bspline_axis(q2 - 0.5f, g.hi_c[2]);
+  const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
+  if (A == 0) out[0] = own_eval<A>(M, W, ow0, ot1, ot2);
+  else out[0] = bspline_eval<false>(make_rsrc(U.ux, 4u * (nx + 1) * ny * nz), ny * nz, nz, ow0, ot1, ot2, nullptr);
+  if (A == 1) out[1] = own_eval<A>(M, W, ot0, ow1, ot2);
+  else out[1] = bspline_eval<false>(make_rsrc(U.uy, 4u * nx * (ny + 1) * nz), (ny + 1) * nz, nz, ot0, ow1, ot2,
+                                    nullptr);
+  if (A == 2) out[2] = own_eval<A>(M, W, ot0, ot1, ow2);
+  else out[2] = bspline_eval<false>(make_rsrc(U.uz, 4u * nx * ny * (nz + 1)), ny * (nz + 1), nz + 1, ot0, ot1, ow2,
+                                    nullptr);
+}
+
+// interp_u_comp(g, U, p, A) with component A's taps held.
+template <int A>
+EDGE_DEV float interp_u_comp_own(const Grid& g, const float p[3], const MacComp<A>& M, const OwnTaps<A>& W) {
+  const float q0 = p[0] * g.idx, q1 = p[1] * g.idx, q2 = p[2] * g.idx;
+  if (A == 0)
+    return own_eval<A>(M, W, bspline_axis(q0, g.hi_f[0]), bspline_axis(q1 - 0.5f, g.hi_c[1]),
+                       bspline_axis(q2 - 0.5f, g.hi_c[2]));
+  if (A == 1)
+    return own_eval<A>(M, W, bspline_axis(q0 - 0.5f, g.hi_c[0]), bspline_axis(q1, g.hi_f[1]),
+                       bspline_axis(q2 - 0.5f, g.hi_c[2]));
+  return own_eval<A>(M, W, bspline_axis(q0 - 0.5f, g.hi_c[0]), bspline_axis(q1 - 0.5f, g.hi_c[1]),
+                     bspline_axis(q2, g.hi_f[2]));
+}
+
 // ---------------------------------------------------------------------------
 // Tricubic Hermite over a packed table.
 //
