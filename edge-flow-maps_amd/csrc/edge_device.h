@@ -339,23 +339,64 @@ EDGE_DEV float own_eval(const MacComp<A>& M, const OwnTaps<A>& W, const Axis3& X
   return bspline_eval<false>(M.f, M.s0, M.s1, X, Y, Z, nullptr);
 }
 
-// interp_u_mac<false> with component A's taps held (the same values).
-template <int A>
+// 27 taps of one component at the first taps i0, held in registers.
+struct Taps27 {
+  float v[9][3];
+  int i0[3];
+};
+EDGE_DEV void load_taps(Rsrc f, uint32_t s0, uint32_t s1, const Axis3& A, const Axis3& B, const Axis3& C,
+                        Taps27& T) {
+  T.i0[0] = A.i0;
+  T.i0[1] = B.i0;
+  T.i0[2] = C.i0;
+  const uint32_t vb = 4u * ((uint32_t)A.i0 * s0 + (uint32_t)B.i0 * s1 + (uint32_t)C.i0);
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const uint32_t so = 4u * ((r / 3) * s0 + (r % 3) * s1);
+    T.v[r][0] = bload(f, vb, so);
+    T.v[r][1] = bload(f, vb + 4u, so);
+    T.v[r][2] = bload(f, vb + 8u, so);
+  }
+}
+// The value at axes A, B, C from held taps when they are the point's taps, else gathered.
+EDGE_DEV float held_eval(Rsrc f, uint32_t s0, uint32_t s1, const Taps27& T, const Axis3& A, const Axis3& B,
+                         const Axis3& C) {
+  if (A.i0 == T.i0[0] && B.i0 == T.i0[1] && C.i0 == T.i0[2]) return bspline_contract(T.v, A, B, C);
+  return bspline_eval<false>(f, s0, s1, A, B, C, nullptr);
+}
+
+// interp_u_mac<false> with component A's taps held (the same values).  HOLD
+// for the two other components: 0 gathered, 1 their taps loaded into T1, T2
+// (and used), 2 read from T1, T2 where they are the point's taps.
+template <int A, int HOLD = 0>
 EDGE_DEV void interp_u_mac_own(const Grid& g, const Mac& U, const float p[3], const MacComp<A>& M,
-                               const OwnTaps<A>& W, float out[3]) {
+                               const OwnTaps<A>& W, float out[3], Taps27* T1 = nullptr, Taps27* T2 = nullptr) {
   float q0 = p[0] * g.idx, q1 = p[1] * g.idx, q2 = p[2] * g.idx;
   Axis3 ow0 = bspline_axis(q0, g.hi_f[0]), ot0 = bspline_axis(q0 - 0.5f, g.hi_c[0]);
   Axis3 ow1 = bspline_axis(q1, g.hi_f[1]), ot1 = bspline_axis(q1 - 0.5f, g.hi_c[1]);
   Axis3 ow2 = bspline_axis(q2, g.hi_f[2]), ot2 = bspline_axis(q2 - 0.5f, g.hi_c[2]);
   const uint32_t nx = g.nx, ny = g.ny, nz = g.nz;
-  if (A == 0) out[0] = own_eval<A>(M, W, ow0, ot1, ot2);
-  else out[0] = bspline_eval<false>(make_rsrc(U.ux, 4u * (nx + 1) * ny * nz), ny * nz, nz, ow0, ot1, ot2, nullptr);
-  if (A == 1) out[1] = own_eval<A>(M, W, ot0, ow1, ot2);
-  else out[1] = bspline_eval<false>(make_rsrc(U.uy, 4u * nx * (ny + 1) * nz), (ny + 1) * nz, nz, ot0, ow1, ot2,
-                                    nullptr);
-  if (A == 2) out[2] = own_eval<A>(M, W, ot0, ot1, ow2);
-  else out[2] = bspline_eval<false>(make_rsrc(U.uz, 4u * nx * ny * (nz + 1)), ny * (nz + 1), nz + 1, ot0, ot1, ow2,
-                                    nullptr);
+  const Axis3* ax[3][3] = {{&ow0, &ot1, &ot2}, {&ot0, &ow1, &ot2}, {&ot0, &ot1, &ow2}};
+  const Rsrc f[3] = {make_rsrc(U.ux, 4u * (nx + 1) * ny * nz), make_rsrc(U.uy, 4u * nx * (ny + 1) * nz),
+                     make_rsrc(U.uz, 4u * nx * ny * (nz + 1))};
+  const uint32_t s0[3] = {ny * nz, (ny + 1) * nz, ny * (nz + 1)}, s1[3] = {nz, nz, nz + 1};
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const Axis3 &X = *ax[m][0], &Y = *ax[m][1], &Z = *ax[m][2];
+    if (m == A) {
+      out[m] = own_eval<A>(M, W, X, Y, Z);
+    } else {
+      Taps27* T = m == (A + 1) % 3 ? T1 : T2;
+      if (HOLD == 1) {
+        load_taps(f[m], s0[m], s1[m], X, Y, Z, *T);
+        out[m] = bspline_contract(T->v, X, Y, Z);
+      } else if (HOLD == 2) {
+        out[m] = held_eval(f[m], s0[m], s1[m], *T, X, Y, Z);
+      } else {
+        out[m] = bspline_eval<false>(f[m], s0[m], s1[m], X, Y, Z, nullptr);
+      }
+    }
+  }
 }
 
 // interp_u_comp(g, U, p, A) with component A's taps held.

@@ -339,6 +339,9 @@ __global__ void __launch_bounds__(256) k_clamp_u(Grid g, CFaces U, CFaces U1, Fa
 #ifndef EDGE_SL_OWN
 #define EDGE_SL_OWN 1
 #endif
+#ifndef EDGE_SL_HOLD
+#define EDGE_SL_HOLD 1
+#endif
 template <int A>
 __device__ __forceinline__ void advect_face(const Grid& g, const Mac& U0, float* __restrict__ out, float dt, int i,
                                             int j, int k) {
@@ -350,6 +353,18 @@ __device__ __forceinline__ void advect_face(const Grid& g, const Mac& U0, float*
   interp_u_mac_own<A>(g, U0, p1, M, W, v1);
 #pragma unroll
   for (int d = 0; d < 3; ++d) q[d] = p1[d] - v1[d] * dt * 0.5f;
+#if EDGE_SL_HOLD
+  // the other two components: stage 2's taps held for stages 3 and 4 (the
+  // stage points move the same way from the face, so they usually share them)
+  Taps27 T1, T2;
+  interp_u_mac_own<A, 1>(g, U0, q, M, W, v2, &T1, &T2);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) q[d] = p1[d] - v2[d] * dt * 0.5f;
+  interp_u_mac_own<A, 2>(g, U0, q, M, W, v3, &T1, &T2);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) q[d] = p1[d] - v3[d] * dt;
+  interp_u_mac_own<A, 2>(g, U0, q, M, W, v4, &T1, &T2);
+#else
   interp_u_mac_own<A>(g, U0, q, M, W, v2);
 #pragma unroll
   for (int d = 0; d < 3; ++d) q[d] = p1[d] - v2[d] * dt * 0.5f;
@@ -357,6 +372,7 @@ __device__ __forceinline__ void advect_face(const Grid& g, const Mac& U0, float*
 #pragma unroll
   for (int d = 0; d < 3; ++d) q[d] = p1[d] - v3[d] * dt;
   interp_u_mac_own<A>(g, U0, q, M, W, v4);
+#endif
 #pragma unroll
   for (int d = 0; d < 3; ++d) q[d] = p1[d] - (v1[d] + 2.0f * v2[d] + 2.0f * v3[d] + v4[d]) / 6.0f * dt;
   *out = interp_u_comp_own<A>(g, q, M, W);   // the face's own component at the departure point
@@ -377,7 +393,15 @@ __device__ __forceinline__ void advect_face(const Grid& g, const Mac& U0, float*
 #endif
 }
 
-__global__ void __launch_bounds__(256) k_advect_sl(Grid g, Mac U0, Faces U1, float dt) {
+#ifndef EDGE_WPE_SL
+#define EDGE_WPE_SL 3   // 167 VGPRs, no spills; without the bound 171 (2 waves per SIMD), 0.454 ms instead of 0.356
+#endif
+#if EDGE_WPE_SL
+#define EDGE_SL_ATTR __attribute__((amdgpu_waves_per_eu(EDGE_WPE_SL, 8)))
+#else
+#define EDGE_SL_ATTR
+#endif
+__global__ void __launch_bounds__(256) EDGE_SL_ATTR k_advect_sl(Grid g, Mac U0, Faces U1, float dt) {
   int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   int a, i, j, k;
   int64_t l, nf;
