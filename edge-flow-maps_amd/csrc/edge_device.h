@@ -416,9 +416,12 @@ EDGE_DEV float interp_u_comp_own(const Grid& g, const float p[3], const MacComp<
 // ---------------------------------------------------------------------------
 // Tricubic Hermite over a packed table.
 //
-// Table layout: for component c, cell q: 8 floats at tab[(c*N + q)*8]:
-//   {psi_c, T_c0, T_c1, T_c2, dT_c0 (yz), dT_c1 (xz), dT_c2 (xy), d2T_c (xyz)}
+// Table layout: per component c two planes of N 16-byte cells, the low plane
+// {psi_c, T_c0, T_c1, T_c2} at float4 index (2c)*N + q and the high plane
+// {dT_c0 (yz), dT_c1 (xz), dT_c2 (xy), d2T_c (xyz)} at (2c+1)*N + q
 // (sim/hermite.py:610-620 "Hermite" row layout; dT as in sim/hermite.py:500-510).
+// An evaluation that needs only the low coefficients (a face centre) reads
+// only the low plane's cache lines.
 // ---------------------------------------------------------------------------
 struct HAxis {
   int i0;          // corner base (cells i0, i0+1)
@@ -469,7 +472,8 @@ EDGE_DEV HPoint hermite_point(const Grid& g, const float p[3]) {
 template <bool GRAD>
 EDGE_DEV float hermite_eval(const Grid& g, Rsrc tab, const HPoint& P, float gr[3]) {
   const uint32_t s1 = (uint32_t)g.nz, s0 = (uint32_t)g.ny * s1;
-  const uint32_t vb = 32u * ((uint32_t)P.X.i0 * s0 + (uint32_t)P.Y.i0 * s1 + (uint32_t)P.Z.i0);
+  const uint32_t vb = 16u * ((uint32_t)P.X.i0 * s0 + (uint32_t)P.Y.i0 * s1 + (uint32_t)P.Z.i0);
+  const uint32_t hib = 16u * (uint32_t)ncell(g);   // the high plane
   // per-axis weights of the combined index (corner, basis): [2 * corner + basis]
   const float Zw[4] = {P.Z.F[0], P.Z.G[0], P.Z.F[1], P.Z.G[1]};
   const float Yw[4] = {P.Y.F[0], P.Y.G[0], P.Y.F[1], P.Y.G[1]};
@@ -484,10 +488,10 @@ EDGE_DEV float hermite_eval(const Grid& g, Rsrc tab, const HPoint& P, float gr[3
     float c[2][2][2][2][2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const uint32_t so = 32u * (i * s0 + j * s1);
+      const uint32_t so = 16u * (i * s0 + j * s1);
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        const float4 lo = bload4(tab, vb + 32u * k, so), hi = bload4(tab, vb + 32u * k + 16u, so);
+        const float4 lo = bload4(tab, vb + 16u * k, so), hi = bload4(tab, vb + 16u * k, so + hib);
         c[j][k][0][0][0] = lo.x;
         c[j][k][1][0][0] = lo.y;
         c[j][k][0][1][0] = lo.z;
@@ -555,7 +559,8 @@ EDGE_DEV float hermite_eval_dir(const Grid& g, Rsrc tab, const HPoint& P, float&
   constexpr int A1 = D == 2 ? 0 : 2;   // contracted first
   constexpr int A2 = D == 1 ? 0 : 1;   // contracted second
   const uint32_t s1 = (uint32_t)g.nz, s0 = (uint32_t)g.ny * s1;
-  const uint32_t vb = 32u * ((uint32_t)P.X.i0 * s0 + (uint32_t)P.Y.i0 * s1 + (uint32_t)P.Z.i0);
+  const uint32_t vb = 16u * ((uint32_t)P.X.i0 * s0 + (uint32_t)P.Y.i0 * s1 + (uint32_t)P.Z.i0);
+  const uint32_t hib = 16u * (uint32_t)ncell(g);   // the high plane
   const HAxis* ax[3] = {&P.X, &P.Y, &P.Z};
   const bool z1 = ax[A1]->F[1] == 0.f && ax[A1]->G[1] == 0.f;  // corner 1 of A1 weighs nothing
   const bool z2 = ax[A2]->F[1] == 0.f && ax[A2]->G[1] == 0.f;
@@ -564,7 +569,7 @@ EDGE_DEV float hermite_eval_dir(const Grid& g, Rsrc tab, const HPoint& P, float&
   for (int i = 0; i < 2; ++i) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const uint32_t so = 32u * (i * s0 + j * s1);
+      const uint32_t so = 16u * (i * s0 + j * s1);
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const int cr[3] = {i, j, k};
@@ -576,9 +581,9 @@ EDGE_DEV float hermite_eval_dir(const Grid& g, Rsrc tab, const HPoint& P, float&
         // point sits on the corner's A1 and A2 coordinates) they add +0, and
         // their 16 bytes are not fetched (the same value, half the bytes)
         const bool hi_needed = w1g != 0.f || w2g != 0.f;
-        const float4 lo = bload4(tab, vb + 32u * k, so);
+        const float4 lo = bload4(tab, vb + 16u * k, so);
         float4 hi = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (hi_needed) hi = bload4(tab, vb + 32u * k + 16u, so);
+        if (hi_needed) hi = bload4(tab, vb + 16u * k, so + hib);
         float c[2][2][2];
         c[0][0][0] = lo.x;
         c[1][0][0] = lo.y;

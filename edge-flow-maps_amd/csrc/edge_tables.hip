@@ -57,8 +57,8 @@ __global__ void __launch_bounds__(256) k_pack_table(Grid g, MapIn m, float4* __r
     hi.y = m.dT[c][N + q];
     hi.z = m.dT[c][2 * N + q];
     hi.w = (mode == 0) ? fd_xyz(m.dT[c], g, i, j, k) : m.d2T[c][q];
-    tab[2 * (c * N + q)] = lo;
-    tab[2 * (c * N + q) + 1] = hi;
+    tab[2 * c * N + q] = lo;          // the low plane of component c
+    tab[(2 * c + 1) * N + q] = hi;    // the high plane
   }
 }
 

@@ -1,8 +1,10 @@
 """Hermite tables of backward maps, cached by storage and version.
 
 A table packs, per component c and cell, the 8 coefficients the tricubic
-Hermite needs (psi_c, T_c, dT_c, d2T_c) into 32 contiguous bytes, so one map
-evaluation reads two 16-byte words per corner instead of eight planes.
+Hermite needs into two 16-byte words, (psi_c, T_c) in a low plane and
+(dT_c, d2T_c) in a high plane, so one map evaluation reads two 16-byte words
+per corner instead of eight planes, and one that needs only psi and T (a face
+centre) reads only the low plane.
 Three kinds correspond to the reference's three Hermite variants:
   "garm"  (sim/hermite.py:298-456)  dT stored, d2T derived from dT
   "full"  (sim/hermite.py:459-607)  dT and d2T stored

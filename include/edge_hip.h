@@ -36,7 +36,8 @@ int edge_stream_sync(void* stream);
 
 /* ---- Hermite tables and finite differences ------------------------------ */
 /* Packs the per-cell Hermite coefficients of a map (3*nx*ny*nz*8 floats,
- * 16-byte aligned).  kind 0: garm, d2T derived from dT
+ * 16-byte aligned; per component a plane of {psi_c, T_c} and one of
+ * {dT_c, d2T_c}, 16 bytes per cell each).  kind 0: garm, d2T derived from dT
  * (sim/hermite.py:298-456 + 1258-1287); kind 1: full, d2T given
  * (sim/hermite.py:459-607).  d2T* may be NULL for kind 0. */
 int edge_pack_hermite_table(int kind, const float* psi, const float* Tx, const float* Ty, const float* Tz,
