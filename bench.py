@@ -97,7 +97,9 @@ TRAFFIC_KERNELS = {
     "k_advect_sl": (8.0 * 12664832, "face: u in 4 + out 4 B"),
     "k_advance_faces": (52.0 * 12664832, "face: phi, F read and written 48 + u 4 B"),
     "k_map_smoke": (68.0 * 4194304, "cell: faces of phi 36 + smoke in 16 + out 16 B"),
-    "k_split_psi": (96.0 * 4194304 + 24.0 * 12664832, "cell: the garm table 96 B read once; face: phi, F out 24 B"),
+    "k_split_psi": (48.0 * 4194304 + 24.0 * 12664832,
+                    "cell: the garm table's {psi, T} planes 48 B read once (face centres weigh no dT, d2T); "
+                    "face: phi, F out 24 B"),
     "k_smooth8": (6.0 * 4194304, "cell: r 2 + z read 2 + z written 2 B (one half-sweep of the mixed projection's "
                                  "BF16 level 0)"),
 }
