@@ -260,6 +260,29 @@ def test_driver_maps_parity(orc32, orc64, dev):
     assert ops.max_speed(gu) == pytest.approx(orc32.max_speed(u), rel=1e-6)
 
 
+@pytest.mark.parametrize("disp_cells", [0.25, 0.6, 3.0])
+def test_advect_sl_displacements(orc32, dev, disp_cells):
+    """k_advect_sl reads the face's own velocity component from 27 taps held in
+    registers (and the other two components' RK4 stage-2 taps for stages 3
+    and 4), gathering again wherever a stage point's taps differ.  The largest
+    displacement sweeps from a quarter cell (the held taps serve every stage),
+    through 0.6 cells (some stages leave them) to 3 cells (nearly every stage
+    leaves them, and the walls clamp): every case within 1e-5 of scale of the
+    FP32 oracle's semi-Lagrangian step (sim/hermite.py:1578-1621)."""
+    from edge_hip import ops
+    n, dx = (32, 24, 20), 1.0 / 24
+    u = smooth_velocity(*n, dx, seed=21)
+    gu = [T_(c, dev) for c in u]
+    umax = max(float(np.abs(c).max()) for c in u)
+    dt = disp_cells * dx / umax
+    sl = ops.advect_u_sl(gu, dx, dt)
+    rsl = orc32.advect_u_sl(u, dx, dt)
+    for a in range(3):
+        e = scale_err(N_(sl[a]), rsl[a])
+        print(f"displacement {disp_cells} cells, u_{'xyz'[a]}: {e:.2e}")
+        assert e < 1e-5
+
+
 def test_prefetched_max_speed(orc32, dev):
     """prefetch_max_speed (launched at the end of a step) gives calc_max_speed's
     value; a field written since (every wrapper bumps its version) is measured again."""
