@@ -52,9 +52,9 @@ FACE_MARCH_FLOP_PER_FACE = 2860.0
 # the reference formulation's count (64 weight products per Hermite evaluation, 27 per
 # B-spline evaluation), the algorithmic work the roofline record is priced with
 SCHEME_FLOP_PER_CELL_ALG = {"edge": 33600.0, "ed8": 21400.0, "ed4": 14500.0, "ge": 6800.0}
-SCHEME_BYTES_PER_CELL = {"edge": 180.0, "ed8": 204.0, "ed4": 108.0, "ge": 324.0}
+SCHEME_BYTES_PER_CELL = {"edge": 228.0, "ed8": 204.0, "ed4": 108.0, "ge": 324.0}
 MG_BYTES_PER_CELL = {"plain": 145.0, "mixed": 98.0, "dense": 145.0}  # one CG iteration
-MARCH_BYTES_PER_CELL = 180.0       # u 12 + old map 84 + new map 84 (compulsory FP32)
+MARCH_BYTES_PER_CELL = 228.0       # u 12 + old map 84 + new map 84 + its table's {psi, T} planes 48 (FP32)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
@@ -92,7 +92,7 @@ def read_pmc(d, counter):
 # kernels whose HBM traffic the counter runs record: short name -> algorithmic bytes per
 # launch at 256x128x128 (4,194,304 cells, 12,664,832 faces) and what they are
 TRAFFIC_KERNELS = {
-    "k_march_edge": (180.0 * 4194304, "cell: u 12 + old map 84 + new map 84 B"),
+    "k_march_edge": (228.0 * 4194304, "cell: u 12 + old map 84 + new map 84 + the new table's {psi, T} planes 48 B"),
     "k_map_u": (32.0 * 12664832, "face: u0 12 (gathered) + phi 12 + F 4 + out 4 B"),
     "k_advect_sl": (8.0 * 12664832, "face: u in 4 + out 4 B"),
     "k_advance_faces": (52.0 * 12664832, "face: phi, F read and written 48 + u 4 B"),

@@ -31,8 +31,13 @@ __device__ __forceinline__ void write_map_row(const MapOut& o, int64_t N, int64_
 // (-,-,-), (+,+,-), (-,+,+), (+,-,+) of sim/hermite.py:1446-1449.  Each point is
 // RK4-marched with its Jacobian (-dt), composed with the old map, and the four
 // eps-points give dT by calc_ed4_garm_func (sim/hermite.py:1049-1075).
+// With tabn non-null the low planes {psi_c, T_c} of the new map's garm table
+// are written too (the words k_pack_table would write from the stored map);
+// the high planes {dT_c, d2T_c} need the neighbours' dT and are k_pack_hi's
+// (edge_tables.hip).
 __global__ void __launch_bounds__(256) EDGE_MARCH_ATTR k_march_edge(Grid g, Mac U, const float4* __restrict__ tab,
-                                                    const float* __restrict__ X, MapOut o, float dt, float eps) {
+                                                    const float* __restrict__ X, MapOut o, float dt, float eps,
+                                                    float4* __restrict__ tabn) {
   const int64_t N = ncell(g);
   const int plane = g.ny * g.nz;
   const int64_t q =
@@ -56,6 +61,10 @@ __global__ void __launch_bounds__(256) EDGE_MARCH_ATTR k_march_edge(Grid g, Mac 
     compose(g, tab, p, T);
     if (pt == 0) {
       write_map_row(o, N, q, p, T);
+      if (tabn) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tabn[2 * c * N + q] = make_float4(p[c], T[c][0], T[c][1], T[c][2]);
+      }
     } else {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -549,7 +558,25 @@ int edge_march_edge(const float* ux, const float* uy, const float* uz, const flo
   Grid g = make_grid(nx, ny, nz, dx);
   MapOut o{psi, {Tx, Ty, Tz}, {dTx, dTy, dTz}, {nullptr, nullptr, nullptr}};
   hipLaunchKernelGGL(k_march_edge, dim3(blocks_for(ncell(g), 256)), dim3(256), 0, (hipStream_t)stream, g,
-                     Mac{ux, uy, uz}, (const float4*)tab_old, X, o, dt, eps);
+                     Mac{ux, uy, uz}, (const float4*)tab_old, X, o, dt, eps, (float4*)nullptr);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+int edge_march_edge_table(const float* ux, const float* uy, const float* uz, const float* tab_old, const float* X,
+                          float* psi, float* Tx, float* Ty, float* Tz, float* dTx, float* dTy, float* dTz,
+                          float* tab_new, int nx, int ny, int nz, float dx, float dt, float eps, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_march_edge_table: bad dims");
+  EDGE_MAC_OK(ux, uy, uz);
+  EDGE_REQUIRE(tab_old && X && psi && Tx && Ty && Tz && dTx && dTy && dTz && tab_new,
+               "edge_march_edge_table: null pointer");
+  EDGE_REQUIRE(tab_new != tab_old, "edge_march_edge_table: the new table must not be the old one");
+  EDGE_REQUIRE(eps != 0.0f, "edge_march_edge_table: eps must be non-zero");
+  Grid g = make_grid(nx, ny, nz, dx);
+  MapOut o{psi, {Tx, Ty, Tz}, {dTx, dTy, dTz}, {nullptr, nullptr, nullptr}};
+  hipLaunchKernelGGL(k_march_edge, dim3(blocks_for(ncell(g), 256)), dim3(256), 0, (hipStream_t)stream, g,
+                     Mac{ux, uy, uz}, (const float4*)tab_old, X, o, dt, eps, (float4*)tab_new);
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }

@@ -62,6 +62,20 @@ __global__ void __launch_bounds__(256) k_pack_table(Grid g, MapIn m, float4* __r
   }
 }
 
+// The high planes {dT_c, d2T_c} of a garm table whose low planes are written
+// elsewhere (by k_march_edge with a new-table output): the words k_pack_table
+// mode 0 stores there, from the same dT planes.
+__global__ void __launch_bounds__(256) k_pack_hi(Grid g, MapIn m, float4* __restrict__ tab) {
+  int64_t N = ncell(g);
+  int64_t q = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  int i, j, k;
+  cell_ijk(g, q, i, j, k);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    tab[(2 * c + 1) * N + q] = make_float4(m.dT[c][q], m.dT[c][N + q], m.dT[c][2 * N + q], fd_xyz(m.dT[c], g, i, j, k));
+}
+
 // sim/hermite.py:1180-1287: dT (want_dT) and d2T by central differences.
 // Writes dT_c planes (3 per component) when want_dT, else only d2T.
 __global__ void __launch_bounds__(256) k_fd_dT(Grid g, MapIn m, float* __restrict__ dTx, float* __restrict__ dTy,
@@ -119,6 +133,21 @@ int edge_pack_hermite_table(int kind, const float* psi, const float* Tx, const f
   int64_t N = ncell(g);
   hipLaunchKernelGGL(k_pack_table, dim3(blocks_for(N, 256)), dim3(256), 0, (hipStream_t)stream, g, m,
                      (float4*)tab, kind);
+  EDGE_CHECK_LAUNCH();
+  return EDGE_OK;
+}
+
+// The high planes of a garm table (dT and calc_fd_3_from_grad_grad of it,
+// sim/hermite.py:1258-1287) whose low planes edge_march_edge_table wrote.
+int edge_pack_table_hi(const float* dTx, const float* dTy, const float* dTz, float* tab, int nx, int ny, int nz,
+                       float dx, void* stream) {
+  clear_error();
+  EDGE_REQUIRE(dims_ok(nx, ny, nz), "edge_pack_table_hi: bad dims");
+  EDGE_REQUIRE(dTx && dTy && dTz && tab, "edge_pack_table_hi: null pointer");
+  Grid g = make_grid(nx, ny, nz, dx);
+  int64_t N = ncell(g);
+  MapIn m{nullptr, {nullptr, nullptr, nullptr}, {dTx, dTy, dTz}, {nullptr, nullptr, nullptr}};
+  hipLaunchKernelGGL(k_pack_hi, dim3(blocks_for(N, 256)), dim3(256), 0, (hipStream_t)stream, g, m, (float4*)tab);
   EDGE_CHECK_LAUNCH();
   return EDGE_OK;
 }

@@ -22,8 +22,10 @@ SIGNATURES = {
     # tables / FD
     "edge_pack_hermite_table": "ipppppppppppiiifp",
     "edge_calc_fd3": "ipppppppppiiifp",
+    "edge_pack_table_hi": "ppppiiifp",
     # marches
     "edge_march_edge": "ppppppppppppiiifffp",
+    "edge_march_edge_table": "pppppppppppppiiifffp",
     "edge_march_ed8": "pppppppppppppppiiifffp",
     "edge_march_ge": "pppppppppiiiffp",
     "edge_march_ed4": "pppppppppiiifffp",

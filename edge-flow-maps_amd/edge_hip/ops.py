@@ -101,6 +101,21 @@ def march_edge(u, tab_old, X, psi, T, dT, dx, dt, eps):
         touch_ptr(t.data_ptr())
 
 
+def march_edge_table(u, tab_old, X, psi, T, dT, dx, dt, eps):
+    """march_edge that also returns the new map's garm table (the march writes
+    its low planes, edge_pack_table_hi the high ones)."""
+    nx, ny, nz = psi.shape[1:]
+    tab_new = torch.empty_like(tab_old)
+    s = stream_handle()
+    _lib.call("edge_march_edge_table", _p(u[0]), _p(u[1]), _p(u[2]), _p(tab_old), _p(X), _p(psi), _p(T[0]),
+              _p(T[1]), _p(T[2]), _p(dT[0]), _p(dT[1]), _p(dT[2]), _p(tab_new), nx, ny, nz, float(dx), float(dt),
+              float(eps), s)
+    _lib.call("edge_pack_table_hi", _p(dT[0]), _p(dT[1]), _p(dT[2]), _p(tab_new), nx, ny, nz, float(dx), s)
+    for t in (psi, T, dT):
+        touch_ptr(t.data_ptr())
+    return tab_new
+
+
 def advance_faces(u, phi, F, dx, dt):
     nx, ny, nz = _cells(u)
     _lib.call("edge_advance_map_faces", _p(u[0]), _p(u[1]), _p(u[2]), *[_p(t) for t in phi], *[_p(t) for t in F],

@@ -17,6 +17,7 @@ from .field import as_ptr, stream_handle, version_of
 
 _KIND = {"garm": 0, "full": 1}
 _cache = {}
+_spare = {}   # per cache key: a second table buffer that a march writes the next map's table into
 _MAX_ENTRIES = 6
 
 
@@ -60,9 +61,42 @@ def hermite_table(kind, psi, T, dT=None, d2T=None, dx=None):
         _lib.call("edge_pack_hermite_table", k, as_ptr(psi), *[as_ptr(t) for t in T], *[as_ptr(t) for t in dT], *d2,
                   tab.data_ptr(), nx, ny, nz, float(dx), s)
     if len(_cache) >= _MAX_ENTRIES and key not in _cache:
-        _cache.pop(next(iter(_cache)))
+        evicted = next(iter(_cache))
+        _cache.pop(evicted)
+        _spare.pop(evicted, None)
     _cache[key] = (ver, tab)
     return tab
+
+
+def spare_table(kind, psi, T, dT=None, d2T=None, dx=None):
+    """A table buffer for the next contents of the map psi (never the one
+    hermite_table returns for its current contents), to be filled by a march
+    and then registered with install_table."""
+    fields = [psi, *T] + (list(dT) if dT is not None else []) + (list(d2T) if d2T is not None else [])
+    key, _ = _key_and_version(kind, fields, dx)
+    tab = _spare.get(key)
+    cur = _cache.get(key)
+    if tab is None or (cur is not None and cur[1] is tab):
+        tab = _spare[key] = _alloc(psi)
+    return tab
+
+
+def install_table(kind, tab, psi, T, dT=None, d2T=None, dx=None):
+    """Register `tab` as the table of the map's current contents (after the
+    march that wrote both has touched the fields); the table it replaces
+    becomes the spare buffer of the next march."""
+    fields = [psi, *T] + (list(dT) if dT is not None else []) + (list(d2T) if d2T is not None else [])
+    key, ver = _key_and_version(kind, fields, dx)
+    old = _cache.get(key)
+    if len(_cache) >= _MAX_ENTRIES and key not in _cache:
+        evicted = next(iter(_cache))
+        _cache.pop(evicted)
+        _spare.pop(evicted, None)
+    _cache[key] = (ver, tab)
+    if old is not None and old[1] is not tab:
+        _spare[key] = old[1]
+    else:
+        _spare.pop(key, None)
 
 
 _fd_bufs = {}
@@ -81,4 +115,5 @@ def _fd_scratch(psi):
 
 def clear():
     _cache.clear()
+    _spare.clear()
     _fd_bufs.clear()

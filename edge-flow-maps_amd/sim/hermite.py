@@ -19,7 +19,7 @@ import ctypes
 
 from edge_hip import _lib, timing
 from edge_hip.field import Field, as_ptr, stream_handle, touch
-from edge_hip.tables import hermite_table
+from edge_hip.tables import hermite_table, install_table, spare_table
 
 __all__ = [
     "reset_to_identity_1", "reset_to_identity_2", "reset_to_identity_3", "reset_to_identity_ed8",
@@ -267,12 +267,20 @@ def advect_psi_garm_ed4_fusion_unroll(psi_tmp, T_x_tmp, T_y_tmp, T_z_tmp, dT_x_t
     """sim/hermite.py:1426-1521: the EDGE march, one fused kernel, new map written in place."""
     nx, ny, nz = _dims(psi)
     _check_mac(u_x, u_y, u_z, nx, ny, nz)
-    tab = hermite_table("garm", psi, (T_x, T_y, T_z), (dT_x, dT_y, dT_z), None, dx)
+    T, dT = (T_x, T_y, T_z), (dT_x, dT_y, dT_z)
+    tab = hermite_table("garm", psi, T, dT, None, dx)
+    # the march also writes the low planes {psi, T} of the new map's table and
+    # the high planes {dT, d2T} are packed from the new dT, so the map's next use
+    # needs no pass over the stored psi and T
+    tab_new = spare_table("garm", psi, T, dT, None, dx)
     with timing.span("k_march_edge"):
-        _lib.call("edge_march_edge", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X), as_ptr(psi),
-                  as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), as_ptr(dT_x), as_ptr(dT_y), as_ptr(dT_z), nx, ny, nz,
-                  float(dx), float(dt), float(epsilon), stream_handle())
+        _lib.call("edge_march_edge_table", as_ptr(u_x), as_ptr(u_y), as_ptr(u_z), tab.data_ptr(), as_ptr(X),
+                  as_ptr(psi), as_ptr(T_x), as_ptr(T_y), as_ptr(T_z), as_ptr(dT_x), as_ptr(dT_y), as_ptr(dT_z),
+                  tab_new.data_ptr(), nx, ny, nz, float(dx), float(dt), float(epsilon), stream_handle())
+    _lib.call("edge_pack_table_hi", as_ptr(dT_x), as_ptr(dT_y), as_ptr(dT_z), tab_new.data_ptr(), nx, ny, nz,
+              float(dx), stream_handle())
     touch(psi, T_x, T_y, T_z, dT_x, dT_y, dT_z)
+    install_table("garm", tab_new, psi, T, dT, None, dx)
 
 
 def advect_psi_garm_ed4(psi_0, T_x_0, T_y_0, T_z_0, psi_1, T_x_1, T_y_1, T_z_1, psi_2, T_x_2, T_y_2, T_z_2, psi_3,

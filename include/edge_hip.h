@@ -49,6 +49,11 @@ int edge_pack_hermite_table(int kind, const float* psi, const float* Tx, const f
 int edge_calc_fd3(int want_dT, const float* Tx, const float* Ty, const float* Tz, float* dTx, float* dTy,
                   float* dTz, float* d2Tx, float* d2Ty, float* d2Tz, int nx, int ny, int nz, float dx,
                   void* stream);
+/* The high planes {dT_c, d2T_c} of a garm table, d2T by calc_fd_3_from_grad_grad
+ * (sim/hermite.py:1258-1287), whose low planes edge_march_edge_table wrote;
+ * after it the table equals edge_pack_hermite_table kind 0 of the same map. */
+int edge_pack_table_hi(const float* dTx, const float* dTy, const float* dTz, float* tab, int nx, int ny, int nz,
+                       float dx, void* stream);
 
 /* ---- backward-map marches ----------------------------------------------- */
 /* advect_psi_garm_ed4_fusion_unroll (sim/hermite.py:1426-1521): reads the old
@@ -57,6 +62,13 @@ int edge_calc_fd3(int want_dT, const float* Tx, const float* Ty, const float* Tz
 int edge_march_edge(const float* ux, const float* uy, const float* uz, const float* tab_old, const float* X,
                     float* psi, float* Tx, float* Ty, float* Tz, float* dTx, float* dTy, float* dTz, int nx, int ny,
                     int nz, float dx, float dt, float eps, void* stream);
+/* edge_march_edge that also writes the low planes {psi_c, T_c} of the new
+ * map's garm table into tab_new (which must not be tab_old); edge_pack_table_hi
+ * adds the high planes: the table the map's next use reads (sim/hermite.py:
+ * 298-456) without a pass over the stored psi and T. */
+int edge_march_edge_table(const float* ux, const float* uy, const float* uz, const float* tab_old, const float* X,
+                          float* psi, float* Tx, float* Ty, float* Tz, float* dTx, float* dTy, float* dTz,
+                          float* tab_new, int nx, int ny, int nz, float dx, float dt, float eps, void* stream);
 /* advect_ps_hermite_ed8 (sim/hermite.py:1290-1338), fused; tab_old is a full table. */
 int edge_march_ed8(const float* ux, const float* uy, const float* uz, const float* tab_old, const float* X,
                    float* psi, float* Tx, float* Ty, float* Tz, float* dTx, float* dTy, float* dTz, float* d2Tx,

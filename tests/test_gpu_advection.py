@@ -149,6 +149,33 @@ def test_edge_march_parity(orc32, orc64, dev):
     assert e_gpu <= 4 * e_orc + 1e-3 * np.max(np.abs(r64[2])), (e_gpu, e_orc)
 
 
+def test_edge_march_writes_the_packed_table(orc64, dev):
+    """The march's new-table output (plus the d2T fill) is bit for bit the table
+    edge_pack_hermite_table builds from the map the march stores, and the stored
+    map is bit for bit the plain march's (32x24x20: a partial last block)."""
+    from edge_hip import ops
+    n, dx = (32, 24, 20), 1.0 / 24
+    u, X, psi, T, dT = realistic_map(orc64, n, dx, steps=2, seed=11)
+    dt, eps = 0.5 * dx, 0.025 * dx
+    tab = ops.pack_table("garm", T_(psi, dev), T_(T, dev), T_(dT, dev), None, dx)
+    ug, Xg = [T_(c, dev) for c in u], T_(X, dev)
+    out = []
+    for with_table in (False, True):
+        gp, gT, gdT = torch.empty((3,) + n, device=dev), torch.empty((3, 3) + n, device=dev), \
+            torch.empty((3, 3) + n, device=dev)
+        if with_table:
+            tab_new = ops.march_edge_table(ug, tab, Xg, gp, gT, gdT, dx, dt, eps)
+        else:
+            ops.march_edge(ug, tab, Xg, gp, gT, gdT, dx, dt, eps)
+        out.append((gp, gT, gdT))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+    packed = ops.pack_table("garm", *out[1], None, dx)
+    assert torch.equal(tab_new, packed)
+    # the old table is left as it was (the march reads it while writing the new one)
+    assert torch.equal(tab, ops.pack_table("garm", T_(psi, dev), T_(T, dev), T_(dT, dev), None, dx))
+
+
 def test_driver_maps_parity(orc32, orc64, dev):
     from edge_hip import ops
     n, dx = (32, 24, 20), 1.0 / 24
