@@ -73,7 +73,8 @@ __global__ void __launch_bounds__(256) k_pack_hi(Grid g, MapIn m, float4* __rest
   cell_ijk(g, q, i, j, k);
 #pragma unroll
   for (int c = 0; c < 3; ++c)
-    tab[(2 * c + 1) * N + q] = make_float4(m.dT[c][q], m.dT[c][N + q], m.dT[c][2 * N + q], fd_xyz(m.dT[c], g, i, j, k));
+    tab[(2 * c + 1) * N + q] =
+        make_float4(m.dT[c][q], m.dT[c][N + q], m.dT[c][2 * N + q], fd_xyz(m.dT[c], g, i, j, k));
 }
 
 // sim/hermite.py:1180-1287: dT (want_dT) and d2T by central differences.
